@@ -1,0 +1,54 @@
+/*
+ * surfmol_host.h -- host-side set-up of the relaxation path (part of
+ * libsurfmol_relax.so).
+ *
+ * C++ mirror of the reference's Rust set-up API, used by callers that do not
+ * have the Rust crates: topology enumeration, neighbour tables, UFF typing,
+ * UFF parameter assignment and REQH lookup.  The result is an smr_topology
+ * (surfmol_relax.h) filled exactly as the reference's Uff/NonBondedFF fields
+ * would be.  A Rust caller keeps its own crates and fills smr_topology from
+ * them directly.
+ */
+#ifndef SURFMOL_HOST_H
+#define SURFMOL_HOST_H
+
+#include "surfmol_relax.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMH_OK 0
+#define SMH_ERR 1
+
+typedef struct smh_molecule smh_molecule;
+
+const char *smh_last_error(void);
+
+/* Build one molecule.  param_dir holds ElementTypes.dat and AtomTypes.dat
+ * (the format of the reference's data/ElementTypes.dat and data/AtomTypes.dat,
+ * moltopo/src/params.rs:130-258).
+ * Bonds: explicit (nbonds > 0, [2*nbonds]) or, when nbonds == 0, every pair
+ * closer than rcut_bond (build_bonds_by_cutoff, moltopo/src/topology.rs:101).
+ * Then angles/dihedrals/inversions (topology.rs:113-182), neighbour tables
+ * (molecular.rs:34-78), UFF types (assign_uff.rs:8-118), parameters
+ * (molff/src/uff.rs:699-887), hneigh slots (uff.rs:250-315) and REQH
+ * (params.rs:595-604).  charges (may be NULL) overrides REQH.Q per atom.
+ * Fails (no abort) on more than 4 bonds per atom. */
+int smh_molecule_build(const char *param_dir, int natoms, const char *const *elems, const double *apos,
+                       const double *charges, int nbonds, const int *bonds, double rcut_bond,
+                       smh_molecule **out);
+/* (natoms, nbonds, nangles, ndih, ninv) */
+int smh_molecule_counts(const smh_molecule *m, int *counts5);
+/* Fill an smr_topology whose pointers stay valid while m lives. */
+int smh_molecule_topology(const smh_molecule *m, smr_topology *t);
+/* Bond list [2*nbonds]. */
+int smh_molecule_bonds(const smh_molecule *m, int *bonds);
+/* Space-separated UFF type names. */
+const char *smh_molecule_types(const smh_molecule *m);
+void smh_molecule_free(smh_molecule *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SURFMOL_HOST_H */

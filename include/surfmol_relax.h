@@ -1,0 +1,234 @@
+/*
+ * surfmol_relax.h -- C ABI of the MI355X (gfx950) multi-system relaxation
+ * library libsurfmol_relax.so.
+ *
+ * This is the drop-in boundary for SurfMol's relaxation path: FIRE / damped
+ * MD driving UFF bonded forces, intra-molecular LJ + Coulomb pairs and the
+ * tricubic B-spline GridFF surface.  Every entry point names the reference
+ * interface it replaces (paths relative to the reference repository root).
+ *
+ * Conventions
+ *  - Return value: SMR_OK (0) or an SMR_ERR_* code; smr_last_error() returns
+ *    a thread-local message with the offending values.  Nothing aborts
+ *    across the ABI (the reference's ffi_uff.rs panics instead:
+ *    crates/libs/molff/src/ffi_uff.rs:22-37).
+ *  - Ownership: the caller owns every host array passed in or out (plain
+ *    pointers + sizes, C-contiguous, double unless stated); the library owns
+ *    device state behind opaque handles.
+ *  - Threading: a context, and everything created from it, is used from one
+ *    thread at a time.  The serving loop (smr_server_*) owns an internal
+ *    worker thread; its entry points may be called from any thread.
+ *  - Units: eV, Angstrom, elementary charge, amu (all masses 1), time unit
+ *    sqrt(amu*A^2/eV) = 10.1805 fs.
+ *  - Numerics: forces in FP32 with compensated (Kahan) per-atom sums;
+ *    energies, FIRE reductions and convergence tests in FP64; GridFF build
+ *    and B-spline fit in FP64.
+ */
+#ifndef SURFMOL_RELAX_H
+#define SURFMOL_RELAX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMR_OK 0
+#define SMR_ERR_INVALID 1     /* bad argument (message says which)            */
+#define SMR_ERR_HIP 2         /* HIP runtime error                            */
+#define SMR_ERR_NOMEM 3       /* device or host allocation failed             */
+#define SMR_ERR_UNSUPPORTED 4 /* valid request outside what is built          */
+#define SMR_ERR_STATE 5       /* call not valid in the current object state   */
+
+/* per-system status codes reported by relax / serve */
+#define SMR_SYS_RUNNING 0
+#define SMR_SYS_CONVERGED 1
+#define SMR_SYS_MAXSTEPS 2    /* step cap reached before Fmax < fconv          */
+#define SMR_SYS_NONFINITE 3   /* a force or position became NaN/Inf            */
+
+typedef struct smr_ctx smr_ctx;
+typedef struct smr_gridff smr_gridff;
+typedef struct smr_batch smr_batch;
+typedef struct smr_server smr_server;
+
+/* Per-thread message of the last failing call ("" if none). */
+const char *smr_last_error(void);
+/* Library version string. */
+const char *smr_version(void);
+
+/* ---------------------------------------------------------------- context */
+/* Replaces oclff::nvidia_proque + UffOcl::new
+ * (crates/libs/oclff/src/lib.rs:15-23, crates/libs/oclff/src/uff.rs:18-20). */
+int smr_ctx_create(int device, smr_ctx **out);
+int smr_ctx_destroy(smr_ctx *ctx);
+
+/* ----------------------------------------------------------------- GridFF */
+typedef struct {
+    int nx, ny, nz;          /* nodes; x,y periodic, z open                  */
+    double p0[3];            /* position of node (0,0,0)                      */
+    double step[3];          /* node spacing (cell = n*step)                  */
+    double alpha_morse;      /* GFFParams.y: Morse decay [1/A]                */
+    double rdamp;            /* GFFParams.x: Coulomb damping radius [A]       */
+    int keep_reference;      /* smr_gridff_build: keep the FP64 reference grid */
+} smr_grid_desc;
+
+/* Upload fitted B-spline coefficients (FP32, z-fastest, 4 channels
+ * Pauli/London/Coulomb/0 per node: index iz + nz*(iy + ny*ix)), the layout of
+ * GridFF.h pack_Bspline_d consumed by fe3d_pbc_comb
+ * (opencl/gridff_eval.cl:234-286).  Replaces the BsplinePLQ buffer argument
+ * of getNonBond_GridFF_Bspline (opencl/UFF.cl:1524-1541). */
+int smr_gridff_upload(smr_ctx *ctx, const smr_grid_desc *desc, const float *coefs, smr_gridff **out);
+
+/* Build the reference grid on the device in FP64 and fit the B-spline
+ * coefficients in FP64, then store them as FP32.  Reference grid:
+ * make_GridFF (opencl/gridff_build.cl:908-1062): node (ix,iy,iz) sits at
+ * p0 + (ix*Lx/nx, iy*Ly/ny, iz*Lz/nz) with the cell L = n*step, and takes
+ * the sum over substrate atoms and their images shifted by (jx*Lx, jy*Ly, 0),
+ * |jx|,|jy| <= npbc (the reference's fixed image set), of
+ *     Pauli  += sqrtE*exp(-2a(r-R)),  London += -2 sqrtE*exp(-a(r-R)),
+ *     Coul   += kC*Q/sqrt(r^2+rdamp^2).
+ * Fit: (Sx (x) Sy (x) Sz) C = E with the stencil [1/6,2/3,1/6], periodic in
+ * x,y and truncated in z (BsplineConv3D, opencl/gridff_build.cl:8-61).
+ * sub_reqs[4*i] = (R, sqrt(E), Q, unused). */
+int smr_gridff_build(smr_ctx *ctx, const smr_grid_desc *desc, int nsub, const double *sub_pos,
+                     const double *sub_reqs, int npbc, smr_gridff **out);
+/* Copy the FP32 coefficients back (nx*ny*nz*4 floats). */
+int smr_gridff_download(const smr_gridff *g, float *coefs);
+/* Copy the FP64 reference-grid values (before the fit) back (nx*ny*nz*4);
+ * available only for grids made by smr_gridff_build with keep_reference. */
+int smr_gridff_download_reference(const smr_gridff *g, double *values);
+int smr_gridff_destroy(smr_gridff *g);
+
+/* ----------------------------------------------------------- systems */
+/* One molecule's force-field tables: exactly the public SoA fields of the
+ * reference's molff::uff::Uff (crates/libs/molff/src/uff.rs:44-85), the
+ * neighbour tables of moltopo::molecular::Atoms (molecular.rs:7-13) and
+ * NonBondedFF.reqs (crates/libs/molff/src/nonbonded.rs:48).  Indices are
+ * 0-based and local to the molecule; *_ngs are hneigh slot indices
+ * (4*atom + slot) as produced by Uff::bake_*_neighs (uff.rs:250-315). */
+typedef struct {
+    int natoms, nbonds, nangles, ndih, ninv;
+    const int *neighs;        /* [4*natoms], -1 padded                        */
+    const int *neigh_bs;      /* [4*natoms], bond index per slot              */
+    const double *bon_params; /* [2*nbonds]  (k, l0)                          */
+    const int *ang_atoms;     /* [3*nangles] (i, j, k), j central             */
+    const int *ang_ngs;       /* [2*nangles]                                  */
+    const double *ang_params; /* [5*nangles] (k, c0, c1, c2, c3)              */
+    const int *dih_atoms;     /* [4*ndih]                                     */
+    const int *dih_ngs;       /* [3*ndih]                                     */
+    const double *dih_params; /* [3*ndih]    (V, d, n)                        */
+    const int *inv_atoms;     /* [4*ninv]    i central                        */
+    const int *inv_ngs;       /* [3*ninv]                                     */
+    const double *inv_params; /* [4*ninv]    (K, C0, C1, C2)                  */
+    const double *reqs;       /* [4*natoms]  (RvdW, sqrt(EvdW), Q, Hb)        */
+} smr_topology;
+
+typedef struct {
+    double rdamp;             /* Coulomb damping radius of the pair term [A]   */
+    int nonbonded;            /* 1: intra-molecular LJ+Coulomb pairs (getNonBond,
+                                 opencl/UFF.cl:1023-1204: 1-2 exclusions,
+                                 E0 = sqrtEi*sqrtEj); 0: off                   */
+    int surface;              /* 1: GridFF surface term; 0: off                */
+} smr_ff_options;
+
+/* Create a batch of nsys systems.  Systems reference one of ntopo
+ * topologies (sys_topo[s] in [0,ntopo)), so replicas share device tables
+ * (the per-replica/shared split of oclff::RRsp3Multi,
+ * crates/libs/oclff/src/rrsp3.rs:742-830).  apos holds the concatenated
+ * initial positions [sum_s natoms(s), 3].  grid may be NULL when
+ * opts->surface == 0.  Replaces the per-call buffer set-up of
+ * UffOcl::eval_bonds (crates/libs/oclff/src/uff.rs:30-58). */
+int smr_batch_create(smr_ctx *ctx, const smr_gridff *grid, int ntopo, const smr_topology *topos, int nsys,
+                     const int *sys_topo, const double *apos, const smr_ff_options *opts, smr_batch **out);
+int smr_batch_destroy(smr_batch *b);
+/* Total atoms in the batch; per-system atom offsets (nsys+1 ints). */
+int smr_batch_natoms(const smr_batch *b, int *total_atoms);
+int smr_batch_atom_offsets(const smr_batch *b, int *offsets);
+int smr_batch_set_positions(smr_batch *b, const double *apos);
+int smr_batch_get_positions(const smr_batch *b, double *apos);
+/* Per-step parity hook: forces [sum N,3] and per-system energy terms
+ * [nsys,6] = (bond, angle, dihedral, inversion, nonbonded, surface), with the
+ * reference's bookkeeping (bond and pair energies counted at both ends,
+ * angle energy without the c1*cos(theta) term: uff.rs:462,482-493,
+ * nonbonded.rs:249).  Replaces MolWorld::eval_forces
+ * (crates/libs/surfmol/src/mol_world.rs:60-86) and the force half of the
+ * OpenCL launch sequence (opencl/UFF.cl:141-880,1524-1718). */
+int smr_batch_eval_forces(smr_batch *b, double *forces, double *energies);
+
+typedef struct {
+    double dt0, dt_max;       /* FireState::new(dt0, dt_max) (molff/src/raff.rs:1057) */
+    double alpha0;            /* 0.1  */
+    int n_min;                /* 5    */
+    double f_inc, f_dec, f_alpha; /* 1.1, 0.5, 0.99 */
+    double flim;              /* <= 0: no clamp; > 0: clamp |F| used by the step (raff.rs:1093) */
+} smr_fire_params;
+
+typedef struct {
+    int steps;                /* force evaluations done                        */
+    int status;               /* SMR_SYS_*                                     */
+    double energy;            /* total energy at the last force evaluation     */
+    double fmax;              /* max_i |F_i| at the last force evaluation      */
+    double terms[6];          /* energy terms, as smr_batch_eval_forces        */
+} smr_relax_result;
+
+/* Batched relaxation: every system of the batch runs the FIRE loop of
+ * surfmol/tests/relax_pentacene_uff.rs:142-193 (FireState semantics,
+ * raff.rs:1040-1062) until max_i|F_i| < fconv or max_steps, all systems in
+ * one persistent kernel (one workgroup per system).  Replaces FireCore's
+ * multi-relax call over relax_multi.cl (opencl/relax_multi.cl:1-45,
+ * 1089-1413, 1916-2086).  results: [nsys]; positions are left on the device
+ * (smr_batch_get_positions). */
+int smr_relax_batched(smr_batch *b, const smr_fire_params *fire, int max_steps, double fconv,
+                      smr_relax_result *results);
+
+/* Plain method: relax one system of the batch with one launch sequence per
+ * FIRE step (bonds -> angles/dihedrals/inversions -> assemble + pairs +
+ * surface -> per-system (v.F, |v|^2, |F|^2, Fmax) reduction) and the FIRE
+ * decision taken on the host after reading the reductions back each step:
+ * the reference's per-step shape (opencl/UFF.cl:141-1011 kernels driven by a
+ * host loop reading cvf, opencl/relax_multi.cl:1384). */
+int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fire, int max_steps, double fconv,
+                    smr_relax_result *result);
+
+/* Launch shape and device time of the last fused launch on this batch:
+ * shape4 = (threads per workgroup, atoms per thread, workgroups, LDS bytes);
+ * kernel_ms = HIP-event time of that launch on the context's stream (only
+ * measured by smr_relax_batched).  No counterpart in the reference. */
+int smr_batch_last_launch(const smr_batch *b, int *shape4, float *kernel_ms);
+
+/* Damped MD of MolWorld::run_md (crates/libs/surfmol/src/mol_world.rs:141-164,
+ * moltopo/src/molecular.rs:118-137) for every system: clamp |F| to flim,
+ * v = (1-damping) v + F dt, x += v dt, v = 0 when sum v.F < 0, stop when
+ * sum|F|^2 < fconv^2.  steps[nsys] receives the steps taken. */
+int smr_run_md(smr_batch *b, int niter, double dt, double fconv, double flim, double damping, int *steps);
+
+/* ----------------------------------------------------------- serving */
+typedef struct {
+    int max_resident;         /* slot count (systems on the device at once)    */
+    int max_atoms;            /* largest request accepted                      */
+    double hbm_fraction;      /* admit while free HBM > (1-hbm_fraction)*total */
+    int chunk_steps;          /* FIRE steps per persistent launch              */
+    int max_steps;            /* step cap per request                          */
+    double fconv;             /* convergence threshold [eV/A]                  */
+    smr_fire_params fire;
+    smr_ff_options ff;
+} smr_server_config;
+
+/* Continuous-batching relax server: requests are admitted into free slots
+ * (and only while free HBM allows), relaxed by chunked persistent launches
+ * that overlap with later arrivals, and retired at convergence.  No
+ * counterpart in the reference (it has no serving loop). */
+int smr_server_create(smr_ctx *ctx, const smr_gridff *grid, const smr_server_config *cfg, smr_server **out);
+/* Submit one request (a topology + initial positions [natoms,3]); *id
+ * receives a request id.  Thread-safe. */
+int smr_server_submit(smr_server *s, const smr_topology *topo, const double *apos, long long *id);
+/* Poll a request: returns SMR_OK with *done = 1 and fills result / final
+ * positions ([natoms,3], may be NULL) when retired, *done = 0 otherwise. */
+int smr_server_poll(smr_server *s, long long id, int *done, smr_relax_result *result, double *apos_out,
+                    double *t_submit, double *t_done);
+/* Block until every submitted request has retired. */
+int smr_server_drain(smr_server *s);
+int smr_server_destroy(smr_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SURFMOL_RELAX_H */

@@ -1,0 +1,313 @@
+"""surfmol_amd -- MI355X (gfx950) relaxation path for SurfMol systems.
+
+Python host mirror of the reference's set-up + relax API over the C ABI of
+``libsurfmol_relax.so`` (include/surfmol_relax.h, include/surfmol_host.h):
+
+* ``Molecule.build``   -- moltopo/molff set-up (topology.rs, assign_uff.rs,
+                          params.rs, uff.rs:699-887) in the library's C++ host
+                          code; yields the Uff/NonBondedFF tables.
+* ``Context``          -- one device + stream (oclff::nvidia_proque analogue).
+* ``GridFF``           -- tricubic B-spline surface grid: device FP64 build +
+                          fit, or upload of fitted coefficients.
+* ``Batch``            -- many systems sharing topologies: eval_forces
+                          (MolWorld::eval_forces), relax_batched (FIRE,
+                          persistent kernel), relax_plain (one launch sequence
+                          per step, host FIRE), run_md (MolWorld::run_md).
+
+All compute runs in HIP kernels; importing works without a GPU, creating a
+Context needs one.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import (DATA_DIR, SMR_SYS_CONVERGED, SMR_SYS_MAXSTEPS, SMR_SYS_NONFINITE, SMR_SYS_RUNNING, SmrError,
+                   check, lib)
+
+__all__ = ["Molecule", "Context", "GridFF", "Batch", "FireParams", "SmrError", "DATA_DIR", "SMR_SYS_CONVERGED",
+           "SMR_SYS_MAXSTEPS", "SMR_SYS_NONFINITE", "SMR_SYS_RUNNING", "lib"]
+
+_P = ctypes.POINTER
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_P(ctypes.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_P(ctypes.c_int))
+
+
+class Molecule:
+    """One molecule's force-field tables, built by the library's C++ host
+    code (the reference's set-up sequence, relax_pentacene_uff.rs:53-78)."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+        self.topo = _lib.SmrTopology()
+        check(lib().smh_molecule_topology(self._h, ctypes.byref(self.topo)), host=True)
+
+    @classmethod
+    def build(cls, elems, apos, charges=None, bonds=None, rcut_bond=1.8, param_dir=DATA_DIR):
+        apos = np.ascontiguousarray(apos, dtype=np.float64).reshape(-1, 3)
+        n = len(elems)
+        if apos.shape[0] != n:
+            raise ValueError(f"{n} elements but {apos.shape[0]} positions")
+        arr = (ctypes.c_char_p * n)(*[e.encode() for e in elems])
+        q = None if charges is None else np.ascontiguousarray(charges, dtype=np.float64)
+        if bonds is None:
+            nb, bp = 0, None
+        else:
+            b = np.ascontiguousarray(bonds, dtype=np.int32).reshape(-1, 2)
+            nb, bp = len(b), b
+        h = ctypes.c_void_p()
+        check(lib().smh_molecule_build(param_dir.encode(), n, arr, _dptr(apos), None if q is None else _dptr(q), nb,
+                                       None if bp is None else _iptr(bp), float(rcut_bond), ctypes.byref(h)), host=True)
+        m = cls(h.value)
+        m._keep = (arr, apos, q, bp)
+        return m
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().smh_molecule_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def counts(self):
+        c = (ctypes.c_int * 5)()
+        check(lib().smh_molecule_counts(self._h, c), host=True)
+        return tuple(c)
+
+    @property
+    def natoms(self):
+        return self.topo.natoms
+
+    @property
+    def types(self):
+        return lib().smh_molecule_types(self._h).decode().split()
+
+    def tables(self):
+        """numpy copies of every table (for checking against the oracle)."""
+        t = self.topo
+        n, nb, na, nd, ni = self.counts
+
+        def arr(p, k, dt):
+            if k == 0:
+                return np.zeros(0, dtype=dt)
+            return np.ctypeslib.as_array(p, shape=(k,)).copy()
+
+        bonds = np.zeros(2 * nb, dtype=np.int32)
+        if nb:
+            check(lib().smh_molecule_bonds(self._h, _iptr(bonds)), host=True)
+        return {
+            "bonds": bonds.reshape(-1, 2),
+            "neighs": arr(t.neighs, 4 * n, np.int32).reshape(-1, 4),
+            "neigh_bs": arr(t.neigh_bs, 4 * n, np.int32).reshape(-1, 4),
+            "bon_params": arr(t.bon_params, 2 * nb, np.float64).reshape(-1, 2),
+            "ang_atoms": arr(t.ang_atoms, 3 * na, np.int32).reshape(-1, 3),
+            "ang_ngs": arr(t.ang_ngs, 2 * na, np.int32).reshape(-1, 2),
+            "ang_params": arr(t.ang_params, 5 * na, np.float64).reshape(-1, 5),
+            "dih_atoms": arr(t.dih_atoms, 4 * nd, np.int32).reshape(-1, 4),
+            "dih_ngs": arr(t.dih_ngs, 3 * nd, np.int32).reshape(-1, 3),
+            "dih_params": arr(t.dih_params, 3 * nd, np.float64).reshape(-1, 3),
+            "inv_atoms": arr(t.inv_atoms, 4 * ni, np.int32).reshape(-1, 4),
+            "inv_ngs": arr(t.inv_ngs, 3 * ni, np.int32).reshape(-1, 3),
+            "inv_params": arr(t.inv_params, 4 * ni, np.float64).reshape(-1, 4),
+            "reqs": arr(t.reqs, 4 * n, np.float64).reshape(-1, 4),
+            "types": self.types,
+        }
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().smr_ctx_create(int(device), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().smr_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class GridSpec:
+    nx: int
+    ny: int
+    nz: int
+    p0: tuple = (0.0, 0.0, 0.0)
+    step: tuple = (0.1, 0.1, 0.1)
+    alpha_morse: float = 1.5
+    rdamp: float = 0.1
+
+    def desc(self, keep_reference=False):
+        d = _lib.SmrGridDesc()
+        d.nx, d.ny, d.nz = int(self.nx), int(self.ny), int(self.nz)
+        for k in range(3):
+            d.p0[k] = float(self.p0[k])
+            d.step[k] = float(self.step[k])
+        d.alpha_morse = float(self.alpha_morse)
+        d.rdamp = float(self.rdamp)
+        d.keep_reference = 1 if keep_reference else 0
+        return d
+
+
+class GridFF:
+    def __init__(self, ctx: Context, handle, spec: GridSpec):
+        self.ctx = ctx
+        self._h = handle
+        self.spec = spec
+
+    @classmethod
+    def build(cls, ctx: Context, spec: GridSpec, sub_pos, sub_reqs, npbc=2, keep_reference=False):
+        sp = np.ascontiguousarray(sub_pos, dtype=np.float64).reshape(-1, 3)
+        sr = np.ascontiguousarray(sub_reqs, dtype=np.float64).reshape(-1, 4)
+        h = ctypes.c_void_p()
+        d = spec.desc(keep_reference)
+        check(lib().smr_gridff_build(ctx._h, ctypes.byref(d), len(sp), _dptr(sp), _dptr(sr), int(npbc), ctypes.byref(h)))
+        return cls(ctx, h, spec)
+
+    @classmethod
+    def upload(cls, ctx: Context, spec: GridSpec, coefs):
+        c = np.ascontiguousarray(coefs, dtype=np.float32).reshape(-1)
+        if c.size != spec.nx * spec.ny * spec.nz * 4:
+            raise ValueError("coefficient array size does not match the grid")
+        h = ctypes.c_void_p()
+        d = spec.desc()
+        check(lib().smr_gridff_upload(ctx._h, ctypes.byref(d), c.ctypes.data_as(_P(ctypes.c_float)), ctypes.byref(h)))
+        return cls(ctx, h, spec)
+
+    def coefficients(self):
+        s = self.spec
+        out = np.zeros((s.nx, s.ny, s.nz, 4), dtype=np.float32)
+        check(lib().smr_gridff_download(self._h, out.ctypes.data_as(_P(ctypes.c_float))))
+        return out
+
+    def reference_values(self):
+        s = self.spec
+        out = np.zeros((s.nx, s.ny, s.nz, 4), dtype=np.float64)
+        check(lib().smr_gridff_download_reference(self._h, _dptr(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().smr_gridff_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class FireParams:
+    """FireState (molff/src/raff.rs:1040-1062); times in internal units."""
+    dt0: float
+    dt_max: float
+    alpha0: float = 0.1
+    n_min: int = 5
+    f_inc: float = 1.1
+    f_dec: float = 0.5
+    f_alpha: float = 0.99
+    flim: float = 0.0
+
+    def cstruct(self):
+        return _lib.SmrFireParams(self.dt0, self.dt_max, self.alpha0, self.n_min, self.f_inc, self.f_dec, self.f_alpha,
+                                  self.flim)
+
+
+def _results(arr):
+    return [{"steps": r.steps, "status": r.status, "energy": r.energy, "fmax": r.fmax, "terms": list(r.terms)}
+            for r in arr]
+
+
+class Batch:
+    """A set of systems on the device (shared topologies, per-system state)."""
+
+    def __init__(self, ctx: Context, molecules, sys_topo, apos, grid: GridFF | None = None, rdamp=0.1,
+                 nonbonded=True, surface=True):
+        self.ctx = ctx
+        self.grid = grid
+        self.molecules = list(molecules)
+        topos = (_lib.SmrTopology * len(self.molecules))(*[m.topo for m in self.molecules])
+        st = np.ascontiguousarray(sys_topo, dtype=np.int32)
+        x = np.ascontiguousarray(apos, dtype=np.float64).reshape(-1, 3)
+        nat = sum(self.molecules[t].natoms for t in st)
+        if x.shape[0] != nat:
+            raise ValueError(f"apos has {x.shape[0]} atoms, systems need {nat}")
+        opts = _lib.SmrFFOptions(float(rdamp), 1 if nonbonded else 0, 1 if (surface and grid is not None) else 0)
+        h = ctypes.c_void_p()
+        check(lib().smr_batch_create(ctx._h, grid._h if grid is not None else None, len(self.molecules), topos,
+                                     len(st), _iptr(st), _dptr(x), ctypes.byref(opts), ctypes.byref(h)))
+        self._h = h
+        self.nsys = len(st)
+        self.natoms = nat
+        off = np.zeros(self.nsys + 1, dtype=np.int32)
+        check(lib().smr_batch_atom_offsets(self._h, _iptr(off)))
+        self.offsets = off
+
+    def positions(self):
+        x = np.zeros((self.natoms, 3))
+        check(lib().smr_batch_get_positions(self._h, _dptr(x)))
+        return x
+
+    def set_positions(self, apos):
+        x = np.ascontiguousarray(apos, dtype=np.float64).reshape(-1, 3)
+        check(lib().smr_batch_set_positions(self._h, _dptr(x)))
+
+    def eval_forces(self):
+        """(F[natoms,3], E[nsys,6]) at the current positions."""
+        f = np.zeros((self.natoms, 3))
+        e = np.zeros((self.nsys, 6))
+        check(lib().smr_batch_eval_forces(self._h, _dptr(f), _dptr(e)))
+        return f, e
+
+    def relax_batched(self, fire: FireParams, max_steps: int, fconv: float):
+        res = (_lib.SmrRelaxResult * self.nsys)()
+        fp = fire.cstruct()
+        check(lib().smr_relax_batched(self._h, ctypes.byref(fp), int(max_steps), float(fconv), res))
+        return _results(res)
+
+    def relax_plain(self, isys: int, fire: FireParams, max_steps: int, fconv: float):
+        res = _lib.SmrRelaxResult()
+        fp = fire.cstruct()
+        check(lib().smr_relax_plain(self._h, int(isys), ctypes.byref(fp), int(max_steps), float(fconv),
+                                    ctypes.byref(res)))
+        return _results([res])[0]
+
+    def run_md(self, niter, dt, fconv, flim, damping):
+        steps = np.zeros(self.nsys, dtype=np.int32)
+        check(lib().smr_run_md(self._h, int(niter), float(dt), float(fconv), float(flim), float(damping), _iptr(steps)))
+        return steps
+
+    def last_launch(self):
+        shape = (ctypes.c_int * 4)()
+        ms = ctypes.c_float()
+        check(lib().smr_batch_last_launch(self._h, shape, ctypes.byref(ms)))
+        return {"block": shape[0], "apt": shape[1], "grid": shape[2], "lds_bytes": shape[3], "kernel_ms": ms.value}
+
+    def close(self):
+        if self._h:
+            lib().smr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

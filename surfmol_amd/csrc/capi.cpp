@@ -1,0 +1,783 @@
+// capi.cpp -- implementation of the C ABI in include/surfmol_relax.h.
+//
+// Host side of the device boundary: validates and packs the caller's
+// Uff/NonBondedFF tables into the device pools of smr_device.h, owns device
+// memory and streams, and drives the kernels of relax_kernels.hip and
+// gridff_kernels.hip.  No device code here.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/surfmol_relax.h"
+#include "smr_internal.h"
+#include "smr_kernels.h"
+
+using namespace smr;
+
+namespace smr {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *where) {
+    return fail(SMR_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+void DevBuf::release() {
+    if (ptr) hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+hipError_t DevBuf::alloc(size_t nbytes) {
+    release();
+    if (nbytes == 0) nbytes = 16;
+    hipError_t e = hipMalloc(&ptr, nbytes);
+    if (e == hipSuccess) bytes = nbytes;
+    else ptr = nullptr;
+    return e;
+}
+
+hipError_t DevBuf::upload(const void *src, size_t nbytes, hipStream_t st) {
+    hipError_t e = alloc(nbytes);
+    if (e != hipSuccess) return e;
+    if (nbytes == 0) return hipSuccess;
+    return hipMemcpyAsync(ptr, src, nbytes, hipMemcpyHostToDevice, st);
+}
+}  // namespace smr
+
+#define HIPCHK(expr)                                              \
+    do {                                                          \
+        hipError_t _e = (expr);                                   \
+        if (_e != hipSuccess) return hip_fail(_e, #expr);         \
+    } while (0)
+
+extern "C" const char *smr_last_error(void) { return g_err.c_str(); }
+extern "C" const char *smr_version(void) { return "surfmol_relax 0.1 (gfx950)"; }
+
+// ============================================================ context
+extern "C" int smr_ctx_create(int device, smr_ctx **out) {
+    if (!out) return fail(SMR_ERR_INVALID, "smr_ctx_create: out == NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0)
+        return fail(SMR_ERR_HIP, std::string("smr_ctx_create: no HIP device (") + hipGetErrorString(e) + ")");
+    if (device < 0 || device >= ndev)
+        return fail(SMR_ERR_INVALID, "smr_ctx_create: device " + std::to_string(device) + " not in [0," +
+                                         std::to_string(ndev) + ")");
+    HIPCHK(hipSetDevice(device));
+    auto *c = new smr_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) { delete c; return hip_fail(e, "hipGetDeviceProperties"); }
+    c->num_cu = prop.multiProcessorCount;
+    c->max_lds = prop.sharedMemPerBlock;
+    c->arch = prop.gcnArchName;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
+    e = c->counter.alloc(sizeof(int) * 64);
+    if (e != hipSuccess) { hipStreamDestroy(c->stream); delete c; return hip_fail(e, "hipMalloc(counter)"); }
+    *out = c;
+    return SMR_OK;
+}
+
+extern "C" int smr_ctx_destroy(smr_ctx *c) {
+    if (!c) return SMR_OK;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    c->counter.release();
+    hipStreamDestroy(c->stream);
+    delete c;
+    return SMR_OK;
+}
+
+// ============================================================ GridFF
+static int check_desc(const smr_grid_desc *d, const char *who) {
+    if (!d) return fail(SMR_ERR_INVALID, std::string(who) + ": desc == NULL");
+    if (d->nx < 4 || d->ny < 4 || d->nz < 4)
+        return fail(SMR_ERR_INVALID, std::string(who) + ": grid " + std::to_string(d->nx) + "x" + std::to_string(d->ny) +
+                                         "x" + std::to_string(d->nz) + " too small (>= 4 per axis)");
+    for (int k = 0; k < 3; k++)
+        if (!(d->step[k] > 0.0))
+            return fail(SMR_ERR_INVALID, std::string(who) + ": step[" + std::to_string(k) + "] must be > 0");
+    if ((double)d->nx * d->ny * d->nz > 4.0e9)
+        return fail(SMR_ERR_UNSUPPORTED, std::string(who) + ": more than 4e9 nodes");
+    return SMR_OK;
+}
+
+extern "C" int smr_gridff_upload(smr_ctx *c, const smr_grid_desc *d, const float *coefs, smr_gridff **out) {
+    if (!c || !out || !coefs) return fail(SMR_ERR_INVALID, "smr_gridff_upload: NULL argument");
+    *out = nullptr;
+    int rc = check_desc(d, "smr_gridff_upload");
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    auto *g = new smr_gridff();
+    g->ctx = c;
+    g->desc = *d;
+    size_t n = (size_t)d->nx * d->ny * d->nz;
+    hipError_t e = g->coef.upload(coefs, n * 4 * sizeof(float), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_upload"); }
+    *out = g;
+    return SMR_OK;
+}
+
+extern "C" int smr_gridff_build(smr_ctx *c, const smr_grid_desc *d, int nsub, const double *sub_pos,
+                                const double *sub_reqs, int npbc, smr_gridff **out) {
+    if (!c || !out) return fail(SMR_ERR_INVALID, "smr_gridff_build: NULL argument");
+    *out = nullptr;
+    int rc = check_desc(d, "smr_gridff_build");
+    if (rc) return rc;
+    if (nsub <= 0 || !sub_pos || !sub_reqs) return fail(SMR_ERR_INVALID, "smr_gridff_build: need nsub > 0 substrate atoms");
+    if (npbc < 0 || npbc > 64) return fail(SMR_ERR_INVALID, "smr_gridff_build: npbc " + std::to_string(npbc) + " not in [0,64]");
+    if (d->nx < 3 || d->ny < 3) return fail(SMR_ERR_INVALID, "smr_gridff_build: periodic axes need >= 3 nodes");
+    HIPCHK(hipSetDevice(c->device));
+    auto *g = new smr_gridff();
+    g->ctx = c;
+    g->desc = *d;
+    const size_t n = (size_t)d->nx * d->ny * d->nz;
+    DevBuf dpos, dreq, E;
+    hipError_t e = dpos.upload(sub_pos, sizeof(double) * 3 * nsub, c->stream);
+    if (e == hipSuccess) e = dreq.upload(sub_reqs, sizeof(double) * 4 * nsub, c->stream);
+    if (e == hipSuccess) e = E.alloc(sizeof(double) * 4 * n);
+    if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_build: allocation"); }
+    GridBuildArgs G;
+    G.nx = d->nx; G.ny = d->ny; G.nz = d->nz;
+    for (int k = 0; k < 3; k++) G.p0[k] = d->p0[k];
+    G.L[0] = d->nx * d->step[0];
+    G.L[1] = d->ny * d->step[1];
+    G.L[2] = d->nz * d->step[2];
+    G.npbc = npbc;
+    G.alpha = d->alpha_morse;
+    G.r2damp = d->rdamp * d->rdamp;
+    G.nsub = nsub;
+    G.sub_pos = (const double *)dpos.ptr;
+    G.sub_req = (const double *)dreq.ptr;
+    G.out = (double *)E.ptr;
+    e = launch_gridff_build(G, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_build: build kernel"); }
+    if (d->keep_reference) {
+        e = g->reference.alloc(sizeof(double) * 4 * n);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(g->reference.ptr, E.ptr, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_build: keep_reference"); }
+    }
+    e = launch_bspline_fit((double *)E.ptr, d->nx, d->ny, d->nz, nullptr, c->stream);
+    if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_build: B-spline fit"); }
+    e = g->coef.alloc(sizeof(float) * 4 * n);
+    if (e == hipSuccess) e = launch_pack_f32((const double *)E.ptr, (float *)g->coef.ptr, n, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_build: pack"); }
+    *out = g;
+    return SMR_OK;
+}
+
+extern "C" int smr_gridff_download(const smr_gridff *g, float *coefs) {
+    if (!g || !coefs) return fail(SMR_ERR_INVALID, "smr_gridff_download: NULL argument");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    HIPCHK(hipMemcpy(coefs, g->coef.ptr, g->coef.bytes, hipMemcpyDeviceToHost));
+    return SMR_OK;
+}
+
+extern "C" int smr_gridff_download_reference(const smr_gridff *g, double *values) {
+    if (!g || !values) return fail(SMR_ERR_INVALID, "smr_gridff_download_reference: NULL argument");
+    if (!g->reference.ptr)
+        return fail(SMR_ERR_STATE, "smr_gridff_download_reference: grid was not built with keep_reference = 1");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    HIPCHK(hipMemcpy(values, g->reference.ptr, g->reference.bytes, hipMemcpyDeviceToHost));
+    return SMR_OK;
+}
+
+extern "C" int smr_gridff_destroy(smr_gridff *g) {
+    if (!g) return SMR_OK;
+    hipSetDevice(g->ctx->device);
+    delete g;
+    return SMR_OK;
+}
+
+// ============================================================ batch packing
+namespace smr {
+
+int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, HostPools &P, TopoDesc &td) {
+    const std::string who = "topology " + std::to_string(ti) + ": ";
+    const int N = t.natoms;
+    if (N <= 0) return fail(SMR_ERR_INVALID, who + "natoms must be > 0");
+    if (N > SMR_MAX_ATOMS)
+        return fail(SMR_ERR_UNSUPPORTED, who + "natoms " + std::to_string(N) + " > " + std::to_string(SMR_MAX_ATOMS));
+    if (t.nbonds < 0 || t.nangles < 0 || t.ndih < 0 || t.ninv < 0) return fail(SMR_ERR_INVALID, who + "negative counts");
+    if (!t.neighs || !t.neigh_bs || !t.reqs || (t.nbonds && !t.bon_params) ||
+        (t.nangles && (!t.ang_atoms || !t.ang_ngs || !t.ang_params)) ||
+        (t.ndih && (!t.dih_atoms || !t.dih_ngs || !t.dih_params)) || (t.ninv && (!t.inv_atoms || !t.inv_ngs || !t.inv_params)))
+        return fail(SMR_ERR_INVALID, who + "NULL table for a non-empty term list");
+    td.natoms = N;
+    td.nangles = t.nangles;
+    td.ndih = t.ndih;
+    td.ninv = t.ninv;
+    td.nf = 4 * t.ndih + 4 * t.ninv + 3 * t.nangles;
+    td.atom_off = (int)P.ng.size();
+    td.ang_off = (int)P.ang_ng.size();
+    td.dih_off = (int)P.dih_ng.size();
+    td.inv_off = (int)P.inv_ng.size();
+    td.a2f_off = (int)P.a2f_idx.size();
+    // ---- atoms ----
+    for (int i = 0; i < N; i++) {
+        int ng[4];
+        float bk[4] = {0, 0, 0, 0}, bl[4] = {0, 0, 0, 0};
+        for (int s = 0; s < 4; s++) {
+            ng[s] = t.neighs[4 * i + s];
+            if (ng[s] < -1 || ng[s] >= N || ng[s] == i)
+                return fail(SMR_ERR_INVALID, who + "neighs[" + std::to_string(i) + "][" + std::to_string(s) + "] = " +
+                                                 std::to_string(ng[s]) + " out of range");
+            if (ng[s] >= 0) {
+                int ib = t.neigh_bs[4 * i + s];
+                if (ib < 0 || ib >= t.nbonds)
+                    return fail(SMR_ERR_INVALID, who + "neigh_bs[" + std::to_string(i) + "][" + std::to_string(s) + "] = " +
+                                                     std::to_string(ib) + " not a bond index");
+                bk[s] = (float)t.bon_params[2 * ib];
+                bl[s] = (float)t.bon_params[2 * ib + 1];
+            }
+        }
+        P.ng.push_back(make_int4(ng[0], ng[1], ng[2], ng[3]));
+        P.bk.push_back(make_float4(bk[0], bk[1], bk[2], bk[3]));
+        P.bl.push_back(make_float4(bl[0], bl[1], bl[2], bl[3]));
+        const double *r = t.reqs + 4 * i;
+        P.req.push_back(make_float4((float)r[0], (float)r[1], (float)r[2], 0.f));
+        // PLQH (opencl/UFF.cl:1681-1687), computed in FP64
+        if (have_grid) {
+            double ej = std::exp(alpha * r[0]);
+            P.plq.push_back(make_float4((float)(ej * ej * r[1]), (float)(ej * r[1]), (float)r[2], 0.f));
+        } else {
+            P.plq.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+        }
+    }
+    auto slot_ok = [&](int g) { return g >= 0 && g < 4 * N && t.neighs[g] >= 0; };
+    // ---- angles ----
+    for (int a = 0; a < t.nangles; a++) {
+        int g0 = t.ang_ngs[2 * a], g1 = t.ang_ngs[2 * a + 1];
+        if (!slot_ok(g0) || !slot_ok(g1))
+            return fail(SMR_ERR_INVALID, who + "angle " + std::to_string(a) + " has hneigh slots (" + std::to_string(g0) +
+                                             "," + std::to_string(g1) + ") that are not bonds");
+        const double *p = t.ang_params + 5 * a;
+        P.ang_ng.push_back(make_int2(g0, g1));
+        P.ang_par.push_back(make_float4((float)(p[0] * p[1]), (float)(p[0] * p[2]), (float)(p[0] * p[3]), (float)(p[0] * p[4])));
+    }
+    // ---- dihedrals ----
+    for (int d = 0; d < t.ndih; d++) {
+        int g0 = t.dih_ngs[3 * d], g1 = t.dih_ngs[3 * d + 1], g2 = t.dih_ngs[3 * d + 2];
+        if (!slot_ok(g0) || !slot_ok(g1) || !slot_ok(g2))
+            return fail(SMR_ERR_INVALID, who + "dihedral " + std::to_string(d) + " has an invalid hneigh slot");
+        const double *p = t.dih_params + 3 * d;
+        int n = (int)p[2];
+        if (n < 1 || n > 12) return fail(SMR_ERR_INVALID, who + "dihedral " + std::to_string(d) + " multiplicity " + std::to_string(n));
+        P.dih_ng.push_back(make_int4(g0, g1, g2, n));
+        P.dih_par.push_back(make_float4((float)p[0], (float)(p[0] * p[1]), (float)(p[0] * p[1] * p[2]), 0.f));
+    }
+    // ---- inversions ----
+    for (int v = 0; v < t.ninv; v++) {
+        int g0 = t.inv_ngs[3 * v], g1 = t.inv_ngs[3 * v + 1], g2 = t.inv_ngs[3 * v + 2];
+        if (!slot_ok(g0) || !slot_ok(g1) || !slot_ok(g2))
+            return fail(SMR_ERR_INVALID, who + "inversion " + std::to_string(v) + " has an invalid hneigh slot");
+        const double *p = t.inv_params + 4 * v;
+        P.inv_ng.push_back(make_int4(g0, g1, g2, 0));
+        P.inv_par.push_back(make_float4((float)(p[0] * p[1]), (float)(p[0] * p[2]), (float)(p[0] * p[3]), 0.f));
+    }
+    // ---- a2f gather lists (Uff::map_atom_interactions, uff.rs:317-370):
+    //      dihedral pieces, then inversion pieces, then angle pieces ----
+    std::vector<int> cnt(N, 0), off(N + 1, 0);
+    auto chk_atom = [&](int a) { return a >= 0 && a < N; };
+    for (int d = 0; d < t.ndih; d++)
+        for (int k = 0; k < 4; k++) {
+            int a = t.dih_atoms[4 * d + k];
+            if (!chk_atom(a)) return fail(SMR_ERR_INVALID, who + "dihedral atom out of range");
+            cnt[a]++;
+        }
+    for (int v = 0; v < t.ninv; v++)
+        for (int k = 0; k < 4; k++) {
+            int a = t.inv_atoms[4 * v + k];
+            if (!chk_atom(a)) return fail(SMR_ERR_INVALID, who + "inversion atom out of range");
+            cnt[a]++;
+        }
+    for (int a3 = 0; a3 < t.nangles; a3++)
+        for (int k = 0; k < 3; k++) {
+            int a = t.ang_atoms[3 * a3 + k];
+            if (!chk_atom(a)) return fail(SMR_ERR_INVALID, who + "angle atom out of range");
+            cnt[a]++;
+        }
+    for (int i = 0; i < N; i++) off[i + 1] = off[i] + cnt[i];
+    std::vector<int> idx(off[N]);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    const int i0inv = 4 * t.ndih, i0ang = i0inv + 4 * t.ninv;
+    for (int d = 0; d < t.ndih; d++)
+        for (int k = 0; k < 4; k++) { int a = t.dih_atoms[4 * d + k]; idx[off[a] + cnt[a]++] = 4 * d + k; }
+    for (int v = 0; v < t.ninv; v++)
+        for (int k = 0; k < 4; k++) { int a = t.inv_atoms[4 * v + k]; idx[off[a] + cnt[a]++] = i0inv + 4 * v + k; }
+    for (int a3 = 0; a3 < t.nangles; a3++)
+        for (int k = 0; k < 3; k++) { int a = t.ang_atoms[3 * a3 + k]; idx[off[a] + cnt[a]++] = i0ang + 3 * a3 + k; }
+    for (int i = 0; i < N; i++) P.a2f.push_back(make_int2(off[i], off[i + 1] - off[i]));
+    P.a2f_idx.insert(P.a2f_idx.end(), idx.begin(), idx.end());
+    return SMR_OK;
+}
+
+hipError_t upload_pools(HostPools &P, DevPools &D, hipStream_t st) {
+    hipError_t e;
+#define UP(name) if ((e = D.name.upload(P.name.data(), P.name.size() * sizeof(P.name[0]), st)) != hipSuccess) return e;
+    UP(topo) UP(sys) UP(ng) UP(bk) UP(bl) UP(req) UP(plq) UP(a2f) UP(ang_ng) UP(ang_par) UP(dih_ng) UP(dih_par)
+    UP(inv_ng) UP(inv_par) UP(a2f_idx)
+#undef UP
+    return hipSuccess;
+}
+
+void fill_tables(const DevPools &D, const smr_gridff *grid, const smr_ff_options &o, DevTables &T) {
+    T.topo = (const TopoDesc *)D.topo.ptr;
+    T.sys = (const SysDesc *)D.sys.ptr;
+    T.ng = (const int4 *)D.ng.ptr;
+    T.bk = (const float4 *)D.bk.ptr;
+    T.bl = (const float4 *)D.bl.ptr;
+    T.req = (const float4 *)D.req.ptr;
+    T.plq = (const float4 *)D.plq.ptr;
+    T.a2f = (const int2 *)D.a2f.ptr;
+    T.ang_ng = (const int2 *)D.ang_ng.ptr;
+    T.ang_par = (const float4 *)D.ang_par.ptr;
+    T.dih_ng = (const int4 *)D.dih_ng.ptr;
+    T.dih_par = (const float4 *)D.dih_par.ptr;
+    T.inv_ng = (const int4 *)D.inv_ng.ptr;
+    T.inv_par = (const float4 *)D.inv_par.ptr;
+    T.a2f_idx = (const int *)D.a2f_idx.ptr;
+    if (grid && o.surface) {
+        T.grid = (const float4 *)grid->coef.ptr;
+        T.gnx = grid->desc.nx; T.gny = grid->desc.ny; T.gnz = grid->desc.nz;
+        T.ginv_x = (float)(1.0 / grid->desc.step[0]);
+        T.ginv_y = (float)(1.0 / grid->desc.step[1]);
+        T.ginv_z = (float)(1.0 / grid->desc.step[2]);
+    } else {
+        T.grid = nullptr;
+        T.gnx = T.gny = T.gnz = 1;
+        T.ginv_x = T.ginv_y = T.ginv_z = 1.f;
+    }
+    T.r2damp = (float)(o.rdamp * o.rdamp);
+    T.use_pairs = o.nonbonded ? 1 : 0;
+}
+
+// system origin: grid-node aligned in x,y (so the surface lookup offset is an
+// exact integer), the grid's z origin in z; centroid-aligned without a grid.
+void system_origin(const double *x, int n, const smr_gridff *grid, bool surface, double org[3], int &iux, int &iuy) {
+    double c[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) c[k] += x[3 * i + k];
+    for (int k = 0; k < 3; k++) c[k] /= n;
+    if (grid && surface) {
+        const smr_grid_desc &d = grid->desc;
+        long long jx = std::llround((c[0] - d.p0[0]) / d.step[0]);
+        long long jy = std::llround((c[1] - d.p0[1]) / d.step[1]);
+        org[0] = d.p0[0] + jx * d.step[0];
+        org[1] = d.p0[1] + jy * d.step[1];
+        org[2] = d.p0[2];
+        iux = (int)(((jx % d.nx) + d.nx) % d.nx);
+        iuy = (int)(((jy % d.ny) + d.ny) % d.ny);
+    } else {
+        for (int k = 0; k < 3; k++) org[k] = std::round(c[k]);
+        iux = iuy = 0;
+    }
+}
+
+LaunchShape choose_shape(const smr_ctx *c, int max_atoms, int nf_max, int nwork) {
+    LaunchShape L;
+    if (max_atoms <= 64) { L.block = 64; L.apt = 1; }
+    else if (max_atoms <= 128) { L.block = 128; L.apt = 1; }
+    else if (max_atoms <= 256) { L.block = 256; L.apt = 1; }
+    else if (max_atoms <= 512) { L.block = 256; L.apt = 2; }
+    else { L.block = 256; L.apt = 4; }
+    size_t with = relax_lds_bytes(L.block, max_atoms, nf_max, true);
+    L.fint_lds = with <= std::min<size_t>(c->max_lds, 64 * 1024);
+    L.lds_bytes = relax_lds_bytes(L.block, max_atoms, nf_max, L.fint_lds);
+    int occ = 0;
+    const void *k = relax_kernel_ptr(L.block, L.apt, L.fint_lds);
+    if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, L.block, L.lds_bytes) != hipSuccess || occ < 1) occ = 1;
+    long long cap = (long long)occ * c->num_cu;
+    L.grid = (int)std::max<long long>(1, std::min<long long>(cap, nwork));
+    return L;
+}
+
+}  // namespace smr
+
+extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, const smr_topology *topos, int nsys,
+                                const int *sys_topo, const double *apos, const smr_ff_options *opts, smr_batch **out) {
+    if (!c || !out || !topos || !sys_topo || !apos || !opts)
+        return fail(SMR_ERR_INVALID, "smr_batch_create: NULL argument");
+    *out = nullptr;
+    if (ntopo <= 0 || nsys <= 0) return fail(SMR_ERR_INVALID, "smr_batch_create: ntopo and nsys must be > 0");
+    if (opts->surface && !grid) return fail(SMR_ERR_INVALID, "smr_batch_create: surface requested but grid == NULL");
+    if (!(opts->rdamp >= 0.0)) return fail(SMR_ERR_INVALID, "smr_batch_create: rdamp must be >= 0");
+    HIPCHK(hipSetDevice(c->device));
+    auto *b = new smr_batch();
+    b->ctx = c;
+    b->grid = opts->surface ? grid : nullptr;
+    b->opts = *opts;
+    const double alpha = grid ? grid->desc.alpha_morse : 0.0;
+    for (int t = 0; t < ntopo; t++) {
+        TopoDesc td;
+        int rc = pack_topology(topos[t], t, alpha, b->grid != nullptr, b->hp, td);
+        if (rc) { delete b; return rc; }
+        b->hp.topo.push_back(td);
+        b->max_atoms = std::max(b->max_atoms, td.natoms);
+        b->nf_max = std::max(b->nf_max, td.nf);
+    }
+    b->nsys = nsys;
+    b->atom_off.assign(nsys + 1, 0);
+    for (int s = 0; s < nsys; s++) {
+        int t = sys_topo[s];
+        if (t < 0 || t >= ntopo) {
+            delete b;
+            return fail(SMR_ERR_INVALID, "smr_batch_create: sys_topo[" + std::to_string(s) + "] = " + std::to_string(t) +
+                                             " not in [0," + std::to_string(ntopo) + ")");
+        }
+        b->atom_off[s + 1] = b->atom_off[s] + b->hp.topo[t].natoms;
+    }
+    b->total_atoms = b->atom_off[nsys];
+    b->origin.assign(3 * (size_t)nsys, 0.0);
+    for (int s = 0; s < nsys; s++) {
+        SysDesc sd;
+        sd.topo = sys_topo[s];
+        sd.atom_base = b->atom_off[s];
+        system_origin(apos + 3 * (size_t)b->atom_off[s], b->hp.topo[sd.topo].natoms, b->grid, b->grid != nullptr,
+                      &b->origin[3 * s], sd.iux, sd.iuy);
+        b->hp.sys.push_back(sd);
+    }
+    hipError_t e = upload_pools(b->hp, b->dp, c->stream);
+    if (e == hipSuccess) e = b->pos.alloc(sizeof(float4) * b->total_atoms);
+    if (e == hipSuccess) e = b->vel.alloc(sizeof(float4) * b->total_atoms);
+    if (e == hipSuccess) e = b->force.alloc(sizeof(float4) * b->total_atoms);
+    if (e == hipSuccess) e = b->state.alloc(sizeof(SysState) * nsys);
+    if (e == hipSuccess) e = b->work.alloc(sizeof(int) * nsys);
+    if (e == hipSuccess) e = hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, c->stream);
+    if (e != hipSuccess) { delete b; return hip_fail(e, "smr_batch_create: device allocation"); }
+    fill_tables(b->dp, b->grid, b->opts, b->tab);
+    int rc = smr_batch_set_positions(b, apos);
+    if (rc) { delete b; return rc; }
+    std::vector<int> wl(nsys);
+    for (int s = 0; s < nsys; s++) wl[s] = s;
+    e = hipMemcpyAsync(b->work.ptr, wl.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { delete b; return hip_fail(e, "smr_batch_create: upload"); }
+    *out = b;
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_destroy(smr_batch *b) {
+    if (!b) return SMR_OK;
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_natoms(const smr_batch *b, int *total) {
+    if (!b || !total) return fail(SMR_ERR_INVALID, "smr_batch_natoms: NULL argument");
+    *total = b->total_atoms;
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_atom_offsets(const smr_batch *b, int *offsets) {
+    if (!b || !offsets) return fail(SMR_ERR_INVALID, "smr_batch_atom_offsets: NULL argument");
+    std::copy(b->atom_off.begin(), b->atom_off.end(), offsets);
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_set_positions(smr_batch *b, const double *apos) {
+    if (!b || !apos) return fail(SMR_ERR_INVALID, "smr_batch_set_positions: NULL argument");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    std::vector<float4> h(b->total_atoms);
+    for (int s = 0; s < b->nsys; s++) {
+        const double *o = &b->origin[3 * s];
+        for (int i = b->atom_off[s]; i < b->atom_off[s + 1]; i++) {
+            double x = apos[3 * (size_t)i] - o[0], y = apos[3 * (size_t)i + 1] - o[1], z = apos[3 * (size_t)i + 2] - o[2];
+            if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z))
+                return fail(SMR_ERR_INVALID, "smr_batch_set_positions: atom " + std::to_string(i) + " is not finite");
+            h[i] = make_float4((float)x, (float)y, (float)z, 0.f);
+        }
+    }
+    HIPCHK(hipMemcpyAsync(b->pos.ptr, h.data(), sizeof(float4) * h.size(), hipMemcpyHostToDevice, b->ctx->stream));
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_get_positions(const smr_batch *b, double *apos) {
+    if (!b || !apos) return fail(SMR_ERR_INVALID, "smr_batch_get_positions: NULL argument");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    std::vector<float4> h(b->total_atoms);
+    HIPCHK(hipMemcpyAsync(h.data(), b->pos.ptr, sizeof(float4) * h.size(), hipMemcpyDeviceToHost, b->ctx->stream));
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    for (int s = 0; s < b->nsys; s++) {
+        const double *o = &b->origin[3 * s];
+        for (int i = b->atom_off[s]; i < b->atom_off[s + 1]; i++) {
+            apos[3 * (size_t)i] = (double)h[i].x + o[0];
+            apos[3 * (size_t)i + 1] = (double)h[i].y + o[1];
+            apos[3 * (size_t)i + 2] = (double)h[i].z + o[2];
+        }
+    }
+    return SMR_OK;
+}
+
+// ============================================================ kernels drivers
+namespace smr {
+
+int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch, bool reset_state,
+              double dt0, double alpha0, float *kernel_ms) {
+    smr_ctx *c = b->ctx;
+    hipStream_t st = c->stream;
+    if (reset_state) {
+        std::vector<SysState> h(b->nsys);
+        for (auto &s : h) {
+            memset(&s, 0, sizeof(s));
+            s.dt = dt0;
+            s.alpha = alpha0;
+            s.status = SMR_SYS_RUNNING;
+        }
+        HIPCHK(hipMemcpyAsync(b->state.ptr, h.data(), sizeof(SysState) * h.size(), hipMemcpyHostToDevice, st));
+    }
+    LaunchShape L = choose_shape(c, b->max_atoms, b->nf_max, b->nsys);
+    if (L.lds_bytes > c->max_lds)
+        return fail(SMR_ERR_UNSUPPORTED, "k_relax needs " + std::to_string(L.lds_bytes) + " B of LDS for " +
+                                             std::to_string(b->max_atoms) + " atoms; the device allows " +
+                                             std::to_string(c->max_lds));
+    if (!L.fint_lds) {
+        size_t need = sizeof(float) * 3 * (size_t)b->nf_max * L.grid;
+        if (b->fint_scratch.bytes < need) {
+            hipError_t e = b->fint_scratch.alloc(need);
+            if (e != hipSuccess) return hip_fail(e, "fint scratch");
+        }
+    }
+    KernelArgs A;
+    A.tab = b->tab;
+    A.fire = fc;
+    A.pos = (float4 *)b->pos.ptr;
+    A.vel = (float4 *)b->vel.ptr;
+    A.force = (float4 *)b->force.ptr;
+    A.state = (SysState *)b->state.ptr;
+    A.work_list = (const int *)b->work.ptr;
+    A.nwork = b->nsys;
+    A.work_counter = (int *)c->counter.ptr;
+    A.fint_scratch = (float *)b->fint_scratch.ptr;
+    A.nf_max = std::max(1, b->nf_max);
+    A.lds_atoms = b->max_atoms;
+    A.steps_per_launch = steps_per_launch;
+    A.mode = mode;
+    HIPCHK(hipMemsetAsync(c->counter.ptr, 0, sizeof(int), st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (kernel_ms) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+    }
+    hipError_t e = launch_relax(A, L, st);
+    if (e != hipSuccess) return hip_fail(e, "launch k_relax");
+    if (kernel_ms) {
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(kernel_ms, e0, e1));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    b->last_shape = L;
+    return SMR_OK;
+}
+
+int read_results(smr_batch *b, smr_relax_result *res) {
+    std::vector<SysState> h(b->nsys);
+    HIPCHK(hipMemcpy(h.data(), b->state.ptr, sizeof(SysState) * h.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < b->nsys; s++) {
+        res[s].steps = h[s].steps;
+        res[s].status = h[s].status;
+        res[s].fmax = h[s].fmax;
+        double E = 0.0;
+        for (int q = 0; q < 6; q++) { res[s].terms[q] = h[s].E[q]; E += h[s].E[q]; }
+        res[s].energy = E;
+    }
+    return SMR_OK;
+}
+
+int check_fire(const smr_fire_params *f, const char *who) {
+    if (!f) return fail(SMR_ERR_INVALID, std::string(who) + ": fire == NULL");
+    if (!(f->dt0 > 0.0) || !(f->dt_max >= f->dt0) || !(f->f_dec > 0.0 && f->f_dec < 1.0) || !(f->f_inc >= 1.0) ||
+        !(f->alpha0 >= 0.0 && f->alpha0 <= 1.0) || !(f->f_alpha > 0.0 && f->f_alpha <= 1.0) || f->n_min < 0)
+        return fail(SMR_ERR_INVALID, std::string(who) + ": FIRE parameters out of range (dt0=" + std::to_string(f->dt0) +
+                                         " dt_max=" + std::to_string(f->dt_max) + ")");
+    return SMR_OK;
+}
+
+FireConst make_fire_const(const smr_fire_params *f, int max_steps, double fconv) {
+    FireConst fc;
+    memset(&fc, 0, sizeof(fc));
+    if (f) {
+        fc.dt0 = f->dt0; fc.dt_max = f->dt_max; fc.alpha0 = f->alpha0; fc.f_inc = f->f_inc;
+        fc.f_dec = f->f_dec; fc.f_alpha = f->f_alpha; fc.flim = f->flim; fc.n_min = f->n_min;
+    }
+    fc.fconv = fconv;
+    fc.max_steps = max_steps;
+    return fc;
+}
+
+}  // namespace smr
+
+extern "C" int smr_batch_eval_forces(smr_batch *b, double *forces, double *energies) {
+    if (!b || !forces || !energies) return fail(SMR_ERR_INVALID, "smr_batch_eval_forces: NULL argument");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    // keep the FIRE state: eval mode reads but does not need it; back it up
+    std::vector<SysState> keep(b->nsys);
+    HIPCHK(hipMemcpy(keep.data(), b->state.ptr, sizeof(SysState) * keep.size(), hipMemcpyDeviceToHost));
+    FireConst fc = make_fire_const(nullptr, 1, 0.0);
+    int rc = run_fused(b, KMODE_EVAL, fc, 1, false, 0, 0, nullptr);
+    if (rc) return rc;
+    std::vector<float4> h(b->total_atoms);
+    HIPCHK(hipMemcpy(h.data(), b->force.ptr, sizeof(float4) * h.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->total_atoms; i++) {
+        forces[3 * (size_t)i] = h[i].x;
+        forces[3 * (size_t)i + 1] = h[i].y;
+        forces[3 * (size_t)i + 2] = h[i].z;
+    }
+    std::vector<SysState> hs(b->nsys);
+    HIPCHK(hipMemcpy(hs.data(), b->state.ptr, sizeof(SysState) * hs.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < b->nsys; s++)
+        for (int q = 0; q < 6; q++) energies[6 * (size_t)s + q] = hs[s].E[q];
+    HIPCHK(hipMemcpy(b->state.ptr, keep.data(), sizeof(SysState) * keep.size(), hipMemcpyHostToDevice));
+    return SMR_OK;
+}
+
+extern "C" int smr_relax_batched(smr_batch *b, const smr_fire_params *fire, int max_steps, double fconv,
+                                 smr_relax_result *results) {
+    if (!b || !results) return fail(SMR_ERR_INVALID, "smr_relax_batched: NULL argument");
+    int rc = check_fire(fire, "smr_relax_batched");
+    if (rc) return rc;
+    if (max_steps <= 0 || !(fconv > 0.0)) return fail(SMR_ERR_INVALID, "smr_relax_batched: need max_steps > 0, fconv > 0");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, b->ctx->stream));
+    FireConst fc = make_fire_const(fire, max_steps, fconv);
+    rc = run_fused(b, KMODE_FIRE, fc, max_steps, true, fire->dt0, fire->alpha0, &b->last_kernel_ms);
+    if (rc) return rc;
+    return read_results(b, results);
+}
+
+extern "C" int smr_run_md(smr_batch *b, int niter, double dt, double fconv, double flim, double damping, int *steps) {
+    if (!b || !steps) return fail(SMR_ERR_INVALID, "smr_run_md: NULL argument");
+    if (niter <= 0 || !(dt > 0.0) || !(flim > 0.0)) return fail(SMR_ERR_INVALID, "smr_run_md: need niter > 0, dt > 0, flim > 0");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    FireConst fc = make_fire_const(nullptr, niter, fconv);
+    fc.md_dt = dt;
+    fc.md_flim = flim;
+    fc.md_cdamp = std::max(0.0, 1.0 - damping);
+    int rc = run_fused(b, KMODE_MD, fc, niter, true, dt, 0.0, nullptr);
+    if (rc) return rc;
+    std::vector<SysState> h(b->nsys);
+    HIPCHK(hipMemcpy(h.data(), b->state.ptr, sizeof(SysState) * h.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < b->nsys; s++) steps[s] = h[s].steps;
+    return SMR_OK;
+}
+
+extern "C" int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fire, int max_steps, double fconv,
+                               smr_relax_result *result) {
+    if (!b || !result) return fail(SMR_ERR_INVALID, "smr_relax_plain: NULL argument");
+    int rc = check_fire(fire, "smr_relax_plain");
+    if (rc) return rc;
+    if (isys < 0 || isys >= b->nsys)
+        return fail(SMR_ERR_INVALID, "smr_relax_plain: isys " + std::to_string(isys) + " not in [0," + std::to_string(b->nsys) + ")");
+    if (max_steps <= 0 || !(fconv > 0.0)) return fail(SMR_ERR_INVALID, "smr_relax_plain: need max_steps > 0, fconv > 0");
+    smr_ctx *c = b->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const SysDesc &sd = b->hp.sys[isys];
+    const TopoDesc &td = b->hp.topo[sd.topo];
+    const int N = td.natoms;
+    if (b->plain_hneigh.bytes < sizeof(float4) * 4 * (size_t)b->max_atoms) {
+        hipError_t e = b->plain_hneigh.alloc(sizeof(float4) * 4 * (size_t)b->max_atoms);
+        if (e == hipSuccess) e = b->plain_fint.alloc(sizeof(float4) * (size_t)std::max(1, b->nf_max));
+        if (e == hipSuccess) e = b->plain_e.alloc((sizeof(float) * 2 + sizeof(double)) * (size_t)b->max_atoms);
+        if (e == hipSuccess) e = b->plain_cvf.alloc(sizeof(double) * 16);
+        if (e != hipSuccess) return hip_fail(e, "smr_relax_plain: scratch");
+    }
+    PlainArgs P;
+    P.tab = b->tab;
+    P.topo = sd.topo;
+    P.iux = sd.iux;
+    P.iuy = sd.iuy;
+    P.pos = (float4 *)b->pos.ptr + sd.atom_base;
+    P.vel = (float4 *)b->vel.ptr + sd.atom_base;
+    P.force = (float4 *)b->force.ptr + sd.atom_base;
+    P.hneigh = (float4 *)b->plain_hneigh.ptr;
+    P.fint = (float4 *)b->plain_fint.ptr;
+    P.eatom = (float *)b->plain_e.ptr;
+    P.esurf = P.eatom + b->max_atoms;
+    P.epair = (double *)(P.esurf + b->max_atoms);
+    P.cvf = (double *)b->plain_cvf.ptr;
+    HIPCHK(hipMemsetAsync(P.vel, 0, sizeof(float4) * N, st));
+    double dt = fire->dt0, alpha = fire->alpha0;
+    int npos = 0, steps = max_steps, status = SMR_SYS_MAXSTEPS;
+    double cvf[10] = {0};
+    for (int itr = 0; itr < max_steps; itr++) {
+        hipError_t e = launch_plain_step_forces(P, N, td.nangles, td.ndih, td.ninv, st);
+        if (e == hipSuccess) e = launch_plain_reduce(P, dt, fire->flim, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(cvf, P.cvf, sizeof(double) * 10, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "smr_relax_plain: step");
+        const double vf = cvf[0], vv = cvf[1], ff = cvf[2];
+        const double dt_old = dt;
+        float a = 1.f, bb = 0.f;
+        int reset = 0;
+        if (!(std::isfinite(vf) && std::isfinite(vv) && std::isfinite(ff))) {
+            steps = itr + 1;
+            status = SMR_SYS_NONFINITE;
+            break;
+        }
+        if (vf > 0.0 && ff > 1e-30 && vv > 1e-30) {
+            a = (float)(1.0 - alpha);
+            bb = (float)(alpha * std::sqrt(vv) / std::sqrt(ff));
+            npos++;
+            if (npos > fire->n_min) {
+                dt = std::min(dt * fire->f_inc, fire->dt_max);
+                alpha *= fire->f_alpha;
+            }
+        } else {
+            npos = 0;
+            dt *= fire->f_dec;
+            alpha = fire->alpha0;
+            reset = 1;
+        }
+        e = launch_plain_apply(P, N, a, bb, reset, (float)dt_old, st);
+        if (e != hipSuccess) return hip_fail(e, "smr_relax_plain: apply");
+        if (std::sqrt(cvf[3]) < fconv) {
+            steps = itr + 1;
+            status = SMR_SYS_CONVERGED;
+            break;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    result->steps = steps;
+    result->status = status;
+    result->fmax = std::sqrt(cvf[3]);
+    // cvf[4..9] = Eb, Ea, Ed, Ei, Enb, Esurf
+    double E = 0.0;
+    for (int q = 0; q < 6; q++) { result->terms[q] = cvf[4 + q]; E += cvf[4 + q]; }
+    result->energy = E;
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_last_launch(const smr_batch *b, int *shape4, float *kernel_ms) {
+    if (!b || !shape4 || !kernel_ms) return fail(SMR_ERR_INVALID, "smr_batch_last_launch: NULL argument");
+    shape4[0] = b->last_shape.block;
+    shape4[1] = b->last_shape.apt;
+    shape4[2] = b->last_shape.grid;
+    shape4[3] = (int)b->last_shape.lds_bytes;
+    *kernel_ms = b->last_kernel_ms;
+    return SMR_OK;
+}

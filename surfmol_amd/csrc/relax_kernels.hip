@@ -1,0 +1,621 @@
+// relax_kernels.hip -- HIP kernels (gfx950) of the multi-system relaxation path.
+//
+//  k_relax      the fused persistent kernel: one workgroup owns one system
+//               at a time and runs bonds -> angles/dihedrals/inversions ->
+//               per-atom gather + pairs + surface -> FIRE (or damped MD) for
+//               many steps without leaving the kernel.  Positions, REQ,
+//               hneigh and the bonded force pieces live in LDS; per-system
+//               reductions are wave shuffles (+ LDS across waves) in FP64.
+//               Systems are taken from a work list through an atomic
+//               counter, so uneven systems balance across the chip.
+//  k_plain_*    the plain method: one system, one kernel per force term per
+//               step, FIRE decided on the host (the reference's shape,
+//               opencl/UFF.cl:141-1011 driven by a host loop).
+//
+// Per-term math is in smr_device.h and shared by both paths.
+#include "smr_device.h"
+#include "smr_kernels.h"
+
+namespace smr {
+
+// ===========================================================================
+// fused persistent kernel
+// ===========================================================================
+template <int BLOCK, int APT, bool FINT_LDS>
+__global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
+    constexpr int NW = BLOCK / 64;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
+    const DevTables &T = A.tab;
+
+    // LDS carve-up (sizes fixed per launch by the host)
+    float4 *s_pos = reinterpret_cast<float4 *>(smem);
+    float4 *s_req = s_pos + A.lds_atoms;
+    float4 *s_h = s_req + A.lds_atoms;                       // [4*natoms] hneigh
+    double *s_red = reinterpret_cast<double *>(s_h + 4 * A.lds_atoms);  // [NW*8]
+    int *s_next = reinterpret_cast<int *>(s_red + NW * 8);
+    float *s_fx = reinterpret_cast<float *>(s_next + 4);
+    float *fint_x, *fint_y, *fint_z;
+
+    while (true) {
+        // ---- fetch the next system from the work list ----
+        if (tid == 0) s_next[0] = atomicAdd(A.work_counter, 1);
+        __syncthreads();
+        const int wi = s_next[0];
+        __syncthreads();
+        if (wi >= A.nwork) break;
+        const int s = A.work_list[wi];
+        const SysDesc sd = T.sys[s];
+        const TopoDesc td = T.topo[sd.topo];
+        SysState st = A.state[s];
+        if (st.status != SMR_SYS_RUNNING) continue;
+        const int N = td.natoms;
+        const int abase = sd.atom_base;
+        if (FINT_LDS) {
+            fint_x = s_fx;
+        } else {
+            fint_x = A.fint_scratch + (size_t)blockIdx.x * 3 * A.nf_max;
+        }
+        fint_y = fint_x + A.nf_max;
+        fint_z = fint_y + A.nf_max;
+
+        // ---- stage the system ----
+        int4 my_ng[APT];
+        float4 my_bk[APT], my_bl[APT], my_plq[APT];
+        int2 my_a2f[APT];
+        float3 my_v[APT];
+#pragma unroll
+        for (int k = 0; k < APT; k++) {
+            const int i = tid + k * BLOCK;
+            if (i < N) {
+                const int ta = td.atom_off + i;
+                s_pos[i] = A.pos[abase + i];
+                s_req[i] = T.req[ta];
+                my_ng[k] = T.ng[ta];
+                my_bk[k] = T.bk[ta];
+                my_bl[k] = T.bl[ta];
+                my_plq[k] = T.plq[ta];
+                my_a2f[k] = T.a2f[ta];
+                float4 v = A.vel[abase + i];
+                my_v[k] = make_float3(v.x, v.y, v.z);
+            } else {
+                my_ng[k] = make_int4(-1, -1, -1, -1);
+                my_bk[k] = my_bl[k] = my_plq[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                my_a2f[k] = make_int2(0, 0);
+                my_v[k] = make_float3(0.f, 0.f, 0.f);
+            }
+        }
+        __syncthreads();
+
+        const int nsteps = A.steps_per_launch;
+        double E6[6];
+        double fmax_cur = st.fmax;
+        int step_in_launch = 0;
+        for (;;) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) E6[q] = 0.0;
+            // ---------------- phase A: bonds + hneigh ----------------
+            float3 fb[APT];
+#pragma unroll
+            for (int k = 0; k < APT; k++) {
+                const int i = tid + k * BLOCK;
+                fb[k] = make_float3(0.f, 0.f, 0.f);
+                if (i < N) {
+                    const float3 pi = f3(s_pos[i]);
+                    const int ngs[4] = {my_ng[k].x, my_ng[k].y, my_ng[k].z, my_ng[k].w};
+                    const float bks[4] = {my_bk[k].x, my_bk[k].y, my_bk[k].z, my_bk[k].w};
+                    const float bls[4] = {my_bl[k].x, my_bl[k].y, my_bl[k].z, my_bl[k].w};
+                    float eb = 0.f;
+#pragma unroll
+                    for (int sl = 0; sl < 4; sl++) {
+                        if (ngs[sl] < 0) {
+                            s_h[4 * i + sl] = make_float4(0.f, 0.f, 0.f, 1e6f);
+                            continue;
+                        }
+                        float4 hn;
+                        eb += bond_term(pi, f3(s_pos[ngs[sl]]), bks[sl], bls[sl], hn, fb[k]);
+                        s_h[4 * i + sl] = hn;
+                    }
+                    E6[0] += (double)eb;
+                }
+            }
+            __syncthreads();
+            // ---------------- phase B: angles, dihedrals, inversions ----------------
+            {
+                const int i0inv = 4 * td.ndih, i0ang = i0inv + 4 * td.ninv;
+                for (int a = tid; a < td.nangles; a += BLOCK) {
+                    const int2 g = T.ang_ng[td.ang_off + a];
+                    float3 fpi, fpj, fpk;
+                    float e = angle_term(s_h[g.x], s_h[g.y], T.ang_par[td.ang_off + a], fpi, fpj, fpk);
+                    const int p = i0ang + 3 * a;
+                    fint_x[p] = fpi.x; fint_y[p] = fpi.y; fint_z[p] = fpi.z;
+                    fint_x[p + 1] = fpj.x; fint_y[p + 1] = fpj.y; fint_z[p + 1] = fpj.z;
+                    fint_x[p + 2] = fpk.x; fint_y[p + 2] = fpk.y; fint_z[p + 2] = fpk.z;
+                    E6[1] += (double)e;
+                }
+                for (int d = tid; d < td.ndih; d += BLOCK) {
+                    const int4 g = T.dih_ng[td.dih_off + d];
+                    float3 f1, f2, f3v, f4;
+                    float e = dihedral_term(s_h[g.x], s_h[g.y], s_h[g.z], T.dih_par[td.dih_off + d], g.w, f1, f2, f3v, f4);
+                    const int p = 4 * d;
+                    fint_x[p] = f1.x; fint_y[p] = f1.y; fint_z[p] = f1.z;
+                    fint_x[p + 1] = f2.x; fint_y[p + 1] = f2.y; fint_z[p + 1] = f2.z;
+                    fint_x[p + 2] = f3v.x; fint_y[p + 2] = f3v.y; fint_z[p + 2] = f3v.z;
+                    fint_x[p + 3] = f4.x; fint_y[p + 3] = f4.y; fint_z[p + 3] = f4.z;
+                    E6[2] += (double)e;
+                }
+                for (int v = tid; v < td.ninv; v += BLOCK) {
+                    const int4 g = T.inv_ng[td.inv_off + v];
+                    float3 f1, f2, f3v, f4;
+                    float e = inversion_term(s_h[g.x], s_h[g.y], s_h[g.z], T.inv_par[td.inv_off + v], f1, f2, f3v, f4);
+                    const int p = i0inv + 4 * v;
+                    fint_x[p] = f1.x; fint_y[p] = f1.y; fint_z[p] = f1.z;
+                    fint_x[p + 1] = f2.x; fint_y[p + 1] = f2.y; fint_z[p + 1] = f2.z;
+                    fint_x[p + 2] = f3v.x; fint_y[p + 2] = f3v.y; fint_z[p + 2] = f3v.z;
+                    fint_x[p + 3] = f4.x; fint_y[p + 3] = f4.y; fint_z[p + 3] = f4.z;
+                    E6[3] += (double)e;
+                }
+            }
+            __syncthreads();
+            // ---------------- phase C: gather pieces + pairs + surface ----------------
+            float3 F[APT];
+#pragma unroll
+            for (int k = 0; k < APT; k++) {
+                const int i = tid + k * BLOCK;
+                F[k] = make_float3(0.f, 0.f, 0.f);
+                if (i < N) {
+                    KahanF3 acc;
+                    acc.init(fb[k]);
+                    const int *idx = T.a2f_idx + td.a2f_off + my_a2f[k].x;
+                    for (int q = 0; q < my_a2f[k].y; q++) {
+                        const int p = idx[q];
+                        acc.add(make_float3(fint_x[p], fint_y[p], fint_z[p]));
+                    }
+                    const float4 pi4 = s_pos[i];
+                    const float3 pi = f3(pi4);
+                    if (T.use_pairs) {
+                        const float4 ri = s_req[i];
+                        const int4 ng = my_ng[k];
+                        double enb = 0.0;
+                        for (int j = 0; j < N; j++) {
+                            if (j == i || j == ng.x || j == ng.y || j == ng.z || j == ng.w) continue;
+                            float3 f;
+                            float e = ljq_pair(f3(s_pos[j]) - pi, ri, s_req[j], T.r2damp, f);
+                            acc.add(f);
+                            enb += (double)e;
+                        }
+                        E6[4] += enb;
+                    }
+                    if (T.grid) {
+                        float3 ul = make_float3(pi.x * T.ginv_x, pi.y * T.ginv_y, pi.z * T.ginv_z);
+                        float4 fe = grid_sample(T.grid, T.gnx, T.gny, T.gnz, sd.iux, sd.iuy, ul, my_plq[k]);
+                        acc.add(make_float3(-fe.x * T.ginv_x, -fe.y * T.ginv_y, -fe.z * T.ginv_z));
+                        E6[5] += (double)fe.w;
+                    }
+                    F[k] = acc.value();
+                }
+            }
+            // ---------------- phase D: integrator ----------------
+            if (A.mode == KMODE_EVAL) {
+#pragma unroll
+                for (int k = 0; k < APT; k++) {
+                    const int i = tid + k * BLOCK;
+                    if (i < N) A.force[abase + i] = make_float4(F[k].x, F[k].y, F[k].z, 0.f);
+                }
+                double f2m = 0.0;
+#pragma unroll
+                for (int k = 0; k < APT; k++)
+                    if (tid + k * BLOCK < N) {
+                        double fx = F[k].x, fy = F[k].y, fz = F[k].z;
+                        f2m = fmax(f2m, fx * fx + fy * fy + fz * fz);
+                    }
+                f2m = wave_max(f2m);
+                if (lane == 0) s_red[wid * 8 + 6] = f2m;
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    double r = wave_sum(E6[q]);
+                    if (lane == 0) s_red[wid * 8 + q] = r;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    double m = 0.0;
+                    for (int w = 0; w < NW; w++) m = fmax(m, s_red[w * 8 + 6]);
+                    for (int q = 0; q < 6; q++) {
+                        double acc = 0.0;
+                        for (int w = 0; w < NW; w++) acc += s_red[w * 8 + q];
+                        st.E[q] = acc;
+                    }
+                    st.fmax = sqrt(m);
+                    A.state[s] = st;
+                }
+                __syncthreads();
+                break;
+            }
+            // per-atom partial sums (FP64)
+            double p_vf = 0.0, p_vv = 0.0, p_ff = 0.0, p_fm = 0.0;
+            if (A.mode == KMODE_FIRE) {
+                const float dtf = (float)st.dt;
+                const double flim = A.fire.flim;
+#pragma unroll
+                for (int k = 0; k < APT; k++) {
+                    if (tid + k * BLOCK < N) {
+                        double fx = F[k].x, fy = F[k].y, fz = F[k].z;
+                        double f2 = fx * fx + fy * fy + fz * fz;
+                        p_fm = fmax(p_fm, f2);
+                        if (flim > 0.0 && f2 > flim * flim) {
+                            float sc = (float)(flim / sqrt(f2));
+                            F[k] = F[k] * sc;
+                            fx = F[k].x; fy = F[k].y; fz = F[k].z;
+                            f2 = fx * fx + fy * fy + fz * fz;
+                        }
+                        p_ff += f2;
+                        my_v[k] = my_v[k] + F[k] * dtf;
+                        double vx = my_v[k].x, vy = my_v[k].y, vz = my_v[k].z;
+                        p_vf += vx * fx + vy * fy + vz * fz;
+                        p_vv += vx * vx + vy * vy + vz * vz;
+                    }
+                }
+            } else {  // KMODE_MD: MolWorld::run_md / move_atom_md
+                const float dtf = (float)A.fire.md_dt;
+                const float cd = (float)A.fire.md_cdamp;
+                const double flim = A.fire.md_flim;
+#pragma unroll
+                for (int k = 0; k < APT; k++) {
+                    const int i = tid + k * BLOCK;
+                    if (i < N) {
+                        double fx = F[k].x, fy = F[k].y, fz = F[k].z;
+                        double vx = my_v[k].x, vy = my_v[k].y, vz = my_v[k].z;
+                        double f2 = fx * fx + fy * fy + fz * fz;
+                        p_vf += vx * fx + vy * fy + vz * fz;
+                        p_ff += f2;
+                        p_fm = fmax(p_fm, f2);
+                        float3 fc = F[k];
+                        if (f2 > flim * flim) fc = fc * (float)(flim / sqrt(f2));
+                        my_v[k] = my_v[k] * cd + fc * dtf;
+                    }
+                }
+            }
+            // block reduction of (vf, vv, ff, fmax^2) and the energy terms
+            {
+                double r0 = wave_sum(p_vf), r1 = wave_sum(p_vv), r2 = wave_sum(p_ff), r3 = wave_max(p_fm);
+                if (NW > 1) {
+                    if (lane == 0) {
+                        s_red[wid * 8 + 0] = r0; s_red[wid * 8 + 1] = r1;
+                        s_red[wid * 8 + 2] = r2; s_red[wid * 8 + 3] = r3;
+                    }
+                    __syncthreads();
+                    r0 = r1 = r2 = r3 = 0.0;
+                    for (int w = 0; w < NW; w++) {
+                        r0 += s_red[w * 8 + 0]; r1 += s_red[w * 8 + 1];
+                        r2 += s_red[w * 8 + 2]; r3 = fmax(r3, s_red[w * 8 + 3]);
+                    }
+                    __syncthreads();
+                }
+                p_vf = r0; p_vv = r1; p_ff = r2; p_fm = r3;
+            }
+            const double dt_old = st.dt;
+            float dt_move = (float)dt_old;
+            bool nonfinite = !(isfinite(p_vf) && isfinite(p_vv) && isfinite(p_ff));
+            if (A.mode == KMODE_FIRE) {
+                // FIRE decision (relax_pentacene_uff.rs:162-181)
+                if (p_vf > 0.0 && p_ff > 1e-30 && p_vv > 1e-30) {
+                    const double a = 1.0 - st.alpha;
+                    const double b = st.alpha * sqrt(p_vv) / sqrt(p_ff);
+                    const float af = (float)a, bf = (float)b;
+#pragma unroll
+                    for (int k = 0; k < APT; k++) my_v[k] = my_v[k] * af + F[k] * bf;
+                    st.npos++;
+                    if (st.npos > A.fire.n_min) {
+                        st.dt = fmin(st.dt * A.fire.f_inc, A.fire.dt_max);
+                        st.alpha *= A.fire.f_alpha;
+                    }
+                } else {
+                    st.npos = 0;
+                    st.dt *= A.fire.f_dec;
+                    st.alpha = A.fire.alpha0;
+#pragma unroll
+                    for (int k = 0; k < APT; k++) my_v[k] = make_float3(0.f, 0.f, 0.f);
+                }
+            } else {
+                dt_move = (float)A.fire.md_dt;
+                if (p_vf < 0.0) {
+                    // v is zeroed after the move (molecular.rs:156-158)
+                }
+            }
+            // move (x += v*dt_old), own atoms only; every lane has finished
+            // reading positions for this step's pairs before any is moved
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < APT; k++) {
+                const int i = tid + k * BLOCK;
+                if (i < N) {
+                    float4 p = s_pos[i];
+                    p.x += my_v[k].x * dt_move;
+                    p.y += my_v[k].y * dt_move;
+                    p.z += my_v[k].z * dt_move;
+                    s_pos[i] = p;
+                }
+            }
+            if (A.mode == KMODE_MD && p_vf < 0.0) {
+#pragma unroll
+                for (int k = 0; k < APT; k++) my_v[k] = make_float3(0.f, 0.f, 0.f);
+            }
+            st.steps++;
+            step_in_launch++;
+            fmax_cur = sqrt(p_fm);
+            bool done = false;
+            if (nonfinite) {
+                st.status = SMR_SYS_NONFINITE;
+                done = true;
+            } else if (A.mode == KMODE_FIRE ? (fmax_cur < A.fire.fconv) : (p_ff < A.fire.fconv * A.fire.fconv)) {
+                st.status = SMR_SYS_CONVERGED;
+                done = true;
+            } else if (st.steps >= A.fire.max_steps) {
+                st.status = SMR_SYS_MAXSTEPS;
+                done = true;
+            }
+            const bool leave = done || step_in_launch >= nsteps;
+            if (leave) {
+                // energies of this (last) force evaluation
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    double r = wave_sum(E6[q]);
+                    if (lane == 0) s_red[wid * 8 + q] = r;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    for (int q = 0; q < 6; q++) {
+                        double acc = 0.0;
+                        for (int w = 0; w < NW; w++) acc += s_red[w * 8 + q];
+                        st.E[q] = acc;
+                    }
+                    st.fmax = fmax_cur;
+                    A.state[s] = st;
+                }
+#pragma unroll
+                for (int k = 0; k < APT; k++) {
+                    const int i = tid + k * BLOCK;
+                    if (i < N) {
+                        A.pos[abase + i] = s_pos[i];
+                        A.vel[abase + i] = make_float4(my_v[k].x, my_v[k].y, my_v[k].z, 0.f);
+                    }
+                }
+                __syncthreads();
+                break;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ===========================================================================
+// plain method: one kernel per term, one system
+// ===========================================================================
+__global__ void k_plain_bonds(PlainArgs P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const TopoDesc td = P.tab.topo[P.topo];
+    if (i >= td.natoms) return;
+    const int ta = td.atom_off + i;
+    const int4 ng = P.tab.ng[ta];
+    const float4 bk = P.tab.bk[ta], bl = P.tab.bl[ta];
+    const int ngs[4] = {ng.x, ng.y, ng.z, ng.w};
+    const float bks[4] = {bk.x, bk.y, bk.z, bk.w}, bls[4] = {bl.x, bl.y, bl.z, bl.w};
+    const float3 pi = f3(P.pos[i]);
+    float3 f = make_float3(0.f, 0.f, 0.f);
+    float eb = 0.f;
+    for (int sl = 0; sl < 4; sl++) {
+        if (ngs[sl] < 0) { P.hneigh[4 * i + sl] = make_float4(0.f, 0.f, 0.f, 1e6f); continue; }
+        float4 hn;
+        eb += bond_term(pi, f3(P.pos[ngs[sl]]), bks[sl], bls[sl], hn, f);
+        P.hneigh[4 * i + sl] = hn;
+    }
+    P.force[i] = make_float4(f.x, f.y, f.z, 0.f);
+    P.eatom[i] = eb;
+}
+
+__global__ void k_plain_angles(PlainArgs P) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const TopoDesc td = P.tab.topo[P.topo];
+    if (a >= td.nangles) return;
+    const int2 g = P.tab.ang_ng[td.ang_off + a];
+    float3 fpi, fpj, fpk;
+    float e = angle_term(P.hneigh[g.x], P.hneigh[g.y], P.tab.ang_par[td.ang_off + a], fpi, fpj, fpk);
+    const int p = 4 * td.ndih + 4 * td.ninv + 3 * a;
+    P.fint[p] = make_float4(fpi.x, fpi.y, fpi.z, e);
+    P.fint[p + 1] = make_float4(fpj.x, fpj.y, fpj.z, 0.f);
+    P.fint[p + 2] = make_float4(fpk.x, fpk.y, fpk.z, 0.f);
+}
+
+__global__ void k_plain_dihedrals(PlainArgs P) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    const TopoDesc td = P.tab.topo[P.topo];
+    if (d >= td.ndih) return;
+    const int4 g = P.tab.dih_ng[td.dih_off + d];
+    float3 f1, f2, f3v, f4;
+    float e = dihedral_term(P.hneigh[g.x], P.hneigh[g.y], P.hneigh[g.z], P.tab.dih_par[td.dih_off + d], g.w, f1, f2, f3v, f4);
+    const int p = 4 * d;
+    P.fint[p] = make_float4(f1.x, f1.y, f1.z, e);
+    P.fint[p + 1] = make_float4(f2.x, f2.y, f2.z, 0.f);
+    P.fint[p + 2] = make_float4(f3v.x, f3v.y, f3v.z, 0.f);
+    P.fint[p + 3] = make_float4(f4.x, f4.y, f4.z, 0.f);
+}
+
+__global__ void k_plain_inversions(PlainArgs P) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const TopoDesc td = P.tab.topo[P.topo];
+    if (v >= td.ninv) return;
+    const int4 g = P.tab.inv_ng[td.inv_off + v];
+    float3 f1, f2, f3v, f4;
+    float e = inversion_term(P.hneigh[g.x], P.hneigh[g.y], P.hneigh[g.z], P.tab.inv_par[td.inv_off + v], f1, f2, f3v, f4);
+    const int p = 4 * td.ndih + 4 * v;
+    P.fint[p] = make_float4(f1.x, f1.y, f1.z, e);
+    P.fint[p + 1] = make_float4(f2.x, f2.y, f2.z, 0.f);
+    P.fint[p + 2] = make_float4(f3v.x, f3v.y, f3v.z, 0.f);
+    P.fint[p + 3] = make_float4(f4.x, f4.y, f4.z, 0.f);
+}
+
+// assemble + pairs + surface (assembleForces_UFF + getNonBond_GridFF_Bspline)
+__global__ void k_plain_assemble(PlainArgs P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const TopoDesc td = P.tab.topo[P.topo];
+    if (i >= td.natoms) return;
+    const int ta = td.atom_off + i;
+    KahanF3 acc;
+    acc.init(f3(P.force[i]));
+    const int2 se = P.tab.a2f[ta];
+    const int *idx = P.tab.a2f_idx + td.a2f_off + se.x;
+    for (int q = 0; q < se.y; q++) acc.add(f3(P.fint[idx[q]]));
+    const float3 pi = f3(P.pos[i]);
+    float enb = 0.f, esurf = 0.f;
+    double enb_d = 0.0;
+    if (P.tab.use_pairs) {
+        const float4 ri = P.tab.req[ta];
+        const int4 ng = P.tab.ng[ta];
+        for (int j = 0; j < td.natoms; j++) {
+            if (j == i || j == ng.x || j == ng.y || j == ng.z || j == ng.w) continue;
+            float3 f;
+            float e = ljq_pair(f3(P.pos[j]) - pi, ri, P.tab.req[td.atom_off + j], P.tab.r2damp, f);
+            acc.add(f);
+            enb_d += (double)e;
+        }
+    }
+    (void)enb;
+    if (P.tab.grid) {
+        float3 ul = make_float3(pi.x * P.tab.ginv_x, pi.y * P.tab.ginv_y, pi.z * P.tab.ginv_z);
+        float4 fe = grid_sample(P.tab.grid, P.tab.gnx, P.tab.gny, P.tab.gnz, P.iux, P.iuy, ul, P.tab.plq[ta]);
+        acc.add(make_float3(-fe.x * P.tab.ginv_x, -fe.y * P.tab.ginv_y, -fe.z * P.tab.ginv_z));
+        esurf = fe.w;
+    }
+    float3 F = acc.value();
+    P.force[i] = make_float4(F.x, F.y, F.z, 0.f);
+    P.epair[i] = enb_d;
+    P.esurf[i] = esurf;
+}
+
+// per-system FIRE reductions: v += F dt, (v.F, |v|^2, |F|^2, max|F|^2) and
+// energy terms, in one workgroup (FP64).
+__global__ void k_plain_fire_reduce(PlainArgs P, double dt, double flim) {
+    const TopoDesc td = P.tab.topo[P.topo];
+    __shared__ double red[16][11];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+    double r[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const float dtf = (float)dt;
+    for (int i = tid; i < td.natoms; i += blockDim.x) {
+        float4 F4 = P.force[i];
+        float3 F = f3(F4);
+        double fx = F.x, fy = F.y, fz = F.z;
+        double f2 = fx * fx + fy * fy + fz * fz;
+        r[3] = fmax(r[3], f2);
+        if (flim > 0.0 && f2 > flim * flim) {
+            F = F * (float)(flim / sqrt(f2));
+            P.force[i] = make_float4(F.x, F.y, F.z, 0.f);
+            fx = F.x; fy = F.y; fz = F.z;
+            f2 = fx * fx + fy * fy + fz * fz;
+        }
+        r[2] += f2;
+        float4 v4 = P.vel[i];
+        float3 v = f3(v4) + F * dtf;
+        P.vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+        double vx = v.x, vy = v.y, vz = v.z;
+        r[0] += vx * fx + vy * fy + vz * fz;
+        r[1] += vx * vx + vy * vy + vz * vz;
+        r[4] += (double)P.eatom[i];
+        r[8] += P.epair[i];
+        r[9] += (double)P.esurf[i];
+    }
+    for (int a = tid; a < td.nangles; a += blockDim.x) r[5] += (double)P.fint[4 * td.ndih + 4 * td.ninv + 3 * a].w;
+    for (int d = tid; d < td.ndih; d += blockDim.x) r[6] += (double)P.fint[4 * d].w;
+    for (int v = tid; v < td.ninv; v += blockDim.x) r[7] += (double)P.fint[4 * td.ndih + 4 * v].w;
+    for (int q = 0; q < 11; q++) r[q] = (q == 3) ? wave_max(r[q]) : wave_sum(r[q]);
+    if (lane == 0)
+        for (int q = 0; q < 11; q++) red[wid][q] = r[q];
+    __syncthreads();
+    if (tid == 0) {
+        double o[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int w = 0; w < nw; w++)
+            for (int q = 0; q < 11; q++) o[q] = (q == 3) ? fmax(o[q], red[w][q]) : o[q] + red[w][q];
+        // out: vf, vv, ff, f2max, Eb, Ea, Ed, Ei, Enb, Esurf
+        for (int q = 0; q < 10; q++) P.cvf[q] = o[q];
+    }
+}
+
+// FIRE velocity mixing / reset and move, coefficients decided on the host.
+__global__ void k_plain_fire_apply(PlainArgs P, float a, float b, int reset, float dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const TopoDesc td = P.tab.topo[P.topo];
+    if (i >= td.natoms) return;
+    float3 v = f3(P.vel[i]);
+    if (reset) v = make_float3(0.f, 0.f, 0.f);
+    else v = v * a + f3(P.force[i]) * b;
+    P.vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+    float4 p = P.pos[i];
+    p.x += v.x * dt;
+    p.y += v.y * dt;
+    p.z += v.z * dt;
+    P.pos[i] = p;
+}
+
+// ===========================================================================
+// host-side launch helpers
+// ===========================================================================
+template <int BLOCK, int APT, bool FL>
+static hipError_t launch_one(const KernelArgs &A, int grid, size_t lds, hipStream_t stream) {
+    hipLaunchKernelGGL((k_relax<BLOCK, APT, FL>), dim3(grid), dim3(BLOCK), lds, stream, A);
+    return hipGetLastError();
+}
+
+template <int BLOCK, int APT, bool FL>
+static const void *kernel_ptr() {
+    return reinterpret_cast<const void *>(&k_relax<BLOCK, APT, FL>);
+}
+
+hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t stream) {
+#define SMR_CASE(B, P)                                                                  \
+    if (L.block == B && L.apt == P) {                                                   \
+        return L.fint_lds ? launch_one<B, P, true>(A, L.grid, L.lds_bytes, stream)      \
+                          : launch_one<B, P, false>(A, L.grid, L.lds_bytes, stream);    \
+    }
+    SMR_CASE(64, 1)
+    SMR_CASE(128, 1)
+    SMR_CASE(256, 1)
+    SMR_CASE(256, 2)
+    SMR_CASE(256, 4)
+#undef SMR_CASE
+    return hipErrorInvalidValue;
+}
+
+const void *relax_kernel_ptr(int block, int apt, bool fint_lds) {
+#define SMR_CASE(B, P) \
+    if (block == B && apt == P) return fint_lds ? kernel_ptr<B, P, true>() : kernel_ptr<B, P, false>();
+    SMR_CASE(64, 1)
+    SMR_CASE(128, 1)
+    SMR_CASE(256, 1)
+    SMR_CASE(256, 2)
+    SMR_CASE(256, 4)
+#undef SMR_CASE
+    return nullptr;
+}
+
+hipError_t launch_plain_step_forces(const PlainArgs &P, int natoms, int nang, int ndih, int ninv, hipStream_t st) {
+    const int B = 64;
+    auto nb = [](int n) { return n > 0 ? (n + 63) / 64 : 1; };
+    hipLaunchKernelGGL(k_plain_bonds, dim3(nb(natoms)), dim3(B), 0, st, P);
+    if (nang > 0) hipLaunchKernelGGL(k_plain_angles, dim3(nb(nang)), dim3(B), 0, st, P);
+    if (ndih > 0) hipLaunchKernelGGL(k_plain_dihedrals, dim3(nb(ndih)), dim3(B), 0, st, P);
+    if (ninv > 0) hipLaunchKernelGGL(k_plain_inversions, dim3(nb(ninv)), dim3(B), 0, st, P);
+    hipLaunchKernelGGL(k_plain_assemble, dim3(nb(natoms)), dim3(B), 0, st, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_reduce(const PlainArgs &P, double dt, double flim, hipStream_t st) {
+    hipLaunchKernelGGL(k_plain_fire_reduce, dim3(1), dim3(256), 0, st, P, dt, flim);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_apply(const PlainArgs &P, int natoms, float a, float b, int reset, float dt, hipStream_t st) {
+    hipLaunchKernelGGL(k_plain_fire_apply, dim3((natoms + 63) / 64), dim3(64), 0, st, P, a, b, reset, dt);
+    return hipGetLastError();
+}
+
+}  // namespace smr

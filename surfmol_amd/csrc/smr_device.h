@@ -1,0 +1,303 @@
+// smr_device.h -- device-side data layout and force-field math shared by the
+// fused relax kernel and the per-term kernels of the plain method.
+//
+// Formulas follow the reference (paths relative to the reference root):
+//   bond       molff/src/uff.rs:440-468      opencl/UFF.cl:141-322
+//   angle      molff/src/uff.rs:471-507      opencl/UFF.cl:326-501
+//   dihedral   molff/src/uff.rs:510-545      opencl/UFF.cl:505-670
+//   inversion  molff/src/uff.rs:548-579      opencl/UFF.cl:674-795
+//   pair       opencl/UFF.cl:57-95 (mixREQ_arithmetic + getLJQH)
+//   surface    opencl/gridff_eval.cl:48-69,234-286; opencl/UFF.cl:1673-1711
+// with the guards of the OpenCL versions where the FP64 CPU code has none
+// (clamped cosines, degenerate-normal checks).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SMR_COULOMB_CONST 14.3996448915f
+
+// ---------------------------------------------------------------------------
+// packed tables (built on the host by capi.cpp)
+// ---------------------------------------------------------------------------
+struct TopoDesc {
+    int natoms, nangles, ndih, ninv;
+    int nf;          // 4*ndih + 4*ninv + 3*nangles force pieces
+    int atom_off;    // into the per-atom pools
+    int ang_off, dih_off, inv_off;
+    int a2f_off;     // into the piece-index pool
+};
+
+struct SysDesc {
+    int topo;        // index into TopoDesc[]
+    int atom_base;   // into the state arrays (pos/vel/force)
+    int iux, iuy;    // grid-node offset of the system origin, wrapped into [0,nx),[0,ny)
+};
+
+// FIRE / MD state kept per system across launches
+struct SysState {
+    double dt, alpha;
+    int npos, steps, status, pad;
+    double fmax;
+    double E[6];     // bond, angle, dihedral, inversion, pair, surface
+};
+
+struct FireConst {
+    double dt0, dt_max, alpha0, f_inc, f_dec, f_alpha, flim, fconv;
+    int n_min, max_steps;
+    // damped MD (mode KMODE_MD)
+    double md_dt, md_flim, md_cdamp;
+};
+
+struct DevTables {
+    const TopoDesc *topo;
+    const SysDesc *sys;
+    // per-atom pools (index TopoDesc.atom_off + i)
+    const int4 *ng;        // neighbour atoms (local), -1 padded
+    const float4 *bk;      // bond k per neighbour slot
+    const float4 *bl;      // bond l0 per neighbour slot
+    const float4 *req;     // (R, sqrtE, Q, 0)
+    const float4 *plq;     // (e^{2aR} sqrtE, e^{aR} sqrtE, Q, 0)
+    const int2 *a2f;       // (start, count) into the piece-index pool (relative to a2f_off)
+    // per-interaction pools
+    const int2 *ang_ng;    // hneigh slots (4*atom+slot), local
+    const float4 *ang_par; // K*(c0, c1, c2, c3)
+    const int4 *dih_ng;    // 3 hneigh slots + multiplicity n
+    const float4 *dih_par; // (V, V*d, V*d*n, 0)
+    const int4 *inv_ng;    // 3 hneigh slots
+    const float4 *inv_par; // K*(C0, C1, C2), 0
+    const int *a2f_idx;    // piece indices (local to the topology's fint layout)
+    // GridFF
+    const float4 *grid;    // [nx*ny*nz] z-fastest (P, L, Q, 0); NULL = no surface
+    int gnx, gny, gnz;
+    float ginv_x, ginv_y, ginv_z;
+    // options
+    float r2damp;
+    int use_pairs;
+};
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float3 f3(float4 a) { return make_float3(a.x, a.y, a.z); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 operator*(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// a*u + b*w (Vec3d::set_lincomb)
+__device__ __forceinline__ float3 lincomb(float a, float3 u, float b, float3 w) {
+    return make_float3(a * u.x + b * w.x, a * u.y + b * w.y, a * u.z + b * w.z);
+}
+
+// Compensated (Kahan) accumulator for one force vector.
+struct KahanF3 {
+    float3 s, c;
+    __device__ __forceinline__ void init(float3 v) { s = v; c = make_float3(0.f, 0.f, 0.f); }
+    __device__ __forceinline__ void add1(float &sv, float &cv, float x) {
+        float y = x - cv;
+        float t = sv + y;
+        cv = (t - sv) - y;
+        sv = t;
+    }
+    __device__ __forceinline__ void add(float3 x) { add1(s.x, c.x, x.x); add1(s.y, c.y, x.y); add1(s.z, c.z, x.z); }
+    __device__ __forceinline__ float3 value() const { return make_float3(s.x - c.x, s.y - c.y, s.z - c.z); }
+};
+
+// ---------------------------------------------------------------------------
+// force-field terms
+// ---------------------------------------------------------------------------
+
+// Bond between atom i (at pi) and neighbour at pj: writes hneigh (h, 1/l),
+// adds 2k(l-l0) h to f, returns k (l-l0)^2.   uff.rs:440-468 / UFF.cl:223-256
+__device__ __forceinline__ float bond_term(float3 pi, float3 pj, float k, float l0, float4 &hn, float3 &f) {
+    float3 dp = pj - pi;
+    float l2 = dot3(dp, dp);
+    l2 = fmaxf(l2, 1e-16f);
+    float l = sqrtf(l2);
+    float inv_l = 1.0f / l;
+    float3 h = dp * inv_l;
+    hn = make_float4(h.x, h.y, h.z, inv_l);
+    float dl = l - l0;
+    f = f + h * (2.0f * k * dl);
+    return k * dl * dl;
+}
+
+// Angle (Fourier, complex powers): uff.rs:471-507 / UFF.cl:398-447.
+// kc = K*(c0, c1, c2, c3).  Energy omits K*c1*cos(theta) as the reference does.
+__device__ __forceinline__ float angle_term(float4 qij, float4 qkj, float4 kc, float3 &fpi, float3 &fpj, float3 &fpk) {
+    float3 a = f3(qij), b = f3(qkj);
+    float3 h = a + b;
+    float c = 0.5f * (dot3(h, h) - 2.0f);
+    c = fminf(fmaxf(c, -1.0f), 1.0f);
+    float s = sqrtf(fmaxf(0.0f, 1.0f - c * c) + 1e-14f);
+    float inv_s = (s > 1e-12f) ? 1.0f / s : 0.0f;
+    float e = kc.x, ff = kc.y;
+    float cx = c * c - s * s, cy = 2.0f * c * s;          // (cos2t, sin2t)
+    e += kc.z * cx;
+    ff += kc.z * cy * inv_s * 2.0f;
+    float c3x = cx * c - cy * s, c3y = cx * s + cy * c;   // (cos3t, sin3t)
+    e += kc.w * c3x;
+    ff += kc.w * c3y * inv_s * 3.0f;
+    float fi = ff * qij.w, fk = ff * qkj.w;
+    float fic = fi * c, fkc = fk * c;
+    fpi = lincomb(fic, a, -fi, b);
+    fpk = lincomb(-fk, a, fkc, b);
+    fpj = lincomb(fk - fic, a, fi - fkc, b);
+    return e;
+}
+
+// Dihedral: uff.rs:510-545 / UFF.cl:563-603.  par = (V, V*d, V*d*n, 0), n = multiplicity.
+__device__ __forceinline__ float dihedral_term(float4 q12, float4 q32, float4 q43, float4 par, int n, float3 &fp1,
+                                               float3 &fp2, float3 &fp3, float3 &fp4) {
+    float3 h12 = f3(q12), h32 = f3(q32), h43 = f3(q43);
+    float3 n123 = cross3(h12, h32);
+    float3 n234 = cross3(h43, h32);
+    float n1 = dot3(n123, n123), n2 = dot3(n234, n234);
+    if (n1 < 1e-30f || n2 < 1e-30f) {
+        fp1 = fp2 = fp3 = fp4 = make_float3(0.f, 0.f, 0.f);
+        return 0.0f;
+    }
+    float il2_123 = 1.0f / n1, il2_234 = 1.0f / n2;
+    float inv_n12 = sqrtf(il2_123 * il2_234);
+    float csx = dot3(n123, n234) * inv_n12;
+    float csy = -dot3(n123, h43) * inv_n12;
+    float cnx = csx, cny = csy;
+    for (int i = 1; i < n; i++) {
+        float tx = cnx * csx - cny * csy;
+        float ty = cnx * csy + cny * csx;
+        cnx = tx; cny = ty;
+    }
+    float e = par.x + par.y * cnx;
+    float f = -par.z * cny;
+    fp1 = n123 * (-f * il2_123 * q12.w);
+    fp4 = n234 * (f * il2_234 * q43.w);
+    float c123 = dot3(h32, h12) * (q32.w / q12.w);
+    float c432 = dot3(h32, h43) * (q32.w / q43.w);
+    fp3 = lincomb(-c123, fp1, -c432 - 1.0f, fp4);
+    fp2 = lincomb(c123 - 1.0f, fp1, c432, fp4);
+    return e;
+}
+
+// Inversion: uff.rs:548-579 / UFF.cl:706-750.  par = K*(C0, C1, C2), 0.
+__device__ __forceinline__ float inversion_term(float4 q21, float4 q31, float4 q41, float4 par, float3 &fp1, float3 &fp2,
+                                                float3 &fp3, float3 &fp4) {
+    float3 h21 = f3(q21), h31 = f3(q31), h41 = f3(q41);
+    float3 n123 = cross3(h21, h31);
+    float nn = dot3(n123, n123);
+    float il123 = 0.0f;
+    if (nn > 1e-16f) {
+        il123 = rsqrtf(nn);
+        n123 = n123 * il123;
+    } else {
+        n123 = make_float3(0.f, 0.f, 0.f);
+    }
+    float s = -dot3(n123, h41);
+    s = fminf(fmaxf(s, -1.0f), 1.0f);
+    float c = sqrtf(fmaxf(0.0f, 1.0f - s * s) + 1e-14f);
+    float c2x = c * c - s * s, c2y = 2.0f * c * s;
+    float e = par.x + par.y * c + par.z * c2x;
+    float f = (c > 1e-7f) ? -(par.y * s + 2.0f * par.z * c2y) / c : 0.0f;
+    float fq41 = f * q41.w;
+    float fi123 = f * il123;
+    float3 tq = lincomb(s * fi123, n123, fi123, h41);
+    fp4 = lincomb(fq41, n123, s * fq41, h41);
+    fp2 = cross3(h31, tq) * q21.w;
+    fp3 = cross3(tq, h21) * q31.w;
+    fp1 = (fp2 + fp3 + fp4) * -1.0f;
+    return e;
+}
+
+// getLJQH (UFF.cl:82-95) after mixREQ_arithmetic (UFF.cl:57-65):
+// R0 = Ri+Rj, E0 = sqrtEi*sqrtEj, Q = Qi*Qj.  dp = pj - pi; returns E and the
+// force on i in f.
+__device__ __forceinline__ float ljq_pair(float3 dp, float4 ri, float4 rj, float r2damp, float3 &f) {
+    float R0 = ri.x + rj.x;
+    float E0 = ri.y * rj.y;
+    float Q = ri.z * rj.z;
+    float r2 = dot3(dp, dp);
+    float ir2_ = 1.0f / (r2 + r2damp);
+    float Ec = SMR_COULOMB_CONST * Q * sqrtf(ir2_);
+    float ir2 = 1.0f / r2;
+    float u2 = R0 * R0 * ir2;
+    float u6 = u2 * u2 * u2;
+    float vdW = u6 * E0;
+    float E = (u6 - 2.0f) * vdW + Ec;
+    float fr = -12.0f * (u6 - 1.0f) * vdW * ir2 - Ec * ir2_;
+    f = dp * fr;
+    return E;
+}
+
+// ---------------------------------------------------------------------------
+// GridFF tricubic B-spline sample: fe3d_pbc_comb (gridff_eval.cl:234-286)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float4 bsp_basis(float u) {
+    const float inv6 = 1.0f / 6.0f;
+    float u2 = u * u, t = 1.0f - u;
+    return make_float4(inv6 * t * t * t, inv6 * (3.0f * u2 * (u - 2.0f) + 4.0f), inv6 * (3.0f * u * (1.0f + u - u2) + 1.0f),
+                       inv6 * u2 * u);
+}
+__device__ __forceinline__ float4 bsp_dbasis(float u) {
+    float u2 = u * u, t = 1.0f - u;
+    return make_float4(-0.5f * t * t, 0.5f * (3.0f * u2 - 4.0f * u), 0.5f * (-3.0f * u2 + 2.0f * u + 1.0f), 0.5f * u2);
+}
+__device__ __forceinline__ float dot4(float4 a, float4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+
+// Sample at grid coordinate u = (ix0 + ul.x, iy0 + ul.y, ul.z), where (ix0,iy0)
+// is the system's integer node offset and ul the local fractional
+// coordinate.  Returns (dE/du, E); zero outside 1 <= iz < nz-2.
+__device__ __forceinline__ float4 grid_sample(const float4 *__restrict__ G, int nx, int ny, int nz, int ix0, int iy0,
+                                              float3 ul, float4 plq) {
+    float fx = floorf(ul.x), fy = floorf(ul.y);
+    int iz = (int)ul.z;  // truncation, as (int)u.z in the reference
+    float tx = ul.x - fx, ty = ul.y - fy, tz = ul.z - (float)iz;
+    if (iz < 1 || iz >= nz - 2) return make_float4(0.f, 0.f, 0.f, 0.f);
+    int ix = ix0 + (int)fx - 1;
+    int iy = iy0 + (int)fy - 1;
+    ix %= nx; if (ix < 0) ix += nx;
+    iy %= ny; if (iy < 0) iy += ny;
+    float4 bz = bsp_basis(tz), dz = bsp_dbasis(tz);
+    float4 by = bsp_basis(ty), dy = bsp_dbasis(ty);
+    float4 bx = bsp_basis(tx), dx = bsp_dbasis(tx);
+    float by_[4] = {by.x, by.y, by.z, by.w}, dy_[4] = {dy.x, dy.y, dy.z, dy.w};
+    float bx_[4] = {bx.x, bx.y, bx.z, bx.w}, dx_[4] = {dx.x, dx.y, dx.z, dx.w};
+    float ox = 0.f, oy = 0.f, oz = 0.f, oe = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        int xi = ix + a; if (xi >= nx) xi -= nx;
+        float e_y = 0.f, e_dy = 0.f, e_dz = 0.f;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            int yi = iy + b; if (yi >= ny) yi -= ny;
+            const float4 *col = G + ((size_t)(xi * ny + yi) * nz + (iz - 1));
+            float4 c0 = col[0], c1 = col[1], c2 = col[2], c3 = col[3];
+            float4 cs = make_float4(dot4(plq, c0), dot4(plq, c1), dot4(plq, c2), dot4(plq, c3));
+            float fe = dot4(bz, cs);   // fe1Dcomb
+            float fd = dot4(dz, cs);
+            e_dy += fe * dy_[b];       // fe2d_comb
+            e_dz += fd * by_[b];
+            e_y += fe * by_[b];
+        }
+        ox += dx_[a] * e_y;            // fe3d_pbc_comb
+        oy += bx_[a] * e_dy;
+        oz += bx_[a] * e_dz;
+        oe += bx_[a] * e_y;
+    }
+    return make_float4(ox, oy, oz, oe);
+}
+
+// ---------------------------------------------------------------------------
+// reductions (deterministic: xor butterflies give bitwise-identical results
+// in every lane; cross-wave partials are summed in wave order)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}

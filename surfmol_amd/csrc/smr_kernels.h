@@ -1,0 +1,77 @@
+// smr_kernels.h -- launch interface between capi.cpp and relax_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/surfmol_relax.h"
+#include "smr_device.h"
+
+namespace smr {
+
+enum { KMODE_FIRE = 0, KMODE_EVAL = 1, KMODE_MD = 2 };
+
+struct KernelArgs {
+    DevTables tab;
+    FireConst fire;
+    float4 *pos;             // [total atoms] local coordinates
+    float4 *vel;
+    float4 *force;           // written in KMODE_EVAL
+    SysState *state;         // [nsys]
+    const int *work_list;    // systems to process
+    int nwork;
+    int *work_counter;       // atomic fetch counter (zeroed before launch)
+    float *fint_scratch;     // [grid * 3 * nf_max] when pieces do not fit LDS
+    int nf_max;
+    int lds_atoms;           // atoms reserved per LDS slot
+    int steps_per_launch;
+    int mode;
+};
+
+struct LaunchShape {
+    int block, apt, grid;
+    bool fint_lds;
+    size_t lds_bytes;
+};
+
+// bytes of dynamic LDS for a launch shape
+inline size_t relax_lds_bytes(int block, int lds_atoms, int nf_max, bool fint_lds) {
+    size_t b = (size_t)lds_atoms * 16 * 6 + (size_t)(block / 64) * 8 * 8 + 16;
+    if (fint_lds) b += (size_t)nf_max * 12;
+    return b;
+}
+
+struct PlainArgs {
+    DevTables tab;
+    int topo;
+    int iux, iuy;
+    float4 *pos, *vel, *force;   // this system's slices
+    float4 *hneigh;              // [4N]
+    float4 *fint;                // [nf] (xyz, energy in piece 0 of each term)
+    float *eatom;                // [N] bond energy per atom
+    double *epair;               // [N]
+    float *esurf;                // [N]
+    double *cvf;                 // [10] reduction output
+};
+
+hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t stream);
+const void *relax_kernel_ptr(int block, int apt, bool fint_lds);
+hipError_t launch_plain_step_forces(const PlainArgs &P, int natoms, int nang, int ndih, int ninv, hipStream_t st);
+hipError_t launch_plain_reduce(const PlainArgs &P, double dt, double flim, hipStream_t st);
+hipError_t launch_plain_apply(const PlainArgs &P, int natoms, float a, float b, int reset, float dt, hipStream_t st);
+
+// GridFF build / fit (gridff_kernels.hip)
+struct GridBuildArgs {
+    int nx, ny, nz;
+    double p0[3];
+    double L[3];             // cell lengths (nx*step, ...)
+    int npbc;
+    double alpha, r2damp;
+    int nsub;
+    const double *sub_pos;   // [nsub*3]
+    const double *sub_req;   // [nsub*4]
+    double *out;             // [nx*ny*nz*4] z-fastest
+};
+hipError_t launch_gridff_build(const GridBuildArgs &G, hipStream_t st);
+hipError_t launch_bspline_fit(double *E, int nx, int ny, int nz, double *work, hipStream_t st);
+hipError_t launch_pack_f32(const double *C, float *out, size_t n4, hipStream_t st);
+
+}  // namespace smr
