@@ -143,6 +143,10 @@ int smr_batch_destroy(smr_batch *b);
 int smr_batch_natoms(const smr_batch *b, int *total_atoms);
 int smr_batch_atom_offsets(const smr_batch *b, int *offsets);
 int smr_batch_set_positions(smr_batch *b, const double *apos);
+/* Restore the positions given at smr_batch_create (kept on the device) and
+ * zero the velocities, without any host transfer.  No counterpart in the
+ * reference; used to repeat a relaxation from the same inputs. */
+int smr_batch_reset(smr_batch *b);
 int smr_batch_get_positions(const smr_batch *b, double *apos);
 /* Per-step parity hook: forces [sum N,3] and per-system energy terms
  * [nsys,6] = (bond, angle, dihedral, inversion, nonbonded, surface), with the

@@ -28,7 +28,7 @@ from . import _lib
 from ._lib import (DATA_DIR, SMR_SYS_CONVERGED, SMR_SYS_MAXSTEPS, SMR_SYS_NONFINITE, SMR_SYS_RUNNING, SmrError,
                    check, lib)
 
-__all__ = ["Molecule", "Context", "GridFF", "Batch", "FireParams", "SmrError", "DATA_DIR", "SMR_SYS_CONVERGED",
+__all__ = ["Molecule", "Context", "GridFF", "GridSpec", "Batch", "FireParams", "substrate_reqs", "SmrError", "DATA_DIR", "SMR_SYS_CONVERGED",
            "SMR_SYS_MAXSTEPS", "SMR_SYS_NONFINITE", "SMR_SYS_RUNNING", "lib"]
 
 _P = ctypes.POINTER
@@ -125,6 +125,17 @@ class Molecule:
         }
 
 
+def substrate_reqs(elems, charges, param_dir=DATA_DIR):
+    """REQH rows (RvdW, sqrt(EvdW), Q, Hb) of isolated substrate atoms
+    (params.rs:595-604 lookup by element type), with the given charges."""
+    pos = np.zeros((len(elems), 3))
+    pos[:, 0] = np.arange(len(elems)) * 10.0
+    m = Molecule.build(list(elems), pos, charges, rcut_bond=0.0, param_dir=param_dir)
+    r = m.tables()["reqs"]
+    r[:, 3] = 0.0
+    return r
+
+
 class Context:
     def __init__(self, device: int = 0):
         h = ctypes.c_void_p()
@@ -151,7 +162,7 @@ class GridSpec:
     p0: tuple = (0.0, 0.0, 0.0)
     step: tuple = (0.1, 0.1, 0.1)
     alpha_morse: float = 1.5
-    rdamp: float = 0.1
+    rdamp: float = 1.0
 
     def desc(self, keep_reference=False):
         d = _lib.SmrGridDesc()
@@ -239,7 +250,7 @@ def _results(arr):
 class Batch:
     """A set of systems on the device (shared topologies, per-system state)."""
 
-    def __init__(self, ctx: Context, molecules, sys_topo, apos, grid: GridFF | None = None, rdamp=0.1,
+    def __init__(self, ctx: Context, molecules, sys_topo, apos, grid: GridFF | None = None, rdamp=1.0,
                  nonbonded=True, surface=True):
         self.ctx = ctx
         self.grid = grid
@@ -269,6 +280,10 @@ class Batch:
     def set_positions(self, apos):
         x = np.ascontiguousarray(apos, dtype=np.float64).reshape(-1, 3)
         check(lib().smr_batch_set_positions(self._h, _dptr(x)))
+
+    def reset(self):
+        """Back to the initial positions, zero velocities (device-side)."""
+        check(lib().smr_batch_reset(self._h))
 
     def eval_forces(self):
         """(F[natoms,3], E[nsys,6]) at the current positions."""

@@ -81,6 +81,7 @@ _SIGNATURES = [
     ("smr_batch_atom_offsets", c_int, [c_void_p, P(c_int)]),
     ("smr_batch_set_positions", c_int, [c_void_p, P(c_double)]),
     ("smr_batch_get_positions", c_int, [c_void_p, P(c_double)]),
+    ("smr_batch_reset", c_int, [c_void_p]),
     ("smr_batch_eval_forces", c_int, [c_void_p, P(c_double), P(c_double)]),
     ("smr_relax_batched", c_int, [c_void_p, P(SmrFireParams), c_int, c_double, P(SmrRelaxResult)]),
     ("smr_relax_plain", c_int, [c_void_p, c_int, P(SmrFireParams), c_int, c_double, P(SmrRelaxResult)]),

@@ -465,6 +465,10 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
     fill_tables(b->dp, b->grid, b->opts, b->tab);
     int rc = smr_batch_set_positions(b, apos);
     if (rc) { delete b; return rc; }
+    e = b->pos0.alloc(sizeof(float4) * b->total_atoms);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->pos0.ptr, b->pos.ptr, sizeof(float4) * b->total_atoms, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) { delete b; return hip_fail(e, "smr_batch_create: initial positions"); }
     std::vector<int> wl(nsys);
     for (int s = 0; s < nsys; s++) wl[s] = s;
     e = hipMemcpyAsync(b->work.ptr, wl.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, c->stream);
@@ -779,5 +783,15 @@ extern "C" int smr_batch_last_launch(const smr_batch *b, int *shape4, float *ker
     shape4[2] = b->last_shape.grid;
     shape4[3] = (int)b->last_shape.lds_bytes;
     *kernel_ms = b->last_kernel_ms;
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_reset(smr_batch *b) {
+    if (!b) return fail(SMR_ERR_INVALID, "smr_batch_reset: NULL argument");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipMemcpyAsync(b->pos.ptr, b->pos0.ptr, sizeof(float4) * b->total_atoms, hipMemcpyDeviceToDevice,
+                          b->ctx->stream));
+    HIPCHK(hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, b->ctx->stream));
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
     return SMR_OK;
 }

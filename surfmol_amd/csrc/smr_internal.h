@@ -84,7 +84,7 @@ struct smr_batch {
     int nsys = 0, total_atoms = 0, max_atoms = 0, nf_max = 0;
     std::vector<int> atom_off;       // [nsys+1]
     std::vector<double> origin;      // [3*nsys]
-    smr::DevBuf pos, vel, force, state, work, fint_scratch;
+    smr::DevBuf pos, pos0, vel, force, state, work, fint_scratch;
     smr::DevBuf plain_hneigh, plain_fint, plain_e, plain_cvf;
     smr::LaunchShape last_shape;
     float last_kernel_ms = 0.f;
