@@ -638,6 +638,27 @@ int orc_run_md(const orc_system *s, double *apos, double *vapos, int niter, doub
     return ret;
 }
 
+/* n plain MD steps of DynamicAtoms::move_atom_md (moltopo/src/molecular.rs:
+ * 118-137) with no velocity purge and no convergence test, as driven by the
+ * invariants loop of surfmol/tests/relax_xylitol_uff.rs:183-198.           */
+void orc_md_steps(const orc_system *s, double *apos, double *vapos, int nsteps, double dt, double flim, double cdamp) {
+    int na = s->natoms;
+    double *f = (double *)malloc(sizeof(double) * 3 * (size_t)na);
+    for (int it = 0; it < nsteps; it++) {
+        double E6[6];
+        orc_eval_forces(s, apos, f, E6);
+        for (int i = 0; i < na; i++) {
+            v3 fi = v3_ld(f + 3 * i), v = v3_ld(vapos + 3 * i), p = v3_ld(apos + 3 * i);
+            double f2 = v3_dot(fi, fi);
+            if (f2 > flim * flim) fi = v3_mul(fi, flim / sqrt(f2));
+            v3 vn = v3_add(v3_mul(v, cdamp), v3_mul(fi, dt));
+            v3_st(vapos + 3 * i, vn);
+            v3_st(apos + 3 * i, v3_add(p, v3_mul(vn, dt)));
+        }
+    }
+    free(f);
+}
+
 /* ------------------------------------------------------------------ */
 /* GridFF reference-grid build and B-spline fit                         */
 /* ------------------------------------------------------------------ */

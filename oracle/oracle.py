@@ -100,6 +100,7 @@ def lib():
         L.orc_relax_fire.restype = i
         L.orc_run_md.argtypes = [P(OrcSystem), P(d), P(d), i, d, d, d, d]
         L.orc_run_md.restype = i
+        L.orc_md_steps.argtypes = [P(OrcSystem), P(d), P(d), i, d, d, d]
         L.orc_make_second_neighs.argtypes = [i, P(i), P(i)]
         L.orc_make_second_neighs.restype = i
         L.orc_fe3d_pbc_comb.argtypes = [P(d), i, i, i, P(d), P(d), P(d)]
@@ -855,6 +856,15 @@ def run_md(sys: OracleSystem, niter, dt, fconv, flim, damping, apos=None, vapos=
     steps = lib().orc_run_md(ctypes.byref(cs), _pd(x), _pd(v), int(niter), float(dt), float(fconv), float(flim),
                              float(damping))
     return steps, x, v
+
+
+def md_steps(sys: OracleSystem, nsteps, dt, flim, cdamp, apos, vapos):
+    """Plain move_atom_md steps (no purge): returns (apos, vapos)."""
+    cs = sys.cstruct()
+    x = _f64(apos, (-1, 3)).copy()
+    v = _f64(vapos, (-1, 3)).copy()
+    lib().orc_md_steps(ctypes.byref(cs), _pd(x), _pd(v), int(nsteps), float(dt), float(flim), float(cdamp))
+    return x, v
 
 
 def fe3d_pbc_comb(u, grid, ns, plqh):
