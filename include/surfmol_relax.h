@@ -192,8 +192,16 @@ int smr_relax_batched(smr_batch *b, const smr_fire_params *fire, int max_steps, 
 int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fire, int max_steps, double fconv,
                     smr_relax_result *result);
 
+/* Launch variant of the fused kernel for this batch: threads per workgroup
+ * (64/128/256; 0 = automatic by system size) and the per-lane register cache
+ * of each atom's surface stencil (1 on, 0 off, -1 automatic).  Results are
+ * identical across variants; only speed differs.  No counterpart in the
+ * reference (tuning knob of this implementation). */
+int smr_batch_set_launch(smr_batch *b, int block, int stencil_cache);
+
 /* Launch shape and device time of the last fused launch on this batch:
- * shape4 = (threads per workgroup, atoms per thread, workgroups, LDS bytes);
+ * shape4 = (threads per workgroup, atoms per thread, workgroups, LDS bytes
+ * with bit 30 set when the stencil cache was on);
  * kernel_ms = HIP-event time of that launch on the context's stream (only
  * measured by smr_relax_batched).  No counterpart in the reference. */
 int smr_batch_last_launch(const smr_batch *b, int *shape4, float *kernel_ms);

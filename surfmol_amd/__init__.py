@@ -310,11 +310,16 @@ class Batch:
         check(lib().smr_run_md(self._h, int(niter), float(dt), float(fconv), float(flim), float(damping), _iptr(steps)))
         return steps
 
+    def set_launch(self, block=0, stencil_cache=-1):
+        """Kernel variant (threads/workgroup, surface stencil cache); 0/-1 = automatic."""
+        check(lib().smr_batch_set_launch(self._h, int(block), int(stencil_cache)))
+
     def last_launch(self):
         shape = (ctypes.c_int * 4)()
         ms = ctypes.c_float()
         check(lib().smr_batch_last_launch(self._h, shape, ctypes.byref(ms)))
-        return {"block": shape[0], "apt": shape[1], "grid": shape[2], "lds_bytes": shape[3], "kernel_ms": ms.value}
+        return {"block": shape[0], "apt": shape[1], "grid": shape[2], "lds_bytes": shape[3] & ((1 << 30) - 1),
+                "stencil_cache": bool(shape[3] >> 30), "kernel_ms": ms.value}
 
     def close(self):
         if self._h:

@@ -91,13 +91,26 @@ extern "C" int smr_ctx_create(int device, smr_ctx **out) {
     return SMR_OK;
 }
 
+namespace smr {
+// A context lives until its handle has been destroyed and every grid, batch
+// and server made from it is gone, whatever the order of the destroy calls.
+void ctx_retain(smr_ctx *c) { c->refs++; }
+
+void ctx_release(smr_ctx *c) {
+    if (--c->refs > 0) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->counter.release();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+}  // namespace smr
+
 extern "C" int smr_ctx_destroy(smr_ctx *c) {
     if (!c) return SMR_OK;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
-    c->counter.release();
-    hipStreamDestroy(c->stream);
-    delete c;
+    if (c->destroy_requested) return fail(SMR_ERR_STATE, "smr_ctx_destroy: context already destroyed");
+    c->destroy_requested = true;
+    ctx_release(c);
     return SMR_OK;
 }
 
@@ -123,11 +136,12 @@ extern "C" int smr_gridff_upload(smr_ctx *c, const smr_grid_desc *d, const float
     HIPCHK(hipSetDevice(c->device));
     auto *g = new smr_gridff();
     g->ctx = c;
+    ctx_retain(c);
     g->desc = *d;
     size_t n = (size_t)d->nx * d->ny * d->nz;
     hipError_t e = g->coef.upload(coefs, n * 4 * sizeof(float), c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_upload"); }
+    if (e != hipSuccess) { delete g; ctx_release(c); return hip_fail(e, "smr_gridff_upload"); }
     *out = g;
     return SMR_OK;
 }
@@ -144,13 +158,14 @@ extern "C" int smr_gridff_build(smr_ctx *c, const smr_grid_desc *d, int nsub, co
     HIPCHK(hipSetDevice(c->device));
     auto *g = new smr_gridff();
     g->ctx = c;
+    ctx_retain(c);
     g->desc = *d;
     const size_t n = (size_t)d->nx * d->ny * d->nz;
     DevBuf dpos, dreq, E;
     hipError_t e = dpos.upload(sub_pos, sizeof(double) * 3 * nsub, c->stream);
     if (e == hipSuccess) e = dreq.upload(sub_reqs, sizeof(double) * 4 * nsub, c->stream);
     if (e == hipSuccess) e = E.alloc(sizeof(double) * 4 * n);
-    if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_build: allocation"); }
+    if (e != hipSuccess) { delete g; ctx_release(c); return hip_fail(e, "smr_gridff_build: allocation"); }
     GridBuildArgs G;
     G.nx = d->nx; G.ny = d->ny; G.nz = d->nz;
     for (int k = 0; k < 3; k++) G.p0[k] = d->p0[k];
@@ -166,19 +181,19 @@ extern "C" int smr_gridff_build(smr_ctx *c, const smr_grid_desc *d, int nsub, co
     G.out = (double *)E.ptr;
     e = launch_gridff_build(G, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_build: build kernel"); }
+    if (e != hipSuccess) { delete g; ctx_release(c); return hip_fail(e, "smr_gridff_build: build kernel"); }
     if (d->keep_reference) {
         e = g->reference.alloc(sizeof(double) * 4 * n);
         if (e == hipSuccess)
             e = hipMemcpyAsync(g->reference.ptr, E.ptr, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice, c->stream);
-        if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_build: keep_reference"); }
+        if (e != hipSuccess) { delete g; ctx_release(c); return hip_fail(e, "smr_gridff_build: keep_reference"); }
     }
     e = launch_bspline_fit((double *)E.ptr, d->nx, d->ny, d->nz, nullptr, c->stream);
-    if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_build: B-spline fit"); }
+    if (e != hipSuccess) { delete g; ctx_release(c); return hip_fail(e, "smr_gridff_build: B-spline fit"); }
     e = g->coef.alloc(sizeof(float) * 4 * n);
     if (e == hipSuccess) e = launch_pack_f32((const double *)E.ptr, (float *)g->coef.ptr, n, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) { delete g; return hip_fail(e, "smr_gridff_build: pack"); }
+    if (e != hipSuccess) { delete g; ctx_release(c); return hip_fail(e, "smr_gridff_build: pack"); }
     *out = g;
     return SMR_OK;
 }
@@ -201,8 +216,11 @@ extern "C" int smr_gridff_download_reference(const smr_gridff *g, double *values
 
 extern "C" int smr_gridff_destroy(smr_gridff *g) {
     if (!g) return SMR_OK;
-    hipSetDevice(g->ctx->device);
+    smr_ctx *c = g->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     delete g;
+    ctx_release(c);
     return SMR_OK;
 }
 
@@ -225,6 +243,8 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
     td.ndih = t.ndih;
     td.ninv = t.ninv;
     td.nf = 4 * t.ndih + 4 * t.ninv + 3 * t.nangles;
+    if (td.nf > 65535 || 4 * N > 65535)
+        return fail(SMR_ERR_UNSUPPORTED, who + std::to_string(td.nf) + " force pieces; 16-bit gather lists hold at most 65535");
     td.atom_off = (int)P.ng.size();
     td.ang_off = (int)P.ang_ng.size();
     td.dih_off = (int)P.dih_ng.size();
@@ -252,7 +272,8 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
         P.bk.push_back(make_float4(bk[0], bk[1], bk[2], bk[3]));
         P.bl.push_back(make_float4(bl[0], bl[1], bl[2], bl[3]));
         const double *r = t.reqs + 4 * i;
-        P.req.push_back(make_float4((float)r[0], (float)r[1], (float)r[2], 0.f));
+        // pair-term row: (R, sqrtE, sqrt(kC)*Q) so that qi*qj = kC*Qi*Qj
+        P.req.push_back(make_float4((float)r[0], (float)r[1], (float)(r[2] * std::sqrt(14.3996448915)), 0.f));
         // PLQH (opencl/UFF.cl:1681-1687), computed in FP64
         if (have_grid) {
             double ej = std::exp(alpha * r[0]);
@@ -269,7 +290,7 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
             return fail(SMR_ERR_INVALID, who + "angle " + std::to_string(a) + " has hneigh slots (" + std::to_string(g0) +
                                              "," + std::to_string(g1) + ") that are not bonds");
         const double *p = t.ang_params + 5 * a;
-        P.ang_ng.push_back(make_int2(g0, g1));
+        P.ang_ng.push_back((unsigned)g0 | ((unsigned)g1 << 16));
         P.ang_par.push_back(make_float4((float)(p[0] * p[1]), (float)(p[0] * p[2]), (float)(p[0] * p[3]), (float)(p[0] * p[4])));
     }
     // ---- dihedrals ----
@@ -280,7 +301,7 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
         const double *p = t.dih_params + 3 * d;
         int n = (int)p[2];
         if (n < 1 || n > 12) return fail(SMR_ERR_INVALID, who + "dihedral " + std::to_string(d) + " multiplicity " + std::to_string(n));
-        P.dih_ng.push_back(make_int4(g0, g1, g2, n));
+        P.dih_ng.push_back(make_uint2((unsigned)g0 | ((unsigned)g1 << 16), (unsigned)g2 | ((unsigned)n << 16)));
         P.dih_par.push_back(make_float4((float)p[0], (float)(p[0] * p[1]), (float)(p[0] * p[1] * p[2]), 0.f));
     }
     // ---- inversions ----
@@ -289,7 +310,7 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
         if (!slot_ok(g0) || !slot_ok(g1) || !slot_ok(g2))
             return fail(SMR_ERR_INVALID, who + "inversion " + std::to_string(v) + " has an invalid hneigh slot");
         const double *p = t.inv_params + 4 * v;
-        P.inv_ng.push_back(make_int4(g0, g1, g2, 0));
+        P.inv_ng.push_back(make_uint2((unsigned)g0 | ((unsigned)g1 << 16), (unsigned)g2));
         P.inv_par.push_back(make_float4((float)(p[0] * p[1]), (float)(p[0] * p[2]), (float)(p[0] * p[3]), 0.f));
     }
     // ---- a2f gather lists (Uff::map_atom_interactions, uff.rs:317-370):
@@ -315,15 +336,15 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
             cnt[a]++;
         }
     for (int i = 0; i < N; i++) off[i + 1] = off[i] + cnt[i];
-    std::vector<int> idx(off[N]);
+    std::vector<unsigned short> idx(off[N]);
     std::fill(cnt.begin(), cnt.end(), 0);
     const int i0inv = 4 * t.ndih, i0ang = i0inv + 4 * t.ninv;
     for (int d = 0; d < t.ndih; d++)
-        for (int k = 0; k < 4; k++) { int a = t.dih_atoms[4 * d + k]; idx[off[a] + cnt[a]++] = 4 * d + k; }
+        for (int k = 0; k < 4; k++) { int a = t.dih_atoms[4 * d + k]; idx[off[a] + cnt[a]++] = (unsigned short)(4 * d + k); }
     for (int v = 0; v < t.ninv; v++)
-        for (int k = 0; k < 4; k++) { int a = t.inv_atoms[4 * v + k]; idx[off[a] + cnt[a]++] = i0inv + 4 * v + k; }
+        for (int k = 0; k < 4; k++) { int a = t.inv_atoms[4 * v + k]; idx[off[a] + cnt[a]++] = (unsigned short)(i0inv + 4 * v + k); }
     for (int a3 = 0; a3 < t.nangles; a3++)
-        for (int k = 0; k < 3; k++) { int a = t.ang_atoms[3 * a3 + k]; idx[off[a] + cnt[a]++] = i0ang + 3 * a3 + k; }
+        for (int k = 0; k < 3; k++) { int a = t.ang_atoms[3 * a3 + k]; idx[off[a] + cnt[a]++] = (unsigned short)(i0ang + 3 * a3 + k); }
     for (int i = 0; i < N; i++) P.a2f.push_back(make_int2(off[i], off[i + 1] - off[i]));
     P.a2f_idx.insert(P.a2f_idx.end(), idx.begin(), idx.end());
     return SMR_OK;
@@ -347,13 +368,13 @@ void fill_tables(const DevPools &D, const smr_gridff *grid, const smr_ff_options
     T.req = (const float4 *)D.req.ptr;
     T.plq = (const float4 *)D.plq.ptr;
     T.a2f = (const int2 *)D.a2f.ptr;
-    T.ang_ng = (const int2 *)D.ang_ng.ptr;
+    T.ang_ng = (const unsigned *)D.ang_ng.ptr;
     T.ang_par = (const float4 *)D.ang_par.ptr;
-    T.dih_ng = (const int4 *)D.dih_ng.ptr;
+    T.dih_ng = (const uint2 *)D.dih_ng.ptr;
     T.dih_par = (const float4 *)D.dih_par.ptr;
-    T.inv_ng = (const int4 *)D.inv_ng.ptr;
+    T.inv_ng = (const uint2 *)D.inv_ng.ptr;
     T.inv_par = (const float4 *)D.inv_par.ptr;
-    T.a2f_idx = (const int *)D.a2f_idx.ptr;
+    T.a2f_idx = (const unsigned short *)D.a2f_idx.ptr;
     if (grid && o.surface) {
         T.grid = (const float4 *)grid->coef.ptr;
         T.gnx = grid->desc.nx; T.gny = grid->desc.ny; T.gnz = grid->desc.nz;
@@ -391,18 +412,28 @@ void system_origin(const double *x, int n, const smr_gridff *grid, bool surface,
     }
 }
 
-LaunchShape choose_shape(const smr_ctx *c, int max_atoms, int nf_max, int nwork) {
+LaunchShape choose_shape(const smr_ctx *c, const smr_batch *b, int nwork) {
+    const int max_atoms = b->max_atoms, nf_max = b->nf_max;
     LaunchShape L;
     if (max_atoms <= 64) { L.block = 64; L.apt = 1; }
     else if (max_atoms <= 128) { L.block = 128; L.apt = 1; }
     else if (max_atoms <= 256) { L.block = 256; L.apt = 1; }
     else if (max_atoms <= 512) { L.block = 256; L.apt = 2; }
     else { L.block = 256; L.apt = 4; }
-    size_t with = relax_lds_bytes(L.block, max_atoms, nf_max, true);
+    // caller / autotuner override of the threads per workgroup (atoms per
+    // thread follows) and of the stencil cache
+    if (b->tune_block > 0) {
+        int apt = (max_atoms + b->tune_block - 1) / b->tune_block;
+        if (apt <= 1) apt = 1; else if (apt <= 2) apt = 2; else apt = 4;
+        if (relax_kernel_ptr(b->tune_block, apt, true, false)) { L.block = b->tune_block; L.apt = apt; }
+    }
+    // the register stencil cache halves occupancy (256 VGPRs): off unless asked for
+    L.stencil_cache = (L.apt == 1) && (b->tune_cache == 1) && b->grid;
+    size_t with = relax_lds_bytes(L.block, max_atoms, nf_max, b->na_max, b->nd_max, b->ni_max, true);
     L.fint_lds = with <= std::min<size_t>(c->max_lds, 64 * 1024);
-    L.lds_bytes = relax_lds_bytes(L.block, max_atoms, nf_max, L.fint_lds);
+    L.lds_bytes = relax_lds_bytes(L.block, max_atoms, nf_max, b->na_max, b->nd_max, b->ni_max, L.fint_lds);
     int occ = 0;
-    const void *k = relax_kernel_ptr(L.block, L.apt, L.fint_lds);
+    const void *k = relax_kernel_ptr(L.block, L.apt, L.fint_lds, L.stencil_cache);
     if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, L.block, L.lds_bytes) != hipSuccess || occ < 1) occ = 1;
     long long cap = (long long)occ * c->num_cu;
     L.grid = (int)std::max<long long>(1, std::min<long long>(cap, nwork));
@@ -422,16 +453,20 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
     HIPCHK(hipSetDevice(c->device));
     auto *b = new smr_batch();
     b->ctx = c;
+    ctx_retain(c);
     b->grid = opts->surface ? grid : nullptr;
     b->opts = *opts;
     const double alpha = grid ? grid->desc.alpha_morse : 0.0;
     for (int t = 0; t < ntopo; t++) {
         TopoDesc td;
         int rc = pack_topology(topos[t], t, alpha, b->grid != nullptr, b->hp, td);
-        if (rc) { delete b; return rc; }
+        if (rc) { delete b; ctx_release(c); return rc; }
         b->hp.topo.push_back(td);
         b->max_atoms = std::max(b->max_atoms, td.natoms);
         b->nf_max = std::max(b->nf_max, td.nf);
+        b->na_max = std::max(b->na_max, td.nangles);
+        b->nd_max = std::max(b->nd_max, td.ndih);
+        b->ni_max = std::max(b->ni_max, td.ninv);
     }
     b->nsys = nsys;
     b->atom_off.assign(nsys + 1, 0);
@@ -439,6 +474,7 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
         int t = sys_topo[s];
         if (t < 0 || t >= ntopo) {
             delete b;
+            ctx_release(c);
             return fail(SMR_ERR_INVALID, "smr_batch_create: sys_topo[" + std::to_string(s) + "] = " + std::to_string(t) +
                                              " not in [0," + std::to_string(ntopo) + ")");
         }
@@ -461,28 +497,30 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
     if (e == hipSuccess) e = b->state.alloc(sizeof(SysState) * nsys);
     if (e == hipSuccess) e = b->work.alloc(sizeof(int) * nsys);
     if (e == hipSuccess) e = hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, c->stream);
-    if (e != hipSuccess) { delete b; return hip_fail(e, "smr_batch_create: device allocation"); }
+    if (e != hipSuccess) { delete b; ctx_release(c); return hip_fail(e, "smr_batch_create: device allocation"); }
     fill_tables(b->dp, b->grid, b->opts, b->tab);
     int rc = smr_batch_set_positions(b, apos);
-    if (rc) { delete b; return rc; }
+    if (rc) { delete b; ctx_release(c); return rc; }
     e = b->pos0.alloc(sizeof(float4) * b->total_atoms);
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->pos0.ptr, b->pos.ptr, sizeof(float4) * b->total_atoms, hipMemcpyDeviceToDevice, c->stream);
-    if (e != hipSuccess) { delete b; return hip_fail(e, "smr_batch_create: initial positions"); }
+    if (e != hipSuccess) { delete b; ctx_release(c); return hip_fail(e, "smr_batch_create: initial positions"); }
     std::vector<int> wl(nsys);
     for (int s = 0; s < nsys; s++) wl[s] = s;
     e = hipMemcpyAsync(b->work.ptr, wl.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) { delete b; return hip_fail(e, "smr_batch_create: upload"); }
+    if (e != hipSuccess) { delete b; ctx_release(c); return hip_fail(e, "smr_batch_create: upload"); }
     *out = b;
     return SMR_OK;
 }
 
 extern "C" int smr_batch_destroy(smr_batch *b) {
     if (!b) return SMR_OK;
-    hipSetDevice(b->ctx->device);
-    hipStreamSynchronize(b->ctx->stream);
+    smr_ctx *c = b->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     delete b;
+    ctx_release(c);
     return SMR_OK;
 }
 
@@ -550,7 +588,7 @@ int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch,
         }
         HIPCHK(hipMemcpyAsync(b->state.ptr, h.data(), sizeof(SysState) * h.size(), hipMemcpyHostToDevice, st));
     }
-    LaunchShape L = choose_shape(c, b->max_atoms, b->nf_max, b->nsys);
+    LaunchShape L = choose_shape(c, b, b->nsys);
     if (L.lds_bytes > c->max_lds)
         return fail(SMR_ERR_UNSUPPORTED, "k_relax needs " + std::to_string(L.lds_bytes) + " B of LDS for " +
                                              std::to_string(b->max_atoms) + " atoms; the device allows " +
@@ -575,6 +613,9 @@ int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch,
     A.fint_scratch = (float *)b->fint_scratch.ptr;
     A.nf_max = std::max(1, b->nf_max);
     A.lds_atoms = b->max_atoms;
+    A.lds_na = b->na_max;
+    A.lds_nd = b->nd_max;
+    A.lds_ni = b->ni_max;
     A.steps_per_launch = steps_per_launch;
     A.mode = mode;
     HIPCHK(hipMemsetAsync(c->counter.ptr, 0, sizeof(int), st));
@@ -781,7 +822,7 @@ extern "C" int smr_batch_last_launch(const smr_batch *b, int *shape4, float *ker
     shape4[0] = b->last_shape.block;
     shape4[1] = b->last_shape.apt;
     shape4[2] = b->last_shape.grid;
-    shape4[3] = (int)b->last_shape.lds_bytes;
+    shape4[3] = (int)b->last_shape.lds_bytes | (b->last_shape.stencil_cache ? (1 << 30) : 0);
     *kernel_ms = b->last_kernel_ms;
     return SMR_OK;
 }
@@ -793,5 +834,16 @@ extern "C" int smr_batch_reset(smr_batch *b) {
                           b->ctx->stream));
     HIPCHK(hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, b->ctx->stream));
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_set_launch(smr_batch *b, int block, int stencil_cache) {
+    if (!b) return fail(SMR_ERR_INVALID, "smr_batch_set_launch: NULL argument");
+    if (block != 0 && block != 64 && block != 128 && block != 256)
+        return fail(SMR_ERR_INVALID, "smr_batch_set_launch: block " + std::to_string(block) + " not in {0,64,128,256}");
+    if (stencil_cache < -1 || stencil_cache > 1)
+        return fail(SMR_ERR_INVALID, "smr_batch_set_launch: stencil_cache must be -1, 0 or 1");
+    b->tune_block = block;
+    b->tune_cache = stencil_cache;
     return SMR_OK;
 }

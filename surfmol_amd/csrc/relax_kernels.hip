@@ -21,9 +21,11 @@ namespace smr {
 // ===========================================================================
 // fused persistent kernel
 // ===========================================================================
-template <int BLOCK, int APT, bool FINT_LDS>
+template <int BLOCK, int APT, bool FINT_LDS, bool SCACHE>
 __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
     constexpr int NW = BLOCK / 64;
+    // one wave holds the whole system: unordered-pair sweep
+    constexpr bool HALF_PAIRS = (BLOCK == 64 && APT == 1);
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
@@ -33,9 +35,14 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
     float4 *s_pos = reinterpret_cast<float4 *>(smem);
     float4 *s_req = s_pos + A.lds_atoms;
     float4 *s_h = s_req + A.lds_atoms;                       // [4*natoms] hneigh
-    double *s_red = reinterpret_cast<double *>(s_h + 4 * A.lds_atoms);  // [NW*8]
+    float4 *s_fp = s_h + 4 * A.lds_atoms;                    // [natoms] pair reaction forces
+    double *s_red = reinterpret_cast<double *>(s_fp + A.lds_atoms);     // [NW*8]
     int *s_next = reinterpret_cast<int *>(s_red + NW * 8);
-    float *s_fx = reinterpret_cast<float *>(s_next + 4);
+    uint2 *s_dih = reinterpret_cast<uint2 *>(s_next + 4);                // packed slots
+    uint2 *s_inv = s_dih + A.lds_nd;
+    unsigned *s_ang = reinterpret_cast<unsigned *>(s_inv + A.lds_ni);
+    float *s_fx = reinterpret_cast<float *>(s_ang + A.lds_na);
+    unsigned short *s_a2f = reinterpret_cast<unsigned short *>(s_fx + (FINT_LDS ? 3 * A.nf_max : 0));
     float *fint_x, *fint_y, *fint_z;
 
     while (true) {
@@ -86,8 +93,18 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                 my_v[k] = make_float3(0.f, 0.f, 0.f);
             }
         }
+        // interaction slots and the piece-gather list stay in LDS for all steps
+        for (int q = tid; q < td.nangles; q += BLOCK) s_ang[q] = T.ang_ng[td.ang_off + q];
+        for (int q = tid; q < td.ndih; q += BLOCK) s_dih[q] = T.dih_ng[td.dih_off + q];
+        for (int q = tid; q < td.ninv; q += BLOCK) s_inv[q] = T.inv_ng[td.inv_off + q];
+        for (int q = tid; q < td.nf; q += BLOCK) s_a2f[q] = T.a2f_idx[td.a2f_off + q];
         __syncthreads();
 
+        // per-lane cache of the atom's PLQH-contracted 4x4x4 stencil; re-read
+        // from the grid only when the atom enters another cell
+        constexpr bool CACHE = SCACHE && (APT == 1);
+        float sv[CACHE ? 64 : 1];
+        int skey = -1;
         const int nsteps = A.steps_per_launch;
         double E6[6];
         double fmax_cur = st.fmax;
@@ -102,6 +119,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                 const int i = tid + k * BLOCK;
                 fb[k] = make_float3(0.f, 0.f, 0.f);
                 if (i < N) {
+                    s_fp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
                     const float3 pi = f3(s_pos[i]);
                     const int ngs[4] = {my_ng[k].x, my_ng[k].y, my_ng[k].z, my_ng[k].w};
                     const float bks[4] = {my_bk[k].x, my_bk[k].y, my_bk[k].z, my_bk[k].w};
@@ -125,9 +143,9 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
             {
                 const int i0inv = 4 * td.ndih, i0ang = i0inv + 4 * td.ninv;
                 for (int a = tid; a < td.nangles; a += BLOCK) {
-                    const int2 g = T.ang_ng[td.ang_off + a];
+                    const unsigned g = s_ang[a];
                     float3 fpi, fpj, fpk;
-                    float e = angle_term(s_h[g.x], s_h[g.y], T.ang_par[td.ang_off + a], fpi, fpj, fpk);
+                    float e = angle_term(s_h[g & 0xffffu], s_h[g >> 16], T.ang_par[td.ang_off + a], fpi, fpj, fpk);
                     const int p = i0ang + 3 * a;
                     fint_x[p] = fpi.x; fint_y[p] = fpi.y; fint_z[p] = fpi.z;
                     fint_x[p + 1] = fpj.x; fint_y[p + 1] = fpj.y; fint_z[p + 1] = fpj.z;
@@ -135,9 +153,10 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     E6[1] += (double)e;
                 }
                 for (int d = tid; d < td.ndih; d += BLOCK) {
-                    const int4 g = T.dih_ng[td.dih_off + d];
+                    const uint2 g = s_dih[d];
                     float3 f1, f2, f3v, f4;
-                    float e = dihedral_term(s_h[g.x], s_h[g.y], s_h[g.z], T.dih_par[td.dih_off + d], g.w, f1, f2, f3v, f4);
+                    float e = dihedral_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y & 0xffffu], T.dih_par[td.dih_off + d],
+                                            (int)(g.y >> 16), f1, f2, f3v, f4);
                     const int p = 4 * d;
                     fint_x[p] = f1.x; fint_y[p] = f1.y; fint_z[p] = f1.z;
                     fint_x[p + 1] = f2.x; fint_y[p + 1] = f2.y; fint_z[p + 1] = f2.z;
@@ -146,9 +165,10 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     E6[2] += (double)e;
                 }
                 for (int v = tid; v < td.ninv; v += BLOCK) {
-                    const int4 g = T.inv_ng[td.inv_off + v];
+                    const uint2 g = s_inv[v];
                     float3 f1, f2, f3v, f4;
-                    float e = inversion_term(s_h[g.x], s_h[g.y], s_h[g.z], T.inv_par[td.inv_off + v], f1, f2, f3v, f4);
+                    float e = inversion_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y], T.inv_par[td.inv_off + v], f1, f2,
+                                             f3v, f4);
                     const int p = i0inv + 4 * v;
                     fint_x[p] = f1.x; fint_y[p] = f1.y; fint_z[p] = f1.z;
                     fint_x[p + 1] = f2.x; fint_y[p + 1] = f2.y; fint_z[p + 1] = f2.z;
@@ -158,6 +178,39 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                 }
             }
             __syncthreads();
+            // ---------------- phase C0: pairs, each unordered pair once ----------------
+            // For a system held by one wave: on sweep d lane i pairs with
+            // j = (i+d) mod N.  For fixed d this map is a permutation, so the
+            // reaction force -f_ij goes to a distinct LDS slot per lane (no
+            // atomics, fixed order).  Sweeps d = 1..floor(N/2) visit every
+            // unordered pair once (for even N the last sweep uses lanes i < N/2).
+            // The pair energy is counted at both ends, as getNonBond does.
+            KahanF3 pacc;
+            pacc.init(make_float3(0.f, 0.f, 0.f));
+            if (HALF_PAIRS && T.use_pairs) {
+                const int i = tid;
+                double enb = 0.0;
+                const int nd = N >> 1;
+                const float4 pi4 = (i < N) ? s_pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 ri = (i < N) ? s_req[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const int4 ng = my_ng[0];
+                int j = i;
+                for (int d = 1; d <= nd; d++) {
+                    j = (j + 1 == N) ? 0 : j + 1;
+                    const bool active = (i < N) && !((N & 1) == 0 && d == nd && i >= nd);
+                    if (active && !(j == ng.x || j == ng.y || j == ng.z || j == ng.w)) {
+                        float3 f;
+                        float e = ljq_pair(f3(s_pos[j]) - f3(pi4), ri, s_req[j], T.r2damp, f);
+                        pacc.add(f);
+                        float4 fj = s_fp[j];
+                        fj.x -= f.x; fj.y -= f.y; fj.z -= f.z;
+                        s_fp[j] = fj;
+                        enb += 2.0 * (double)e;
+                    }
+                }
+                E6[4] += enb;
+                __syncthreads();
+            }
             // ---------------- phase C: gather pieces + pairs + surface ----------------
             float3 F[APT];
 #pragma unroll
@@ -167,14 +220,19 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                 if (i < N) {
                     KahanF3 acc;
                     acc.init(fb[k]);
-                    const int *idx = T.a2f_idx + td.a2f_off + my_a2f[k].x;
+                    if (HALF_PAIRS && T.use_pairs) {
+                        acc.add(pacc.value());
+                        acc.add(f3(s_fp[i]));
+                    }
+                    const unsigned short *idx = s_a2f + my_a2f[k].x;
                     for (int q = 0; q < my_a2f[k].y; q++) {
                         const int p = idx[q];
                         acc.add(make_float3(fint_x[p], fint_y[p], fint_z[p]));
                     }
                     const float4 pi4 = s_pos[i];
                     const float3 pi = f3(pi4);
-                    if (T.use_pairs) {
+                    if (T.use_pairs && !HALF_PAIRS) {
+                        // every ordered pair (i, j), as getNonBond does
                         const float4 ri = s_req[i];
                         const int4 ng = my_ng[k];
                         double enb = 0.0;
@@ -189,7 +247,18 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     }
                     if (T.grid) {
                         float3 ul = make_float3(pi.x * T.ginv_x, pi.y * T.ginv_y, pi.z * T.ginv_z);
-                        float4 fe = grid_sample(T.grid, T.gnx, T.gny, T.gnz, sd.iux, sd.iuy, ul, my_plq[k]);
+                        float4 fe;
+                        if constexpr (CACHE) {
+                            GridCell gc = grid_cell(T.gnx, T.gny, T.gnz, sd.iux, sd.iuy, ul);
+                            const int key = gc.iz < 0 ? -2 : (gc.ix * T.gny + gc.iy) * T.gnz + gc.iz;
+                            if (key != skey) {
+                                if (gc.iz >= 0) grid_load_stencil(T.grid, T.gnx, T.gny, T.gnz, gc, my_plq[k], sv);
+                                skey = key;
+                            }
+                            fe = grid_eval_stencil(sv, gc);
+                        } else {
+                            fe = grid_sample(T.grid, T.gnx, T.gny, T.gnz, sd.iux, sd.iuy, ul, my_plq[k]);
+                        }
                         acc.add(make_float3(-fe.x * T.ginv_x, -fe.y * T.ginv_y, -fe.z * T.ginv_z));
                         E6[5] += (double)fe.w;
                     }
@@ -418,9 +487,9 @@ __global__ void k_plain_angles(PlainArgs P) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const TopoDesc td = P.tab.topo[P.topo];
     if (a >= td.nangles) return;
-    const int2 g = P.tab.ang_ng[td.ang_off + a];
+    const unsigned g = P.tab.ang_ng[td.ang_off + a];
     float3 fpi, fpj, fpk;
-    float e = angle_term(P.hneigh[g.x], P.hneigh[g.y], P.tab.ang_par[td.ang_off + a], fpi, fpj, fpk);
+    float e = angle_term(P.hneigh[g & 0xffffu], P.hneigh[g >> 16], P.tab.ang_par[td.ang_off + a], fpi, fpj, fpk);
     const int p = 4 * td.ndih + 4 * td.ninv + 3 * a;
     P.fint[p] = make_float4(fpi.x, fpi.y, fpi.z, e);
     P.fint[p + 1] = make_float4(fpj.x, fpj.y, fpj.z, 0.f);
@@ -431,9 +500,10 @@ __global__ void k_plain_dihedrals(PlainArgs P) {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     const TopoDesc td = P.tab.topo[P.topo];
     if (d >= td.ndih) return;
-    const int4 g = P.tab.dih_ng[td.dih_off + d];
+    const uint2 g = P.tab.dih_ng[td.dih_off + d];
     float3 f1, f2, f3v, f4;
-    float e = dihedral_term(P.hneigh[g.x], P.hneigh[g.y], P.hneigh[g.z], P.tab.dih_par[td.dih_off + d], g.w, f1, f2, f3v, f4);
+    float e = dihedral_term(P.hneigh[g.x & 0xffffu], P.hneigh[g.x >> 16], P.hneigh[g.y & 0xffffu],
+                            P.tab.dih_par[td.dih_off + d], (int)(g.y >> 16), f1, f2, f3v, f4);
     const int p = 4 * d;
     P.fint[p] = make_float4(f1.x, f1.y, f1.z, e);
     P.fint[p + 1] = make_float4(f2.x, f2.y, f2.z, 0.f);
@@ -445,9 +515,10 @@ __global__ void k_plain_inversions(PlainArgs P) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     const TopoDesc td = P.tab.topo[P.topo];
     if (v >= td.ninv) return;
-    const int4 g = P.tab.inv_ng[td.inv_off + v];
+    const uint2 g = P.tab.inv_ng[td.inv_off + v];
     float3 f1, f2, f3v, f4;
-    float e = inversion_term(P.hneigh[g.x], P.hneigh[g.y], P.hneigh[g.z], P.tab.inv_par[td.inv_off + v], f1, f2, f3v, f4);
+    float e = inversion_term(P.hneigh[g.x & 0xffffu], P.hneigh[g.x >> 16], P.hneigh[g.y], P.tab.inv_par[td.inv_off + v],
+                             f1, f2, f3v, f4);
     const int p = 4 * td.ndih + 4 * v;
     P.fint[p] = make_float4(f1.x, f1.y, f1.z, e);
     P.fint[p + 1] = make_float4(f2.x, f2.y, f2.z, 0.f);
@@ -464,7 +535,7 @@ __global__ void k_plain_assemble(PlainArgs P) {
     KahanF3 acc;
     acc.init(f3(P.force[i]));
     const int2 se = P.tab.a2f[ta];
-    const int *idx = P.tab.a2f_idx + td.a2f_off + se.x;
+    const unsigned short *idx = P.tab.a2f_idx + td.a2f_off + se.x;
     for (int q = 0; q < se.y; q++) acc.add(f3(P.fint[idx[q]]));
     const float3 pi = f3(P.pos[i]);
     float enb = 0.f, esurf = 0.f;
@@ -559,40 +630,42 @@ __global__ void k_plain_fire_apply(PlainArgs P, float a, float b, int reset, flo
 // ===========================================================================
 // host-side launch helpers
 // ===========================================================================
-template <int BLOCK, int APT, bool FL>
+template <int BLOCK, int APT, bool FL, bool SC>
 static hipError_t launch_one(const KernelArgs &A, int grid, size_t lds, hipStream_t stream) {
-    hipLaunchKernelGGL((k_relax<BLOCK, APT, FL>), dim3(grid), dim3(BLOCK), lds, stream, A);
+    hipLaunchKernelGGL((k_relax<BLOCK, APT, FL, SC>), dim3(grid), dim3(BLOCK), lds, stream, A);
     return hipGetLastError();
 }
 
-template <int BLOCK, int APT, bool FL>
+template <int BLOCK, int APT, bool FL, bool SC>
 static const void *kernel_ptr() {
-    return reinterpret_cast<const void *>(&k_relax<BLOCK, APT, FL>);
+    return reinterpret_cast<const void *>(&k_relax<BLOCK, APT, FL, SC>);
 }
 
+// instantiated shapes: (threads per workgroup, atoms per thread); the
+// stencil-cache variant exists for one atom per thread
+#define SMR_SHAPES(X) X(64, 1) X(128, 1) X(256, 1) X(256, 2) X(256, 4)
+
 hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t stream) {
-#define SMR_CASE(B, P)                                                                  \
-    if (L.block == B && L.apt == P) {                                                   \
-        return L.fint_lds ? launch_one<B, P, true>(A, L.grid, L.lds_bytes, stream)      \
-                          : launch_one<B, P, false>(A, L.grid, L.lds_bytes, stream);    \
+#define SMR_CASE(B, P)                                                                                   \
+    if (L.block == B && L.apt == P) {                                                                    \
+        if (P == 1 && L.stencil_cache)                                                                   \
+            return L.fint_lds ? launch_one<B, P, true, true>(A, L.grid, L.lds_bytes, stream)             \
+                              : launch_one<B, P, false, true>(A, L.grid, L.lds_bytes, stream);           \
+        return L.fint_lds ? launch_one<B, P, true, false>(A, L.grid, L.lds_bytes, stream)                \
+                          : launch_one<B, P, false, false>(A, L.grid, L.lds_bytes, stream);              \
     }
-    SMR_CASE(64, 1)
-    SMR_CASE(128, 1)
-    SMR_CASE(256, 1)
-    SMR_CASE(256, 2)
-    SMR_CASE(256, 4)
+    SMR_SHAPES(SMR_CASE)
 #undef SMR_CASE
     return hipErrorInvalidValue;
 }
 
-const void *relax_kernel_ptr(int block, int apt, bool fint_lds) {
-#define SMR_CASE(B, P) \
-    if (block == B && apt == P) return fint_lds ? kernel_ptr<B, P, true>() : kernel_ptr<B, P, false>();
-    SMR_CASE(64, 1)
-    SMR_CASE(128, 1)
-    SMR_CASE(256, 1)
-    SMR_CASE(256, 2)
-    SMR_CASE(256, 4)
+const void *relax_kernel_ptr(int block, int apt, bool fint_lds, bool stencil_cache) {
+#define SMR_CASE(B, P)                                                                                   \
+    if (block == B && apt == P) {                                                                        \
+        if (P == 1 && stencil_cache) return fint_lds ? kernel_ptr<B, P, true, true>() : kernel_ptr<B, P, false, true>(); \
+        return fint_lds ? kernel_ptr<B, P, true, false>() : kernel_ptr<B, P, false, false>();            \
+    }
+    SMR_SHAPES(SMR_CASE)
 #undef SMR_CASE
     return nullptr;
 }

@@ -58,14 +58,14 @@ struct DevTables {
     const float4 *req;     // (R, sqrtE, Q, 0)
     const float4 *plq;     // (e^{2aR} sqrtE, e^{aR} sqrtE, Q, 0)
     const int2 *a2f;       // (start, count) into the piece-index pool (relative to a2f_off)
-    // per-interaction pools
-    const int2 *ang_ng;    // hneigh slots (4*atom+slot), local
-    const float4 *ang_par; // K*(c0, c1, c2, c3)
-    const int4 *dih_ng;    // 3 hneigh slots + multiplicity n
-    const float4 *dih_par; // (V, V*d, V*d*n, 0)
-    const int4 *inv_ng;    // 3 hneigh slots
-    const float4 *inv_par; // K*(C0, C1, C2), 0
-    const int *a2f_idx;    // piece indices (local to the topology's fint layout)
+    // per-interaction pools; hneigh slots (4*atom+slot, local) packed 16-bit
+    const unsigned *ang_ng;      // slot_ji | slot_jk << 16
+    const float4 *ang_par;       // K*(c0, c1, c2, c3)
+    const uint2 *dih_ng;         // (s12 | s32 << 16, s43 | n << 16), n = multiplicity
+    const float4 *dih_par;       // (V, V*d, V*d*n, 0)
+    const uint2 *inv_ng;         // (s21 | s31 << 16, s41)
+    const float4 *inv_par;       // K*(C0, C1, C2), 0
+    const unsigned short *a2f_idx;  // piece indices (local to the topology's fint layout)
     // GridFF
     const float4 *grid;    // [nx*ny*nz] z-fastest (P, L, Q, 0); NULL = no surface
     int gnx, gny, gnz;
@@ -91,6 +91,11 @@ __device__ __forceinline__ float3 lincomb(float a, float3 u, float b, float3 w) 
     return make_float3(a * u.x + b * w.x, a * u.y + b * w.y, a * u.z + b * w.z);
 }
 
+// 1-ulp hardware reciprocal and reciprocal square root (v_rcp_f32,
+// v_rsq_f32) instead of the IEEE-exact division / sqrt sequences.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // Compensated (Kahan) accumulator for one force vector.
 struct KahanF3 {
     float3 s, c;
@@ -115,8 +120,8 @@ __device__ __forceinline__ float bond_term(float3 pi, float3 pj, float k, float 
     float3 dp = pj - pi;
     float l2 = dot3(dp, dp);
     l2 = fmaxf(l2, 1e-16f);
-    float l = sqrtf(l2);
-    float inv_l = 1.0f / l;
+    float inv_l = frsq(l2);
+    float l = l2 * inv_l;
     float3 h = dp * inv_l;
     hn = make_float4(h.x, h.y, h.z, inv_l);
     float dl = l - l0;
@@ -131,8 +136,9 @@ __device__ __forceinline__ float angle_term(float4 qij, float4 qkj, float4 kc, f
     float3 h = a + b;
     float c = 0.5f * (dot3(h, h) - 2.0f);
     c = fminf(fmaxf(c, -1.0f), 1.0f);
-    float s = sqrtf(fmaxf(0.0f, 1.0f - c * c) + 1e-14f);
-    float inv_s = (s > 1e-12f) ? 1.0f / s : 0.0f;
+    float s2 = fmaxf(0.0f, 1.0f - c * c) + 1e-14f;
+    float inv_s = frsq(s2);
+    float s = s2 * inv_s;
     float e = kc.x, ff = kc.y;
     float cx = c * c - s * s, cy = 2.0f * c * s;          // (cos2t, sin2t)
     e += kc.z * cx;
@@ -159,8 +165,8 @@ __device__ __forceinline__ float dihedral_term(float4 q12, float4 q32, float4 q4
         fp1 = fp2 = fp3 = fp4 = make_float3(0.f, 0.f, 0.f);
         return 0.0f;
     }
-    float il2_123 = 1.0f / n1, il2_234 = 1.0f / n2;
-    float inv_n12 = sqrtf(il2_123 * il2_234);
+    float il2_123 = frcp(n1), il2_234 = frcp(n2);
+    float inv_n12 = frsq(n1) * frsq(n2);
     float csx = dot3(n123, n234) * inv_n12;
     float csy = -dot3(n123, h43) * inv_n12;
     float cnx = csx, cny = csy;
@@ -173,8 +179,8 @@ __device__ __forceinline__ float dihedral_term(float4 q12, float4 q32, float4 q4
     float f = -par.z * cny;
     fp1 = n123 * (-f * il2_123 * q12.w);
     fp4 = n234 * (f * il2_234 * q43.w);
-    float c123 = dot3(h32, h12) * (q32.w / q12.w);
-    float c432 = dot3(h32, h43) * (q32.w / q43.w);
+    float c123 = dot3(h32, h12) * (q32.w * frcp(q12.w));
+    float c432 = dot3(h32, h43) * (q32.w * frcp(q43.w));
     fp3 = lincomb(-c123, fp1, -c432 - 1.0f, fp4);
     fp2 = lincomb(c123 - 1.0f, fp1, c432, fp4);
     return e;
@@ -195,10 +201,12 @@ __device__ __forceinline__ float inversion_term(float4 q21, float4 q31, float4 q
     }
     float s = -dot3(n123, h41);
     s = fminf(fmaxf(s, -1.0f), 1.0f);
-    float c = sqrtf(fmaxf(0.0f, 1.0f - s * s) + 1e-14f);
+    float cc = fmaxf(0.0f, 1.0f - s * s) + 1e-14f;
+    float inv_c = frsq(cc);
+    float c = cc * inv_c;
     float c2x = c * c - s * s, c2y = 2.0f * c * s;
     float e = par.x + par.y * c + par.z * c2x;
-    float f = (c > 1e-7f) ? -(par.y * s + 2.0f * par.z * c2y) / c : 0.0f;
+    float f = (c > 1e-7f) ? -(par.y * s + 2.0f * par.z * c2y) * inv_c : 0.0f;
     float fq41 = f * q41.w;
     float fi123 = f * il123;
     float3 tq = lincomb(s * fi123, n123, fi123, h41);
@@ -210,16 +218,17 @@ __device__ __forceinline__ float inversion_term(float4 q21, float4 q31, float4 q
 }
 
 // getLJQH (UFF.cl:82-95) after mixREQ_arithmetic (UFF.cl:57-65):
-// R0 = Ri+Rj, E0 = sqrtEi*sqrtEj, Q = Qi*Qj.  dp = pj - pi; returns E and the
-// force on i in f.
+// R0 = Ri+Rj, E0 = sqrtEi*sqrtEj, kC*Q = qi*qj where the staged REQ rows
+// hold q = sqrt(kC)*Q.  dp = pj - pi; returns E and the force on i in f.
 __device__ __forceinline__ float ljq_pair(float3 dp, float4 ri, float4 rj, float r2damp, float3 &f) {
     float R0 = ri.x + rj.x;
     float E0 = ri.y * rj.y;
-    float Q = ri.z * rj.z;
+    float kQ = ri.z * rj.z;
     float r2 = dot3(dp, dp);
-    float ir2_ = 1.0f / (r2 + r2damp);
-    float Ec = SMR_COULOMB_CONST * Q * sqrtf(ir2_);
-    float ir2 = 1.0f / r2;
+    float ir_ = frsq(r2 + r2damp);
+    float ir2_ = ir_ * ir_;
+    float Ec = kQ * ir_;
+    float ir2 = frcp(r2);
     float u2 = R0 * R0 * ir2;
     float u6 = u2 * u2 * u2;
     float vdW = u6 * E0;
@@ -243,6 +252,8 @@ __device__ __forceinline__ float4 bsp_dbasis(float u) {
     return make_float4(-0.5f * t * t, 0.5f * (3.0f * u2 - 4.0f * u), 0.5f * (-3.0f * u2 + 2.0f * u + 1.0f), 0.5f * u2);
 }
 __device__ __forceinline__ float dot4(float4 a, float4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+// PLQH . C over the three used channels (the fourth channel is zero)
+__device__ __forceinline__ float dotplq(float4 p, float4 c) { return p.x * c.x + p.y * c.y + p.z * c.z; }
 
 // Sample at grid coordinate u = (ix0 + ul.x, iy0 + ul.y, ul.z), where (ix0,iy0)
 // is the system's integer node offset and ul the local fractional
@@ -272,7 +283,7 @@ __device__ __forceinline__ float4 grid_sample(const float4 *__restrict__ G, int 
             int yi = iy + b; if (yi >= ny) yi -= ny;
             const float4 *col = G + ((size_t)(xi * ny + yi) * nz + (iz - 1));
             float4 c0 = col[0], c1 = col[1], c2 = col[2], c3 = col[3];
-            float4 cs = make_float4(dot4(plq, c0), dot4(plq, c1), dot4(plq, c2), dot4(plq, c3));
+            float4 cs = make_float4(dotplq(plq, c0), dotplq(plq, c1), dotplq(plq, c2), dotplq(plq, c3));
             float fe = dot4(bz, cs);   // fe1Dcomb
             float fd = dot4(dz, cs);
             e_dy += fe * dy_[b];       // fe2d_comb
@@ -280,6 +291,80 @@ __device__ __forceinline__ float4 grid_sample(const float4 *__restrict__ G, int 
             e_y += fe * by_[b];
         }
         ox += dx_[a] * e_y;            // fe3d_pbc_comb
+        oy += bx_[a] * e_dy;
+        oz += bx_[a] * e_dz;
+        oe += bx_[a] * e_y;
+    }
+    return make_float4(ox, oy, oz, oe);
+}
+
+// The same sample split in two: load the 4x4x4 stencil of one cell already
+// contracted with the atom's PLQH (v[16a+4b+c] = PLQH . C[x+a][y+b][z+c]),
+// then interpolate inside the cell.  grid_eval_stencil(grid_load_stencil())
+// performs exactly the arithmetic of grid_sample, in the same order.
+struct GridCell {
+    int ix, iy, iz;     // first stencil node (wrapped x,y); iz < 0: outside
+    float tx, ty, tz;
+};
+
+__device__ __forceinline__ GridCell grid_cell(int nx, int ny, int nz, int ix0, int iy0, float3 ul) {
+    GridCell g;
+    float fx = floorf(ul.x), fy = floorf(ul.y);
+    int iz = (int)ul.z;
+    g.tx = ul.x - fx;
+    g.ty = ul.y - fy;
+    g.tz = ul.z - (float)iz;
+    if (iz < 1 || iz >= nz - 2) {
+        g.ix = g.iy = 0;
+        g.iz = -1;
+        return g;
+    }
+    int ix = ix0 + (int)fx - 1;
+    int iy = iy0 + (int)fy - 1;
+    ix %= nx; if (ix < 0) ix += nx;
+    iy %= ny; if (iy < 0) iy += ny;
+    g.ix = ix;
+    g.iy = iy;
+    g.iz = iz - 1;
+    return g;
+}
+
+__device__ __forceinline__ void grid_load_stencil(const float4 *__restrict__ G, int nx, int ny, int nz, const GridCell &g,
+                                                  float4 plq, float *v) {
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        int xi = g.ix + a; if (xi >= nx) xi -= nx;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            int yi = g.iy + b; if (yi >= ny) yi -= ny;
+            const float4 *col = G + ((size_t)(xi * ny + yi) * nz + g.iz);
+#pragma unroll
+            for (int c = 0; c < 4; c++) v[16 * a + 4 * b + c] = dotplq(plq, col[c]);
+        }
+    }
+}
+
+__device__ __forceinline__ float4 grid_eval_stencil(const float *v, const GridCell &g) {
+    if (g.iz < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 bz = bsp_basis(g.tz), dz = bsp_dbasis(g.tz);
+    float4 by = bsp_basis(g.ty), dy = bsp_dbasis(g.ty);
+    float4 bx = bsp_basis(g.tx), dx = bsp_dbasis(g.tx);
+    float by_[4] = {by.x, by.y, by.z, by.w}, dy_[4] = {dy.x, dy.y, dy.z, dy.w};
+    float bx_[4] = {bx.x, bx.y, bx.z, bx.w}, dx_[4] = {dx.x, dx.y, dx.z, dx.w};
+    float ox = 0.f, oy = 0.f, oz = 0.f, oe = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        float e_y = 0.f, e_dy = 0.f, e_dz = 0.f;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            float4 cs = make_float4(v[16 * a + 4 * b], v[16 * a + 4 * b + 1], v[16 * a + 4 * b + 2], v[16 * a + 4 * b + 3]);
+            float fe = dot4(bz, cs);
+            float fd = dot4(dz, cs);
+            e_dy += fe * dy_[b];
+            e_dz += fd * by_[b];
+            e_y += fe * by_[b];
+        }
+        ox += dx_[a] * e_y;
         oy += bx_[a] * e_dy;
         oz += bx_[a] * e_dz;
         oe += bx_[a] * e_y;

@@ -36,13 +36,13 @@ struct HostPools {
     std::vector<int4> ng;
     std::vector<float4> bk, bl, req, plq;
     std::vector<int2> a2f;
-    std::vector<int2> ang_ng;
+    std::vector<unsigned> ang_ng;
     std::vector<float4> ang_par;
-    std::vector<int4> dih_ng;
+    std::vector<uint2> dih_ng;
     std::vector<float4> dih_par;
-    std::vector<int4> inv_ng;
+    std::vector<uint2> inv_ng;
     std::vector<float4> inv_par;
-    std::vector<int> a2f_idx;
+    std::vector<unsigned short> a2f_idx;
 };
 
 struct DevPools {
@@ -52,13 +52,17 @@ struct DevPools {
 int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, HostPools &P, TopoDesc &td);
 hipError_t upload_pools(HostPools &P, DevPools &D, hipStream_t st);
 void system_origin(const double *x, int n, const smr_gridff *grid, bool surface, double org[3], int &iux, int &iuy);
-LaunchShape choose_shape(const smr_ctx *c, int max_atoms, int nf_max, int nwork);
+LaunchShape choose_shape(const smr_ctx *c, const smr_batch *b, int nwork);
 int check_fire(const smr_fire_params *f, const char *who);
+void ctx_retain(smr_ctx *c);
+void ctx_release(smr_ctx *c);
 FireConst make_fire_const(const smr_fire_params *f, int max_steps, double fconv);
 
 }  // namespace smr
 
 struct smr_ctx {
+    int refs = 1;                // the caller's handle + one per live grid/batch/server
+    bool destroy_requested = false;
     int device = 0;
     int num_cu = 0;
     size_t max_lds = 0;
@@ -81,11 +85,13 @@ struct smr_batch {
     smr::HostPools hp;
     smr::DevPools dp;
     DevTables tab;
-    int nsys = 0, total_atoms = 0, max_atoms = 0, nf_max = 0;
+    int nsys = 0, total_atoms = 0, max_atoms = 0, nf_max = 0, na_max = 0, nd_max = 0, ni_max = 0;
     std::vector<int> atom_off;       // [nsys+1]
     std::vector<double> origin;      // [3*nsys]
     smr::DevBuf pos, pos0, vel, force, state, work, fint_scratch;
     smr::DevBuf plain_hneigh, plain_fint, plain_e, plain_cvf;
     smr::LaunchShape last_shape;
+    int tune_block = 0;              // 0: automatic
+    int tune_cache = -1;             // -1: automatic
     float last_kernel_ms = 0.f;
 };

@@ -22,20 +22,26 @@ struct KernelArgs {
     float *fint_scratch;     // [grid * 3 * nf_max] when pieces do not fit LDS
     int nf_max;
     int lds_atoms;           // atoms reserved per LDS slot
+    int lds_na, lds_nd, lds_ni;  // interaction slots reserved per LDS slot
     int steps_per_launch;
     int mode;
 };
 
 struct LaunchShape {
-    int block, apt, grid;
-    bool fint_lds;
-    size_t lds_bytes;
+    int block = 0, apt = 0, grid = 0;
+    bool fint_lds = false;
+    bool stencil_cache = false;   // per-lane register cache of the surface stencil
+    size_t lds_bytes = 0;
 };
 
-// bytes of dynamic LDS for a launch shape
-inline size_t relax_lds_bytes(int block, int lds_atoms, int nf_max, bool fint_lds) {
-    size_t b = (size_t)lds_atoms * 16 * 6 + (size_t)(block / 64) * 8 * 8 + 16;
+// bytes of dynamic LDS for a launch shape: positions, REQ rows, hneigh,
+// pair reaction forces, reduction scratch, packed interaction slots, force pieces (optional) and
+// the 16-bit piece-gather list
+inline size_t relax_lds_bytes(int block, int lds_atoms, int nf_max, int na, int nd, int ni, bool fint_lds) {
+    size_t b = (size_t)lds_atoms * 16 * 7 + (size_t)(block / 64) * 8 * 8 + 16;
+    b += (size_t)nd * 8 + (size_t)ni * 8 + (size_t)na * 4;
     if (fint_lds) b += (size_t)nf_max * 12;
+    b += ((size_t)nf_max * 2 + 15) & ~(size_t)15;
     return b;
 }
 
@@ -53,7 +59,7 @@ struct PlainArgs {
 };
 
 hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t stream);
-const void *relax_kernel_ptr(int block, int apt, bool fint_lds);
+const void *relax_kernel_ptr(int block, int apt, bool fint_lds, bool stencil_cache);
 hipError_t launch_plain_step_forces(const PlainArgs &P, int natoms, int nang, int ndih, int ninv, hipStream_t st);
 hipError_t launch_plain_reduce(const PlainArgs &P, double dt, double flim, hipStream_t st);
 hipError_t launch_plain_apply(const PlainArgs &P, int natoms, float a, float b, int reset, float dt, hipStream_t st);

@@ -34,6 +34,8 @@ def ctx():
     import surfmol_amd as S
     c = S.Context(0)
     yield c
+    # grids/batches still referenced (e.g. by a failing test's traceback) keep
+    # the device context alive; close() only drops this handle
     c.close()
 
 

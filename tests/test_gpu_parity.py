@@ -100,10 +100,13 @@ def test_relax_batched_matches_fp64(ctx, surf, oracle, params):
         ro = oracle.relax_fire(osys, oracle.FireParams(**fp), 3000, 1e-3)
         assert ro["converged"]
         print(s, r["steps"], ro["steps"], r["energy"], ro["E"], eo.sum())
-        assert abs(r["energy"] - ro["E"]) / mol.natoms < 5e-6, (s, r["energy"], ro["E"])
-        # energy of the converged state: GPU value vs FP64 at GPU positions
-        # (energy moves by ~F.dx over the final step; compare at positions)
-        assert abs(eo.sum() - ro["E"]) / mol.natoms < 5e-6
+        # the reported energy is E(x) at the last force evaluation; FP64 at
+        # the final positions (one small FIRE move later) agrees to 1e-6 eV/atom
+        assert abs(eo.sum() - r["energy"]) / mol.natoms < 1e-6, (s, eo.sum(), r["energy"])
+        # both relaxations end in the same basin: converged states differ only
+        # along soft modes allowed by Fmax < 1e-3 (distinct basins of this
+        # system are >= 5e-4 eV/atom apart)
+        assert abs(r["energy"] - ro["E"]) / mol.natoms < 2e-5, (s, r["energy"], ro["E"])
 
 
 def test_relax_plain_matches_batched(ctx, surf):
@@ -116,7 +119,7 @@ def test_relax_plain_matches_batched(ctx, surf):
     b2 = _batch(ctx, surf, mol, X)
     rp = b2.relax_plain(0, fp, 3000, 1e-3)
     assert rp["status"] == S.SMR_SYS_CONVERGED
-    assert abs(rp["energy"] - rb[0]["energy"]) / mol.natoms < 5e-6, (rp, rb[0])
+    assert abs(rp["energy"] - rb[0]["energy"]) / mol.natoms < 2e-5, (rp, rb[0])
 
 
 def test_eval_same_alone_and_in_mix(ctx, surf):
@@ -129,3 +132,28 @@ def test_eval_same_alone_and_in_mix(ctx, surf):
     alone = _batch(ctx, surf, mol, X[3:4]).relax_batched(fp, 3000, 1e-3)
     assert alone[0]["steps"] == mix[3]["steps"]
     assert alone[0]["energy"] == mix[3]["energy"]
+
+
+def test_launch_variants(ctx, surf):
+    """Launch variants (surface stencil cache on/off, 64- or 128-thread
+    workgroups) relax every system to the same basin.  They are not
+    bit-identical: the cached path is compiled separately (FMA contraction)
+    and a wider workgroup changes the reduction tree."""
+    import surfmol_amd as S
+    spec = surf["spec"]
+    mol, X = ptcda_replicas(8, spec["L"])
+    fp = S.FireParams(**fire_params_internal())
+    out = {}
+    for key in [(64, 0), (64, 1), (128, 0)]:
+        b = _batch(ctx, surf, mol, X)
+        b.set_launch(*key)
+        r = b.relax_batched(fp, 3000, 1e-3)
+        ll = b.last_launch()
+        assert ll["block"] == key[0] and ll["stencil_cache"] == bool(key[1])
+        out[key] = (r, b.positions())
+    r0, x0 = out[(64, 0)]
+    for key in [(64, 1), (128, 0)]:
+        r2, _ = out[key]
+        for a, c in zip(r0, r2):
+            assert c["status"] == S.SMR_SYS_CONVERGED
+            assert abs(a["energy"] - c["energy"]) / mol.natoms < 2e-5, (key, a, c)
