@@ -1,16 +1,22 @@
 """Benchmark of the multi-system relaxation path on one MI355X.
 
-Metric: relaxed systems/s at Fmax < 1e-3 eV/A.  Workload: the uniform batch
--- 4096 replicas of the 38-atom PTCDA-like molecule (C24H8O6), each with a
-seeded in-plane rotation, xy offset U(0,24) A and height U(3,5) A, on a
-24 x 24 x 20 A rock-salt GridFF at 0.1 A (240 x 240 x 200 nodes, 3 FP32
-channels), FIRE with dt0 = 0.1 fs, dt_max = 1 fs, at most 2000 steps.
+Metric: relaxed systems/s at Fmax < 1e-3 eV/A.
 
-One "step" of this script is one complete batched relaxation of all 4096
-systems (positions restored on the device before each).  `value` counts only
-systems that reached Fmax < 1e-3 eV/A.  Also reported: the plain method (one
-launch sequence per FIRE step, host FIRE decision) on a sample of systems,
-and the FP64 CPU restatement (single thread) on a bounded sample.
+--workload uniform (default): 4096 replicas of the 38-atom PTCDA-like
+  molecule (C24H8O6), each with a seeded in-plane rotation, xy offset
+  U(0,24) A and height U(3,5) A, on a 24 x 24 x 20 A rock-salt GridFF at
+  0.1 A (240 x 240 x 200 nodes, 3 FP32 channels), FIRE with dt0 = 0.1 fs,
+  dt_max = 1 fs, at most 2000 steps.
+--workload uneven: 2048 random fused-ring molecules with atom counts
+  log-uniform in [20, 400], xy U(0,48) A, height U(3,5) A, on a 48 x 48 x 20 A
+  GridFF at 0.1 A (480 x 480 x 200 nodes); same FIRE settings.
+
+One "step" of this script is one complete batched relaxation of every
+system of the workload (positions restored on the device before each).
+`value` counts only systems that reached Fmax < 1e-3 eV/A.  Also reported:
+the plain method (one launch sequence per FIRE step, host FIRE decision) on a
+sample of systems, and the FP64 CPU restatement (single thread) on a bounded
+sample of the same systems.
 """
 from __future__ import annotations
 
@@ -35,13 +41,48 @@ def falg_per_system_step(nb, na, nd, ni, npairs, n):
     return 28 * nb + 80 * na + 120 * nd + 120 * ni + 41 * npairs + 740 * n + 40 * n
 
 
+def grid_for(ctx, S, W, L, zlen=20.0, step=0.1, alpha=1.5, rdamp=1.0):
+    """Rock-salt GridFF of an L x L x zlen cell at `step` (device FP64 build + fit)."""
+    nxy, nz = int(round(L / step)), int(round(zlen / step))
+    el, spos, sq = W.rocksalt_slab(L, L)
+    sreq = S.substrate_reqs(el, sq)
+    spec = S.GridSpec(nxy, nxy, nz, (0.0, 0.0, -1.0), (step, step, step), alpha, rdamp)
+    t0 = time.time()
+    grid = S.GridFF.build(ctx, spec, spos, sreq, npbc=2)
+    return grid, spec, time.time() - t0
+
+
+def workload_uniform(args, S, W):
+    mol = W.ptcda_like(W.SEED_BASE)
+    ang, xy, h = W.uniform_batch_poses(args.nsys, L=24.0, hmin=3.0, hmax=5.0, seed=W.SEED_BASE + 1)
+    X = [W.place_on_surface(mol, ang[s], xy[s], h[s]) for s in range(args.nsys)]
+    desc = ("uniform batch: %d PTCDA-like C24H8O6 replicas (38 atoms), rotation U(0,2pi), xy U(0,24) A, height "
+            "U(3,5) A, on a 24x24x20 A rock-salt GridFF at 0.1 A (240x240x200 nodes); UFF + intra-molecular "
+            "LJ/Coulomb + GridFF B-spline; FIRE dt0 0.1 fs, dt_max 1 fs, <= 2000 steps" % args.nsys)
+    return [mol], np.zeros(args.nsys, dtype=np.int32), X, 24.0, desc
+
+
+def workload_uneven(args, S, W):
+    nsys = args.nsys if args.nsys_set else 2048
+    mols = W.uneven_batch(nsys, 20, 400, seed=W.SEED_BASE + 2)
+    ang, xy, h = W.uniform_batch_poses(nsys, L=48.0, hmin=3.0, hmax=5.0, seed=W.SEED_BASE + 3)
+    X = [W.place_on_surface(m, ang[s], xy[s], h[s]) for s, m in enumerate(mols)]
+    desc = ("uneven batch: %d random fused-6-ring C/N/O/H molecules, atom count log-uniform [20,400], rotation "
+            "U(0,2pi), xy U(0,48) A, height U(3,5) A, on a 48x48x20 A rock-salt GridFF at 0.1 A (480x480x200 "
+            "nodes); UFF + intra-molecular LJ/Coulomb + GridFF B-spline; FIRE dt0 0.1 fs, dt_max 1 fs, "
+            "<= 2000 steps" % nsys)
+    return mols, np.arange(nsys, dtype=np.int32), X, 48.0, desc
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nsys", type=int, default=4096)
+    ap.add_argument("--workload", choices=["uniform", "uneven"], default="uniform")
+    ap.add_argument("--nsys", type=int, default=None)
     ap.add_argument("--max-steps", type=int, default=2000)
+    ap.add_argument("--autotune", action="store_true", help="time every launch shape per size class first")
     ap.add_argument("--plain-sample", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -49,6 +90,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run only a plain device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
+    args.nsys_set = args.nsys is not None
+    if args.nsys is None:
+        args.nsys = 4096
     if args.copy_probe:
         import torch
         n = args.copy_probe // 4
@@ -67,24 +111,23 @@ def main():
     import surfmol_amd as S
     from surfmol_amd import workloads as W
 
-    L, step, zlen = 24.0, 0.1, 20.0
-    nxy, nz = int(round(L / step)), int(round(zlen / step))
     alpha, rdamp = 1.5, 1.0
-    ctx = S.Context(0)
-    el, spos, sq = W.rocksalt_slab(L, L)
-    sreq = S.substrate_reqs(el, sq)
-    spec = S.GridSpec(nxy, nxy, nz, (0.0, 0.0, -1.0), (step, step, step), alpha, rdamp)
     t0 = time.time()
-    grid = S.GridFF.build(ctx, spec, spos, sreq, npbc=2)
-    t_grid = time.time() - t0
-
-    mol = W.ptcda_like(W.SEED_BASE)
-    ang, xy, h = W.uniform_batch_poses(args.nsys, L=L, hmin=3.0, hmax=5.0, seed=W.SEED_BASE + 1)
-    X = np.concatenate([W.place_on_surface(mol, ang[s], xy[s], h[s]) for s in range(args.nsys)])
-    hm = S.Molecule.build(mol.elems, X[:mol.natoms], mol.charges, bonds=mol.bonds)
-    batch = S.Batch(ctx, [hm], np.zeros(args.nsys, dtype=np.int32), X, grid=grid, rdamp=rdamp)
+    mols, sys_topo, Xs, L, desc = (workload_uniform if args.workload == "uniform" else workload_uneven)(args, S, W)
+    t_gen = time.time() - t0
+    nsys = len(sys_topo)
+    ctx = S.Context(0)
+    grid, spec, t_grid = grid_for(ctx, S, W, L, alpha=alpha, rdamp=rdamp)
+    hms = [S.Molecule.build(m.elems, Xs[list(sys_topo).index(t)], m.charges, bonds=m.bonds) for t, m in enumerate(mols)]
+    X = np.concatenate(Xs)
+    batch = S.Batch(ctx, hms, sys_topo, X, grid=grid, rdamp=rdamp)
     fire = S.FireParams(dt0=W.fs_to_internal(0.1), dt_max=W.fs_to_internal(1.0))
     fconv = 1e-3
+    tuned = None
+    if args.autotune:
+        t0 = time.time()
+        tuned = batch.autotune(fire, 50)
+        t_tune = time.time() - t0
 
     for _ in range(args.warmup):
         batch.reset()
@@ -96,20 +139,38 @@ def main():
         results = batch.relax_batched(fire, args.max_steps, fconv)   # synchronises the stream
         times.append(time.perf_counter() - t0)
         kms.append(batch.last_launch()["kernel_ms"])
-    launch = batch.last_launch()
     T = sum(times)
     nconv = sum(1 for r in results if r["status"] == S.SMR_SYS_CONVERGED)
     steps_arr = np.array([r["steps"] for r in results])
     value = nconv * args.steps / T
 
-    n, nb, na, nd, ni = hm.counts
-    npairs = n * (n - 1) // 2 - nb
-    falg = falg_per_system_step(nb, na, nd, ni, npairs, n)
-    flop_launch = falg * float(steps_arr.sum())
+    # algorithmic FLOP and grid-lookup bytes of one relaxation (sum over systems of per-step work x steps taken)
+    counts = [hm.counts for hm in hms]
+    flop_launch, bytes_launch, natoms_sys = 0.0, 0.0, []
+    for s in range(nsys):
+        n, nb, na, nd, ni = counts[sys_topo[s]]
+        npairs = n * (n - 1) // 2 - nb
+        flop_launch += falg_per_system_step(nb, na, nd, ni, npairs, n) * float(steps_arr[s])
+        bytes_launch += 768.0 * n * float(steps_arr[s])
+        natoms_sys.append(n)
     kavg_s = (sum(kms) / len(kms)) / 1e3
     achieved_tf = flop_launch / kavg_s / 1e12
-    bytes_launch = 768.0 * n * float(steps_arr.sum())
+    natoms_sys = np.array(natoms_sys)
 
+    cfg = {
+        "workload": desc,
+        "nsys": nsys,
+        "natoms_per_system": int(natoms_sys[0]) if args.workload == "uniform" else
+        {"min": int(natoms_sys.min()), "mean": round(float(natoms_sys.mean()), 1), "max": int(natoms_sys.max())},
+        "converged": nconv,
+        "fire_steps_mean": round(float(steps_arr.mean()), 1),
+        "fire_steps_max": int(steps_arr.max()),
+        "grid_build_fit_s": round(t_grid, 3),
+        "plan": batch.plan(),
+        "precision": "FP32 forces with Kahan per-atom sums; FP64 energies, FIRE reductions and convergence test",
+    }
+    if tuned is not None:
+        cfg["autotune_s"] = round(t_tune, 2)
     out = {
         "metric": "relaxed systems/sec at Fmax<1e-3 eV/A (batched)",
         "value": round(value, 2),
@@ -123,19 +184,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {
-            "workload": "uniform batch: 4096 PTCDA-like C24H8O6 replicas (38 atoms), rotation U(0,2pi), xy U(0,24) A, "
-                        "height U(3,5) A, on a 24x24x20 A rock-salt GridFF at 0.1 A (240x240x200 nodes); UFF + "
-                        "intra-molecular LJ/Coulomb + GridFF B-spline; FIRE dt0 0.1 fs, dt_max 1 fs, <= 2000 steps",
-            "nsys": args.nsys,
-            "natoms_per_system": n,
-            "converged": nconv,
-            "fire_steps_mean": round(float(steps_arr.mean()), 1),
-            "fire_steps_max": int(steps_arr.max()),
-            "grid_build_fit_s": round(t_grid, 3),
-            "launch": launch,
-            "precision": "FP32 forces with Kahan per-atom sums; FP64 energies, FIRE reductions and convergence test",
-        },
+        "config": cfg,
         "roofline": {
             "bound": "mfma",
             "note": "compute roof = FP32 vector peak; the kernel runs on the VALU (no dense contraction on the path)",
@@ -145,7 +194,7 @@ def main():
             "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4),
             "traffic": None,
             "kernel": "k_relax",
-            "flop_per_system_step": falg,
+            "flop_per_relaxation": flop_launch,
             "kernel_ms_avg": round(1e3 * kavg_s, 3),
             "grid_lookup_GBs": round(bytes_launch / kavg_s / 1e9, 1),
         },
@@ -153,7 +202,7 @@ def main():
 
     if not args.no_plain:
         pts = []
-        for s in range(min(args.plain_sample, args.nsys)):
+        for s in range(min(args.plain_sample, nsys)):
             batch.reset()
             t0 = time.perf_counter()
             r = batch.relax_plain(s, fire, args.max_steps, fconv)
@@ -161,8 +210,8 @@ def main():
             pts.append((dt, r["status"] == S.SMR_SYS_CONVERGED, r["steps"]))
         tp = sum(p[0] for p in pts)
         plain_rate = sum(1 for p in pts if p[1]) / tp
-        out["plain_gpu"] = {"value": round(plain_rate, 3), "unit": "systems/s", "sample": f"{len(pts)} systems, "
-                            f"one launch sequence + host FIRE decision per step", "ms_per_fire_step":
+        out["plain_gpu"] = {"value": round(plain_rate, 3), "unit": "systems/s", "sample": f"first {len(pts)} "
+                            f"systems, one launch sequence + host FIRE decision per step", "ms_per_fire_step":
                             round(1e3 * tp / sum(p[2] for p in pts), 4)}
         out["speedup_vs_plain"] = round(value / plain_rate, 1) if plain_rate > 0 else None
 
@@ -173,12 +222,13 @@ def main():
         C = grid.coefficients().astype(np.float64)
         ofire = O.FireParams(dt0=fire.dt0, dt_max=fire.dt_max)
         done, conv, t_cpu = 0, 0, 0.0
-        while t_cpu < args.cpu_seconds and done < args.nsys:
-            xs = X[done * n:(done + 1) * n]
-            osys = O.system_from_geometry(mol.elems, xs, P, charges=mol.charges, bonds=mol.bonds, nb_mode=O.NB_GPU,
-                                          rdamp=rdamp, grid=C, grid_ns=(nxy, nxy, nz), grid_p0=spec.p0,
+        # a sample that follows the workload's size mix: systems in batch order
+        while t_cpu < args.cpu_seconds and done < nsys:
+            m = mols[sys_topo[done]]
+            osys = O.system_from_geometry(m.elems, Xs[done], P, charges=m.charges, bonds=m.bonds, nb_mode=O.NB_GPU,
+                                          rdamp=rdamp, grid=C, grid_ns=(spec.nx, spec.ny, spec.nz), grid_p0=spec.p0,
                                           grid_step=spec.step, alpha_morse=alpha)
-            cs = osys.cstruct()  # set-up outside the timed region
+            osys.cstruct()  # set-up outside the timed region
             t0 = time.perf_counter()
             r = O.relax_fire(osys, ofire, args.max_steps, fconv)
             t_cpu += time.perf_counter() - t0

@@ -193,11 +193,13 @@ int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fire, int max
                     smr_relax_result *result);
 
 /* Launch variant of the fused kernel for this batch: threads per workgroup
- * (64/128/256; 0 = automatic by system size) and the per-lane register cache
- * of each atom's surface stencil (1 on, 0 off, -1 automatic).  Results are
- * identical across variants; only speed differs.  No counterpart in the
- * reference (tuning knob of this implementation). */
-int smr_batch_set_launch(smr_batch *b, int block, int stencil_cache);
+ * (64/128/256; 0 = automatic by system size; atoms per thread follow), the
+ * per-lane register cache of each atom's surface stencil (1 on, 0 off, -1
+ * automatic) and the pair loop (1 unordered-pair sweep, 0 every ordered
+ * pair, -1 automatic).  All variants compute the same forces; results differ
+ * only by FP32 summation order.  No counterpart in the reference (tuning
+ * knob of this implementation). */
+int smr_batch_set_launch(smr_batch *b, int block, int stencil_cache, int pair_sweep);
 
 /* Launch shape and device time of the last fused launch on this batch:
  * shape4 = (threads per workgroup, atoms per thread, workgroups, LDS bytes
@@ -205,6 +207,30 @@ int smr_batch_set_launch(smr_batch *b, int block, int stencil_cache);
  * kernel_ms = HIP-event time of that launch on the context's stream (only
  * measured by smr_relax_batched).  No counterpart in the reference. */
 int smr_batch_last_launch(const smr_batch *b, int *shape4, float *kernel_ms);
+
+/* Size classes of the batch and their launch shapes.  Systems are grouped
+ * by atom count (<= 64, 128, 256, 512, 1024); each class is one persistent
+ * launch.  *nclass receives the number of non-empty classes and info
+ * [8 * nclass] per class: (class id, systems, max atoms, threads per
+ * workgroup, atoms per thread, workgroups, LDS bytes, flags: 1 force pieces
+ * in LDS, 2 unordered-pair sweep, 4 stencil cache).  No counterpart in the
+ * reference (its multi-system ABI has one fixed work-group shape,
+ * opencl/relax_multi.cl:425-443). */
+int smr_batch_plan(const smr_batch *b, int *nclass, int *info);
+
+/* Autotuner: time every launch shape that can hold each size class of this
+ * batch on a FIRE run of trial_steps steps and keep the fastest per class in
+ * the context (used by later batches and servers of the context).  The
+ * batch's positions, velocities and FIRE state are restored.  Results of a
+ * relaxation depend on the launch shape only through FP32 summation order.
+ * No counterpart in the reference. */
+int smr_batch_autotune(smr_batch *b, const smr_fire_params *fire, int trial_steps);
+
+/* 1 (default): capture the launch sequence of a relaxation (counter reset +
+ * one persistent launch per size class on forked streams) into a HIP graph
+ * once and replay it while nothing changes; 0: issue the launches directly.
+ * No counterpart in the reference. */
+int smr_batch_set_graph(smr_batch *b, int use_graph);
 
 /* Damped MD of MolWorld::run_md (crates/libs/surfmol/src/mol_world.rs:141-164,
  * moltopo/src/molecular.rs:118-137) for every system: clamp |F| to flim,

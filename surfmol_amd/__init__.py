@@ -310,9 +310,10 @@ class Batch:
         check(lib().smr_run_md(self._h, int(niter), float(dt), float(fconv), float(flim), float(damping), _iptr(steps)))
         return steps
 
-    def set_launch(self, block=0, stencil_cache=-1):
-        """Kernel variant (threads/workgroup, surface stencil cache); 0/-1 = automatic."""
-        check(lib().smr_batch_set_launch(self._h, int(block), int(stencil_cache)))
+    def set_launch(self, block=0, stencil_cache=-1, pair_sweep=-1):
+        """Kernel variant (threads/workgroup, surface stencil cache, unordered-pair
+        sweep); 0/-1/-1 = automatic."""
+        check(lib().smr_batch_set_launch(self._h, int(block), int(stencil_cache), int(pair_sweep)))
 
     def last_launch(self):
         shape = (ctypes.c_int * 4)()
@@ -320,6 +321,28 @@ class Batch:
         check(lib().smr_batch_last_launch(self._h, shape, ctypes.byref(ms)))
         return {"block": shape[0], "apt": shape[1], "grid": shape[2], "lds_bytes": shape[3] & ((1 << 30) - 1),
                 "stencil_cache": bool(shape[3] >> 30), "kernel_ms": ms.value}
+
+    def plan(self):
+        """Size classes of the batch: one dict per class with its launch shape."""
+        n = ctypes.c_int()
+        info = (ctypes.c_int * (8 * 8))()
+        check(lib().smr_batch_plan(self._h, ctypes.byref(n), info))
+        out = []
+        for q in range(n.value):
+            o = info[8 * q:8 * q + 8]
+            out.append({"cls": o[0], "systems": o[1], "max_atoms": o[2], "block": o[3], "apt": o[4], "grid": o[5],
+                        "lds_bytes": o[6], "fint_lds": bool(o[7] & 1), "pair_sweep": bool(o[7] & 2),
+                        "stencil_cache": bool(o[7] & 4)})
+        return out
+
+    def autotune(self, fire: FireParams, trial_steps: int = 50):
+        """Time the launch shapes of every size class; the context keeps the fastest."""
+        fp = fire.cstruct()
+        check(lib().smr_batch_autotune(self._h, ctypes.byref(fp), int(trial_steps)))
+        return self.plan()
+
+    def set_graph(self, use_graph: bool):
+        check(lib().smr_batch_set_graph(self._h, 1 if use_graph else 0))
 
     def close(self):
         if self._h:

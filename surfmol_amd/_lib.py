@@ -87,7 +87,10 @@ _SIGNATURES = [
     ("smr_relax_plain", c_int, [c_void_p, c_int, P(SmrFireParams), c_int, c_double, P(SmrRelaxResult)]),
     ("smr_run_md", c_int, [c_void_p, c_int, c_double, c_double, c_double, c_double, P(c_int)]),
     ("smr_batch_last_launch", c_int, [c_void_p, P(c_int), P(c_float)]),
-    ("smr_batch_set_launch", c_int, [c_void_p, c_int, c_int]),
+    ("smr_batch_set_launch", c_int, [c_void_p, c_int, c_int, c_int]),
+    ("smr_batch_plan", c_int, [c_void_p, P(c_int), P(c_int)]),
+    ("smr_batch_autotune", c_int, [c_void_p, P(SmrFireParams), c_int]),
+    ("smr_batch_set_graph", c_int, [c_void_p, c_int]),
     ("smr_server_create", c_int, [c_void_p, c_void_p, P(SmrServerConfig), P(c_void_p)]),
     ("smr_server_submit", c_int, [c_void_p, P(SmrTopology), P(c_double), P(ctypes.c_longlong)]),
     ("smr_server_poll", c_int, [c_void_p, ctypes.c_longlong, P(c_int), P(SmrRelaxResult), P(c_double),

@@ -32,7 +32,7 @@ int hip_fail(hipError_t e, const char *where) {
 }
 
 void DevBuf::release() {
-    if (ptr) hipFree(ptr);
+    if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
 }
@@ -83,10 +83,9 @@ extern "C" int smr_ctx_create(int device, smr_ctx **out) {
     c->num_cu = prop.multiProcessorCount;
     c->max_lds = prop.sharedMemPerBlock;
     c->arch = prop.gcnArchName;
-    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
-    e = c->counter.alloc(sizeof(int) * 64);
-    if (e != hipSuccess) { hipStreamDestroy(c->stream); delete c; return hip_fail(e, "hipMalloc(counter)"); }
+    e = c->ss.create();
+    if (e != hipSuccess) { delete c; return hip_fail(e, "smr_ctx_create: streams"); }
+    c->stream = c->ss.main;
     *out = c;
     return SMR_OK;
 }
@@ -99,9 +98,7 @@ void ctx_retain(smr_ctx *c) { c->refs++; }
 void ctx_release(smr_ctx *c) {
     if (--c->refs > 0) return;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    c->counter.release();
-    (void)hipStreamDestroy(c->stream);
+    c->ss.destroy();
     delete c;
 }
 }  // namespace smr
@@ -271,15 +268,27 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
         P.ng.push_back(make_int4(ng[0], ng[1], ng[2], ng[3]));
         P.bk.push_back(make_float4(bk[0], bk[1], bk[2], bk[3]));
         P.bl.push_back(make_float4(bl[0], bl[1], bl[2], bl[3]));
+        double bk64[4] = {0, 0, 0, 0}, bl64[4] = {0, 0, 0, 0};
+        for (int s = 0; s < 4; s++)
+            if (ng[s] >= 0) {
+                bk64[s] = t.bon_params[2 * t.neigh_bs[4 * i + s]];
+                bl64[s] = t.bon_params[2 * t.neigh_bs[4 * i + s] + 1];
+            }
+        P.bk64.push_back(make_double4(bk64[0], bk64[1], bk64[2], bk64[3]));
+        P.bl64.push_back(make_double4(bl64[0], bl64[1], bl64[2], bl64[3]));
         const double *r = t.reqs + 4 * i;
         // pair-term row: (R, sqrtE, sqrt(kC)*Q) so that qi*qj = kC*Qi*Qj
-        P.req.push_back(make_float4((float)r[0], (float)r[1], (float)(r[2] * std::sqrt(14.3996448915)), 0.f));
+        const double sq = r[2] * std::sqrt(14.3996448915);
+        P.req.push_back(make_float4((float)r[0], (float)r[1], (float)sq, 0.f));
+        P.req64.push_back(make_double4(r[0], r[1], sq, 0.0));
         // PLQH (opencl/UFF.cl:1681-1687), computed in FP64
         if (have_grid) {
             double ej = std::exp(alpha * r[0]);
             P.plq.push_back(make_float4((float)(ej * ej * r[1]), (float)(ej * r[1]), (float)r[2], 0.f));
+            P.plq64.push_back(make_double4(ej * ej * r[1], ej * r[1], r[2], 0.0));
         } else {
             P.plq.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+            P.plq64.push_back(make_double4(0.0, 0.0, 0.0, 0.0));
         }
     }
     auto slot_ok = [&](int g) { return g >= 0 && g < 4 * N && t.neighs[g] >= 0; };
@@ -292,6 +301,7 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
         const double *p = t.ang_params + 5 * a;
         P.ang_ng.push_back((unsigned)g0 | ((unsigned)g1 << 16));
         P.ang_par.push_back(make_float4((float)(p[0] * p[1]), (float)(p[0] * p[2]), (float)(p[0] * p[3]), (float)(p[0] * p[4])));
+        P.ang64.push_back(make_double4(p[0], p[1], p[3], p[4]));
     }
     // ---- dihedrals ----
     for (int d = 0; d < t.ndih; d++) {
@@ -303,6 +313,7 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
         if (n < 1 || n > 12) return fail(SMR_ERR_INVALID, who + "dihedral " + std::to_string(d) + " multiplicity " + std::to_string(n));
         P.dih_ng.push_back(make_uint2((unsigned)g0 | ((unsigned)g1 << 16), (unsigned)g2 | ((unsigned)n << 16)));
         P.dih_par.push_back(make_float4((float)p[0], (float)(p[0] * p[1]), (float)(p[0] * p[1] * p[2]), 0.f));
+        P.dih64.push_back(make_double4(p[0], p[1], (double)n, 0.0));
     }
     // ---- inversions ----
     for (int v = 0; v < t.ninv; v++) {
@@ -312,6 +323,7 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
         const double *p = t.inv_params + 4 * v;
         P.inv_ng.push_back(make_uint2((unsigned)g0 | ((unsigned)g1 << 16), (unsigned)g2));
         P.inv_par.push_back(make_float4((float)(p[0] * p[1]), (float)(p[0] * p[2]), (float)(p[0] * p[3]), 0.f));
+        P.inv64.push_back(make_double4(p[0], p[1], p[2], p[3]));
     }
     // ---- a2f gather lists (Uff::map_atom_interactions, uff.rs:317-370):
     //      dihedral pieces, then inversion pieces, then angle pieces ----
@@ -354,7 +366,7 @@ hipError_t upload_pools(HostPools &P, DevPools &D, hipStream_t st) {
     hipError_t e;
 #define UP(name) if ((e = D.name.upload(P.name.data(), P.name.size() * sizeof(P.name[0]), st)) != hipSuccess) return e;
     UP(topo) UP(sys) UP(ng) UP(bk) UP(bl) UP(req) UP(plq) UP(a2f) UP(ang_ng) UP(ang_par) UP(dih_ng) UP(dih_par)
-    UP(inv_ng) UP(inv_par) UP(a2f_idx)
+    UP(inv_ng) UP(inv_par) UP(a2f_idx) UP(bk64) UP(bl64) UP(req64) UP(plq64) UP(ang64) UP(dih64) UP(inv64)
 #undef UP
     return hipSuccess;
 }
@@ -375,16 +387,28 @@ void fill_tables(const DevPools &D, const smr_gridff *grid, const smr_ff_options
     T.inv_ng = (const uint2 *)D.inv_ng.ptr;
     T.inv_par = (const float4 *)D.inv_par.ptr;
     T.a2f_idx = (const unsigned short *)D.a2f_idx.ptr;
+    T.bk64 = (const double4 *)D.bk64.ptr;
+    T.bl64 = (const double4 *)D.bl64.ptr;
+    T.req64 = (const double4 *)D.req64.ptr;
+    T.plq64 = (const double4 *)D.plq64.ptr;
+    T.ang64 = (const double4 *)D.ang64.ptr;
+    T.dih64 = (const double4 *)D.dih64.ptr;
+    T.inv64 = (const double4 *)D.inv64.ptr;
+    T.r2damp64 = o.rdamp * o.rdamp;
     if (grid && o.surface) {
         T.grid = (const float4 *)grid->coef.ptr;
         T.gnx = grid->desc.nx; T.gny = grid->desc.ny; T.gnz = grid->desc.nz;
         T.ginv_x = (float)(1.0 / grid->desc.step[0]);
         T.ginv_y = (float)(1.0 / grid->desc.step[1]);
         T.ginv_z = (float)(1.0 / grid->desc.step[2]);
+        T.ginv64_x = 1.0 / grid->desc.step[0];
+        T.ginv64_y = 1.0 / grid->desc.step[1];
+        T.ginv64_z = 1.0 / grid->desc.step[2];
     } else {
         T.grid = nullptr;
         T.gnx = T.gny = T.gnz = 1;
         T.ginv_x = T.ginv_y = T.ginv_z = 1.f;
+        T.ginv64_x = T.ginv64_y = T.ginv64_z = 1.0;
     }
     T.r2damp = (float)(o.rdamp * o.rdamp);
     T.use_pairs = o.nonbonded ? 1 : 0;
@@ -412,32 +436,11 @@ void system_origin(const double *x, int n, const smr_gridff *grid, bool surface,
     }
 }
 
-LaunchShape choose_shape(const smr_ctx *c, const smr_batch *b, int nwork) {
-    const int max_atoms = b->max_atoms, nf_max = b->nf_max;
-    LaunchShape L;
-    if (max_atoms <= 64) { L.block = 64; L.apt = 1; }
-    else if (max_atoms <= 128) { L.block = 128; L.apt = 1; }
-    else if (max_atoms <= 256) { L.block = 256; L.apt = 1; }
-    else if (max_atoms <= 512) { L.block = 256; L.apt = 2; }
-    else { L.block = 256; L.apt = 4; }
-    // caller / autotuner override of the threads per workgroup (atoms per
-    // thread follows) and of the stencil cache
-    if (b->tune_block > 0) {
-        int apt = (max_atoms + b->tune_block - 1) / b->tune_block;
-        if (apt <= 1) apt = 1; else if (apt <= 2) apt = 2; else apt = 4;
-        if (relax_kernel_ptr(b->tune_block, apt, true, false)) { L.block = b->tune_block; L.apt = apt; }
-    }
-    // the register stencil cache halves occupancy (256 VGPRs): off unless asked for
-    L.stencil_cache = (L.apt == 1) && (b->tune_cache == 1) && b->grid;
-    size_t with = relax_lds_bytes(L.block, max_atoms, nf_max, b->na_max, b->nd_max, b->ni_max, true);
-    L.fint_lds = with <= std::min<size_t>(c->max_lds, 64 * 1024);
-    L.lds_bytes = relax_lds_bytes(L.block, max_atoms, nf_max, b->na_max, b->nd_max, b->ni_max, L.fint_lds);
-    int occ = 0;
-    const void *k = relax_kernel_ptr(L.block, L.apt, L.fint_lds, L.stencil_cache);
-    if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, L.block, L.lds_bytes) != hipSuccess || occ < 1) occ = 1;
-    long long cap = (long long)occ * c->num_cu;
-    L.grid = (int)std::max<long long>(1, std::min<long long>(cap, nwork));
-    return L;
+// launch shapes of a batch's classes (after a change of the tuning knobs)
+void refresh_shapes(smr_batch *b) {
+    for (auto &p : b->plans)
+        p.shape = class_shape(b->ctx, p, b->grid != nullptr, b->tune_block, b->tune_cache, b->tune_pairs);
+    b->ls.drop_graph();
 }
 
 }  // namespace smr
@@ -505,8 +508,14 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->pos0.ptr, b->pos.ptr, sizeof(float4) * b->total_atoms, hipMemcpyDeviceToDevice, c->stream);
     if (e != hipSuccess) { delete b; ctx_release(c); return hip_fail(e, "smr_batch_create: initial positions"); }
-    std::vector<int> wl(nsys);
-    for (int s = 0; s < nsys; s++) wl[s] = s;
+    std::vector<SysSize> sizes(nsys);
+    for (int s = 0; s < nsys; s++) {
+        const TopoDesc &td = b->hp.topo[sys_topo[s]];
+        sizes[s] = SysSize{s, td.natoms, td.nf, td.nangles, td.ndih, td.ninv};
+    }
+    std::vector<int> wl;
+    build_plans(sizes, b->plans, wl);
+    refresh_shapes(b);
     e = hipMemcpyAsync(b->work.ptr, wl.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) { delete b; ctx_release(c); return hip_fail(e, "smr_batch_create: upload"); }
@@ -588,19 +597,8 @@ int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch,
         }
         HIPCHK(hipMemcpyAsync(b->state.ptr, h.data(), sizeof(SysState) * h.size(), hipMemcpyHostToDevice, st));
     }
-    LaunchShape L = choose_shape(c, b, b->nsys);
-    if (L.lds_bytes > c->max_lds)
-        return fail(SMR_ERR_UNSUPPORTED, "k_relax needs " + std::to_string(L.lds_bytes) + " B of LDS for " +
-                                             std::to_string(b->max_atoms) + " atoms; the device allows " +
-                                             std::to_string(c->max_lds));
-    if (!L.fint_lds) {
-        size_t need = sizeof(float) * 3 * (size_t)b->nf_max * L.grid;
-        if (b->fint_scratch.bytes < need) {
-            hipError_t e = b->fint_scratch.alloc(need);
-            if (e != hipSuccess) return hip_fail(e, "fint scratch");
-        }
-    }
     KernelArgs A;
+    memset(&A, 0, sizeof(A));
     A.tab = b->tab;
     A.fire = fc;
     A.pos = (float4 *)b->pos.ptr;
@@ -608,34 +606,18 @@ int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch,
     A.force = (float4 *)b->force.ptr;
     A.state = (SysState *)b->state.ptr;
     A.work_list = (const int *)b->work.ptr;
-    A.nwork = b->nsys;
-    A.work_counter = (int *)c->counter.ptr;
-    A.fint_scratch = (float *)b->fint_scratch.ptr;
-    A.nf_max = std::max(1, b->nf_max);
-    A.lds_atoms = b->max_atoms;
-    A.lds_na = b->na_max;
-    A.lds_nd = b->nd_max;
-    A.lds_ni = b->ni_max;
     A.steps_per_launch = steps_per_launch;
     A.mode = mode;
-    HIPCHK(hipMemsetAsync(c->counter.ptr, 0, sizeof(int), st));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (kernel_ms) {
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, st));
-    }
-    hipError_t e = launch_relax(A, L, st);
-    if (e != hipSuccess) return hip_fail(e, "launch k_relax");
-    if (kernel_ms) {
-        HIPCHK(hipEventRecord(e1, st));
-        HIPCHK(hipEventSynchronize(e1));
-        HIPCHK(hipEventElapsedTime(kernel_ms, e0, e1));
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-    }
-    HIPCHK(hipStreamSynchronize(st));
-    b->last_shape = L;
+    return launch_plans(c->ss, b->ls, b->plans, A, kernel_ms);
+}
+
+// FP64 energies of every system at its current positions (into state[].E)
+int eval_energies64(smr_batch *b) {
+    hipStream_t st = b->ctx->stream;
+    hipError_t e = launch_energy64(b->tab, (const float4 *)b->pos.ptr, (SysState *)b->state.ptr,
+                                   (const int *)b->work.ptr, b->nsys, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "k_energy64");
     return SMR_OK;
 }
 
@@ -685,6 +667,8 @@ extern "C" int smr_batch_eval_forces(smr_batch *b, double *forces, double *energ
     FireConst fc = make_fire_const(nullptr, 1, 0.0);
     int rc = run_fused(b, KMODE_EVAL, fc, 1, false, 0, 0, nullptr);
     if (rc) return rc;
+    rc = eval_energies64(b);
+    if (rc) return rc;
     std::vector<float4> h(b->total_atoms);
     HIPCHK(hipMemcpy(h.data(), b->force.ptr, sizeof(float4) * h.size(), hipMemcpyDeviceToHost));
     for (int i = 0; i < b->total_atoms; i++) {
@@ -710,6 +694,8 @@ extern "C" int smr_relax_batched(smr_batch *b, const smr_fire_params *fire, int 
     HIPCHK(hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, b->ctx->stream));
     FireConst fc = make_fire_const(fire, max_steps, fconv);
     rc = run_fused(b, KMODE_FIRE, fc, max_steps, true, fire->dt0, fire->alpha0, &b->last_kernel_ms);
+    if (rc) return rc;
+    rc = eval_energies64(b);
     if (rc) return rc;
     return read_results(b, results);
 }
@@ -806,24 +792,159 @@ extern "C" int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fi
             break;
         }
     }
+    // energies of the final positions, FP64 (as for the batched path)
+    HIPCHK(hipMemcpyAsync(P.cvf + 12, &isys, sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(launch_energy64(b->tab, (const float4 *)b->pos.ptr, (SysState *)b->state.ptr, (const int *)(P.cvf + 12), 1, st));
+    SysState hs;
+    HIPCHK(hipMemcpyAsync(&hs, (SysState *)b->state.ptr + isys, sizeof(SysState), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     result->steps = steps;
     result->status = status;
     result->fmax = std::sqrt(cvf[3]);
-    // cvf[4..9] = Eb, Ea, Ed, Ei, Enb, Esurf
     double E = 0.0;
-    for (int q = 0; q < 6; q++) { result->terms[q] = cvf[4 + q]; E += cvf[4 + q]; }
+    for (int q = 0; q < 6; q++) { result->terms[q] = hs.E[q]; E += hs.E[q]; }
     result->energy = E;
     return SMR_OK;
 }
 
 extern "C" int smr_batch_last_launch(const smr_batch *b, int *shape4, float *kernel_ms) {
     if (!b || !shape4 || !kernel_ms) return fail(SMR_ERR_INVALID, "smr_batch_last_launch: NULL argument");
-    shape4[0] = b->last_shape.block;
-    shape4[1] = b->last_shape.apt;
-    shape4[2] = b->last_shape.grid;
-    shape4[3] = (int)b->last_shape.lds_bytes | (b->last_shape.stencil_cache ? (1 << 30) : 0);
+    // the class with the most systems
+    const ClassPlan *p = nullptr;
+    for (const auto &q : b->plans)
+        if (!p || q.count > p->count) p = &q;
+    const LaunchShape L = p ? p->shape : LaunchShape();
+    shape4[0] = L.block;
+    shape4[1] = L.apt;
+    shape4[2] = L.grid;
+    shape4[3] = (int)L.lds_bytes | (L.stencil_cache ? (1 << 30) : 0);
     *kernel_ms = b->last_kernel_ms;
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_plan(const smr_batch *b, int *nclass, int *info) {
+    if (!b || !nclass || !info) return fail(SMR_ERR_INVALID, "smr_batch_plan: NULL argument");
+    *nclass = (int)b->plans.size();
+    for (size_t q = 0; q < b->plans.size(); q++) {
+        const ClassPlan &p = b->plans[q];
+        int *o = info + 8 * q;
+        o[0] = p.cls;
+        o[1] = p.count;
+        o[2] = p.max_atoms;
+        o[3] = p.shape.block;
+        o[4] = p.shape.apt;
+        o[5] = p.shape.grid;
+        o[6] = (int)p.shape.lds_bytes;
+        o[7] = (p.shape.fint_lds ? 1 : 0) | (p.shape.pair_mode ? 2 : 0) | (p.shape.stencil_cache ? 4 : 0);
+    }
+    return SMR_OK;
+}
+
+// Autotuner: for every size class of the batch, time each launch shape that
+// holds the class (threads per workgroup x atoms per thread at the class's
+// capacity and one step up, pair sweep or ordered pairs, force pieces in LDS
+// or in global scratch) on a FIRE run of trial_steps steps from the batch's
+// current state, and keep the fastest in the context.  Positions, velocities
+// and FIRE state are restored afterwards.
+extern "C" int smr_batch_autotune(smr_batch *b, const smr_fire_params *fire, int trial_steps) {
+    if (!b) return fail(SMR_ERR_INVALID, "smr_batch_autotune: NULL argument");
+    int rc = check_fire(fire, "smr_batch_autotune");
+    if (rc) return rc;
+    if (trial_steps <= 0 || trial_steps > 100000)
+        return fail(SMR_ERR_INVALID, "smr_batch_autotune: trial_steps must be in [1, 100000]");
+    smr_ctx *c = b->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const size_t apos_b = sizeof(float4) * b->total_atoms, st_b = sizeof(SysState) * b->nsys;
+    DevBuf kpos, kvel, kst;
+    HIPCHK(kpos.alloc(apos_b));
+    HIPCHK(kvel.alloc(apos_b));
+    HIPCHK(kst.alloc(st_b));
+    HIPCHK(hipMemcpyAsync(kpos.ptr, b->pos.ptr, apos_b, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(kvel.ptr, b->vel.ptr, apos_b, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(kst.ptr, b->state.ptr, st_b, hipMemcpyDeviceToDevice, st));
+    std::vector<SysState> h0(b->nsys);
+    for (auto &x : h0) {
+        memset(&x, 0, sizeof(x));
+        x.dt = fire->dt0;
+        x.alpha = fire->alpha0;
+        x.status = SMR_SYS_RUNNING;
+    }
+    // fconv = 0: nobody converges inside the trial
+    FireConst fc = make_fire_const(fire, 1 << 30, 0.0);
+    KernelArgs A;
+    memset(&A, 0, sizeof(A));
+    A.tab = b->tab;
+    A.fire = fc;
+    A.pos = (float4 *)b->pos.ptr;
+    A.vel = (float4 *)b->vel.ptr;
+    A.force = (float4 *)b->force.ptr;
+    A.state = (SysState *)b->state.ptr;
+    A.work_list = (const int *)b->work.ptr;
+    A.steps_per_launch = trial_steps;
+    A.mode = KMODE_FIRE;
+    static const int shapes[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {128, 4}, {256, 1}, {256, 2}, {256, 4}};
+    LaunchSet ls;
+    ls.use_graph = false;
+    auto trial = [&](const ClassPlan &p, float &ms) -> int {
+        HIPCHK(hipMemcpyAsync(b->pos.ptr, kpos.ptr, apos_b, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemsetAsync(b->vel.ptr, 0, apos_b, st));
+        HIPCHK(hipMemcpyAsync(b->state.ptr, h0.data(), st_b, hipMemcpyHostToDevice, st));
+        std::vector<ClassPlan> one(1, p);
+        return launch_plans(c->ss, ls, one, A, &ms);
+    };
+    int rc2 = SMR_OK;
+    for (auto &p : b->plans) {
+        const int cap0 = kClassMaxAtoms[p.cls];
+        std::vector<LaunchShape> cand;
+        for (auto &sh : shapes) {
+            const int cap = sh[0] * sh[1];
+            if (cap < p.max_atoms || cap > 2 * cap0) continue;
+            for (int pm = 1; pm >= 0; pm--)
+                for (int fl = 1; fl >= 0; fl--) {
+                    LaunchShape L = make_shape(c, p, sh[0], sh[1], fl, pm, false, b->grid != nullptr);
+                    if (L.block) cand.push_back(L);
+                }
+        }
+        float best = 1e30f;
+        LaunchShape win;
+        win.block = 0;
+        for (size_t q = 0; q < cand.size() && rc2 == SMR_OK; q++) {
+            ClassPlan tp = p;
+            tp.shape = cand[q];
+            float ms = 0.f, ms2 = 0.f;
+            rc2 = trial(tp, ms);
+            if (rc2 == SMR_OK) rc2 = trial(tp, ms2);   // the first run also warms caches
+            if (rc2 == SMR_OK && std::min(ms, ms2) < best) {
+                best = std::min(ms, ms2);
+                win = cand[q];
+            }
+        }
+        if (rc2 != SMR_OK) break;
+        if (win.block) {
+            TunedShape &t = c->tuned[p.cls];
+            t.valid = true;
+            t.block = win.block;
+            t.apt = win.apt;
+            t.fint_lds = win.fint_lds;
+            t.pair_mode = win.pair_mode;
+        }
+    }
+    // restore the batch whatever happened above
+    (void)hipMemcpyAsync(b->pos.ptr, kpos.ptr, apos_b, hipMemcpyDeviceToDevice, st);
+    (void)hipMemcpyAsync(b->vel.ptr, kvel.ptr, apos_b, hipMemcpyDeviceToDevice, st);
+    (void)hipMemcpyAsync(b->state.ptr, kst.ptr, st_b, hipMemcpyDeviceToDevice, st);
+    hipError_t e = hipStreamSynchronize(st);
+    if (rc2 != SMR_OK) return rc2;
+    if (e != hipSuccess) return hip_fail(e, "smr_batch_autotune: restore");
+    refresh_shapes(b);
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_set_graph(smr_batch *b, int use_graph) {
+    if (!b) return fail(SMR_ERR_INVALID, "smr_batch_set_graph: NULL argument");
+    b->ls.drop_graph();
+    b->ls.use_graph = use_graph != 0;
     return SMR_OK;
 }
 
@@ -837,13 +958,17 @@ extern "C" int smr_batch_reset(smr_batch *b) {
     return SMR_OK;
 }
 
-extern "C" int smr_batch_set_launch(smr_batch *b, int block, int stencil_cache) {
+extern "C" int smr_batch_set_launch(smr_batch *b, int block, int stencil_cache, int pair_sweep) {
     if (!b) return fail(SMR_ERR_INVALID, "smr_batch_set_launch: NULL argument");
     if (block != 0 && block != 64 && block != 128 && block != 256)
         return fail(SMR_ERR_INVALID, "smr_batch_set_launch: block " + std::to_string(block) + " not in {0,64,128,256}");
     if (stencil_cache < -1 || stencil_cache > 1)
         return fail(SMR_ERR_INVALID, "smr_batch_set_launch: stencil_cache must be -1, 0 or 1");
+    if (pair_sweep < -1 || pair_sweep > 1)
+        return fail(SMR_ERR_INVALID, "smr_batch_set_launch: pair_sweep must be -1, 0 or 1");
     b->tune_block = block;
     b->tune_cache = stencil_cache;
+    b->tune_pairs = pair_sweep;
+    refresh_shapes(b);
     return SMR_OK;
 }

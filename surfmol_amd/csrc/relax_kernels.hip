@@ -6,6 +6,8 @@
 //               many steps without leaving the kernel.  Positions, REQ,
 //               hneigh and the bonded force pieces live in LDS; per-system
 //               reductions are wave shuffles (+ LDS across waves) in FP64.
+//               Forces only: integration needs no energy, and the energies
+//               reported at the end are evaluated in FP64 by k_energy64.
 //               Systems are taken from a work list through an atomic
 //               counter, so uneven systems balance across the chip.
 //  k_plain_*    the plain method: one system, one kernel per force term per
@@ -24,8 +26,6 @@ namespace smr {
 template <int BLOCK, int APT, bool FINT_LDS, bool SCACHE>
 __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
     constexpr int NW = BLOCK / 64;
-    // one wave holds the whole system: unordered-pair sweep
-    constexpr bool HALF_PAIRS = (BLOCK == 64 && APT == 1);
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
@@ -35,14 +35,23 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
     float4 *s_pos = reinterpret_cast<float4 *>(smem);
     float4 *s_req = s_pos + A.lds_atoms;
     float4 *s_h = s_req + A.lds_atoms;                       // [4*natoms] hneigh
-    float4 *s_fp = s_h + 4 * A.lds_atoms;                    // [natoms] pair reaction forces
-    double *s_red = reinterpret_cast<double *>(s_fp + A.lds_atoms);     // [NW*8]
+    float4 *s_fp = s_h + 4 * A.lds_atoms;                    // [NWP][natoms] pair reaction forces
+    const int NWP = A.pair_mode ? NW : 1;                    // one reaction buffer per wave
+    double *s_red = reinterpret_cast<double *>(s_fp + NWP * A.lds_atoms);  // [NW*8]
     int *s_next = reinterpret_cast<int *>(s_red + NW * 8);
+    // interaction slots and the piece-gather list: staged in LDS, except for
+    // the widest shapes (4 atoms per thread, up to 1024 atoms), whose tables
+    // are read through the caches instead so that the atom data fits
+    constexpr bool TABLES_LDS = (APT < 4);
+    const int tn_d = TABLES_LDS ? A.lds_nd : 0, tn_i = TABLES_LDS ? A.lds_ni : 0, tn_a = TABLES_LDS ? A.lds_na : 0;
     uint2 *s_dih = reinterpret_cast<uint2 *>(s_next + 4);                // packed slots
-    uint2 *s_inv = s_dih + A.lds_nd;
-    unsigned *s_ang = reinterpret_cast<unsigned *>(s_inv + A.lds_ni);
-    float *s_fx = reinterpret_cast<float *>(s_ang + A.lds_na);
+    uint2 *s_inv = s_dih + tn_d;
+    unsigned *s_ang = reinterpret_cast<unsigned *>(s_inv + tn_i);
+    float *s_fx = reinterpret_cast<float *>(s_ang + tn_a);
     unsigned short *s_a2f = reinterpret_cast<unsigned short *>(s_fx + (FINT_LDS ? 3 * A.nf_max : 0));
+    const uint2 *t_dih, *t_inv;
+    const unsigned *t_ang;
+    const unsigned short *t_a2f;
     float *fint_x, *fint_y, *fint_z;
 
     while (true) {
@@ -94,10 +103,16 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
             }
         }
         // interaction slots and the piece-gather list stay in LDS for all steps
-        for (int q = tid; q < td.nangles; q += BLOCK) s_ang[q] = T.ang_ng[td.ang_off + q];
-        for (int q = tid; q < td.ndih; q += BLOCK) s_dih[q] = T.dih_ng[td.dih_off + q];
-        for (int q = tid; q < td.ninv; q += BLOCK) s_inv[q] = T.inv_ng[td.inv_off + q];
-        for (int q = tid; q < td.nf; q += BLOCK) s_a2f[q] = T.a2f_idx[td.a2f_off + q];
+        if constexpr (TABLES_LDS) {
+            for (int q = tid; q < td.nangles; q += BLOCK) s_ang[q] = T.ang_ng[td.ang_off + q];
+            for (int q = tid; q < td.ndih; q += BLOCK) s_dih[q] = T.dih_ng[td.dih_off + q];
+            for (int q = tid; q < td.ninv; q += BLOCK) s_inv[q] = T.inv_ng[td.inv_off + q];
+            for (int q = tid; q < td.nf; q += BLOCK) s_a2f[q] = T.a2f_idx[td.a2f_off + q];
+            t_ang = s_ang; t_dih = s_dih; t_inv = s_inv; t_a2f = s_a2f;
+        } else {
+            t_ang = T.ang_ng + td.ang_off; t_dih = T.dih_ng + td.dih_off;
+            t_inv = T.inv_ng + td.inv_off; t_a2f = T.a2f_idx + td.a2f_off;
+        }
         __syncthreads();
 
         // per-lane cache of the atom's PLQH-contracted 4x4x4 stencil; re-read
@@ -106,25 +121,23 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
         float sv[CACHE ? 64 : 1];
         int skey = -1;
         const int nsteps = A.steps_per_launch;
-        double E6[6];
         double fmax_cur = st.fmax;
         int step_in_launch = 0;
         for (;;) {
-#pragma unroll
-            for (int q = 0; q < 6; q++) E6[q] = 0.0;
             // ---------------- phase A: bonds + hneigh ----------------
+            if (A.pair_mode)
+                for (int w = 0; w < NWP; w++)
+                    for (int q = tid; q < N; q += BLOCK) s_fp[w * A.lds_atoms + q] = make_float4(0.f, 0.f, 0.f, 0.f);
             float3 fb[APT];
 #pragma unroll
             for (int k = 0; k < APT; k++) {
                 const int i = tid + k * BLOCK;
                 fb[k] = make_float3(0.f, 0.f, 0.f);
                 if (i < N) {
-                    s_fp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
                     const float3 pi = f3(s_pos[i]);
                     const int ngs[4] = {my_ng[k].x, my_ng[k].y, my_ng[k].z, my_ng[k].w};
                     const float bks[4] = {my_bk[k].x, my_bk[k].y, my_bk[k].z, my_bk[k].w};
                     const float bls[4] = {my_bl[k].x, my_bl[k].y, my_bl[k].z, my_bl[k].w};
-                    float eb = 0.f;
 #pragma unroll
                     for (int sl = 0; sl < 4; sl++) {
                         if (ngs[sl] < 0) {
@@ -132,10 +145,9 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                             continue;
                         }
                         float4 hn;
-                        eb += bond_term(pi, f3(s_pos[ngs[sl]]), bks[sl], bls[sl], hn, fb[k]);
+                        (void)bond_term(pi, f3(s_pos[ngs[sl]]), bks[sl], bls[sl], hn, fb[k]);
                         s_h[4 * i + sl] = hn;
                     }
-                    E6[0] += (double)eb;
                 }
             }
             __syncthreads();
@@ -143,72 +155,77 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
             {
                 const int i0inv = 4 * td.ndih, i0ang = i0inv + 4 * td.ninv;
                 for (int a = tid; a < td.nangles; a += BLOCK) {
-                    const unsigned g = s_ang[a];
+                    const unsigned g = t_ang[a];
                     float3 fpi, fpj, fpk;
-                    float e = angle_term(s_h[g & 0xffffu], s_h[g >> 16], T.ang_par[td.ang_off + a], fpi, fpj, fpk);
+                    (void)angle_term(s_h[g & 0xffffu], s_h[g >> 16], T.ang_par[td.ang_off + a], fpi, fpj, fpk);
                     const int p = i0ang + 3 * a;
                     fint_x[p] = fpi.x; fint_y[p] = fpi.y; fint_z[p] = fpi.z;
                     fint_x[p + 1] = fpj.x; fint_y[p + 1] = fpj.y; fint_z[p + 1] = fpj.z;
                     fint_x[p + 2] = fpk.x; fint_y[p + 2] = fpk.y; fint_z[p + 2] = fpk.z;
-                    E6[1] += (double)e;
                 }
                 for (int d = tid; d < td.ndih; d += BLOCK) {
-                    const uint2 g = s_dih[d];
+                    const uint2 g = t_dih[d];
                     float3 f1, f2, f3v, f4;
-                    float e = dihedral_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y & 0xffffu], T.dih_par[td.dih_off + d],
+                    (void)dihedral_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y & 0xffffu], T.dih_par[td.dih_off + d],
                                             (int)(g.y >> 16), f1, f2, f3v, f4);
                     const int p = 4 * d;
                     fint_x[p] = f1.x; fint_y[p] = f1.y; fint_z[p] = f1.z;
                     fint_x[p + 1] = f2.x; fint_y[p + 1] = f2.y; fint_z[p + 1] = f2.z;
                     fint_x[p + 2] = f3v.x; fint_y[p + 2] = f3v.y; fint_z[p + 2] = f3v.z;
                     fint_x[p + 3] = f4.x; fint_y[p + 3] = f4.y; fint_z[p + 3] = f4.z;
-                    E6[2] += (double)e;
                 }
                 for (int v = tid; v < td.ninv; v += BLOCK) {
-                    const uint2 g = s_inv[v];
+                    const uint2 g = t_inv[v];
                     float3 f1, f2, f3v, f4;
-                    float e = inversion_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y], T.inv_par[td.inv_off + v], f1, f2,
+                    (void)inversion_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y], T.inv_par[td.inv_off + v], f1, f2,
                                              f3v, f4);
                     const int p = i0inv + 4 * v;
                     fint_x[p] = f1.x; fint_y[p] = f1.y; fint_z[p] = f1.z;
                     fint_x[p + 1] = f2.x; fint_y[p + 1] = f2.y; fint_z[p + 1] = f2.z;
                     fint_x[p + 2] = f3v.x; fint_y[p + 2] = f3v.y; fint_z[p + 2] = f3v.z;
                     fint_x[p + 3] = f4.x; fint_y[p + 3] = f4.y; fint_z[p + 3] = f4.z;
-                    E6[3] += (double)e;
                 }
             }
             __syncthreads();
             // ---------------- phase C0: pairs, each unordered pair once ----------------
-            // For a system held by one wave: on sweep d lane i pairs with
-            // j = (i+d) mod N.  For fixed d this map is a permutation, so the
-            // reaction force -f_ij goes to a distinct LDS slot per lane (no
-            // atomics, fixed order).  Sweeps d = 1..floor(N/2) visit every
-            // unordered pair once (for even N the last sweep uses lanes i < N/2).
-            // The pair energy is counted at both ends, as getNonBond does.
-            KahanF3 pacc;
-            pacc.init(make_float3(0.f, 0.f, 0.f));
-            if (HALF_PAIRS && T.use_pairs) {
-                const int i = tid;
-                double enb = 0.0;
+            // On sweep d the thread of atom i pairs it with j = (i+d) mod N.
+            // For fixed d this map is a permutation, so within one wave (all
+            // lanes on the same d and k) the reaction force -f_ij goes to a
+            // distinct slot of that wave's own LDS buffer: no atomics and a
+            // fixed summation order.  Sweeps d = 1..floor(N/2) visit every
+            // unordered pair once (for even N the last sweep uses i < N/2).
+            float3 pf[APT];
+#pragma unroll
+            for (int k = 0; k < APT; k++) pf[k] = make_float3(0.f, 0.f, 0.f);
+            if (A.pair_mode && T.use_pairs) {
+                float4 *fpw = s_fp + wid * A.lds_atoms;
                 const int nd = N >> 1;
-                const float4 pi4 = (i < N) ? s_pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const float4 ri = (i < N) ? s_req[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const int4 ng = my_ng[0];
-                int j = i;
-                for (int d = 1; d <= nd; d++) {
-                    j = (j + 1 == N) ? 0 : j + 1;
-                    const bool active = (i < N) && !((N & 1) == 0 && d == nd && i >= nd);
-                    if (active && !(j == ng.x || j == ng.y || j == ng.z || j == ng.w)) {
-                        float3 f;
-                        float e = ljq_pair(f3(s_pos[j]) - f3(pi4), ri, s_req[j], T.r2damp, f);
-                        pacc.add(f);
-                        float4 fj = s_fp[j];
-                        fj.x -= f.x; fj.y -= f.y; fj.z -= f.z;
-                        s_fp[j] = fj;
-                        enb += 2.0 * (double)e;
+                const bool even = (N & 1) == 0;
+#pragma unroll
+                for (int k = 0; k < APT; k++) {
+                    if (k * BLOCK + wid * 64 >= N) break;   // wave-uniform: no atom of this wave left
+                    const int i = tid + k * BLOCK;
+                    const bool own = i < N;
+                    const float4 pi4 = own ? s_pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    const float4 ri = own ? s_req[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    const int4 ng = my_ng[k];
+                    KahanF3 pacc;
+                    pacc.init(make_float3(0.f, 0.f, 0.f));
+                    int j = own ? i : 0;
+                    for (int d = 1; d <= nd; d++) {
+                        j = (j + 1 == N) ? 0 : j + 1;
+                        const bool active = own && !(even && d == nd && i >= nd);
+                        if (active && !(j == ng.x || j == ng.y || j == ng.z || j == ng.w)) {
+                            float3 f;
+                            (void)ljq_pair(f3(s_pos[j]) - f3(pi4), ri, s_req[j], T.r2damp, f);
+                            pacc.add(f);
+                            float4 fj = fpw[j];
+                            fj.x -= f.x; fj.y -= f.y; fj.z -= f.z;
+                            fpw[j] = fj;
+                        }
                     }
+                    pf[k] = pacc.value();
                 }
-                E6[4] += enb;
                 __syncthreads();
             }
             // ---------------- phase C: gather pieces + pairs + surface ----------------
@@ -220,30 +237,27 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                 if (i < N) {
                     KahanF3 acc;
                     acc.init(fb[k]);
-                    if (HALF_PAIRS && T.use_pairs) {
-                        acc.add(pacc.value());
-                        acc.add(f3(s_fp[i]));
+                    if (A.pair_mode && T.use_pairs) {
+                        acc.add(pf[k]);
+                        for (int w = 0; w < NW; w++) acc.add(f3(s_fp[w * A.lds_atoms + i]));
                     }
-                    const unsigned short *idx = s_a2f + my_a2f[k].x;
+                    const unsigned short *idx = t_a2f + my_a2f[k].x;
                     for (int q = 0; q < my_a2f[k].y; q++) {
                         const int p = idx[q];
                         acc.add(make_float3(fint_x[p], fint_y[p], fint_z[p]));
                     }
                     const float4 pi4 = s_pos[i];
                     const float3 pi = f3(pi4);
-                    if (T.use_pairs && !HALF_PAIRS) {
+                    if (T.use_pairs && !A.pair_mode) {
                         // every ordered pair (i, j), as getNonBond does
                         const float4 ri = s_req[i];
                         const int4 ng = my_ng[k];
-                        double enb = 0.0;
                         for (int j = 0; j < N; j++) {
                             if (j == i || j == ng.x || j == ng.y || j == ng.z || j == ng.w) continue;
                             float3 f;
-                            float e = ljq_pair(f3(s_pos[j]) - pi, ri, s_req[j], T.r2damp, f);
+                            (void)ljq_pair(f3(s_pos[j]) - pi, ri, s_req[j], T.r2damp, f);
                             acc.add(f);
-                            enb += (double)e;
                         }
-                        E6[4] += enb;
                     }
                     if (T.grid) {
                         float3 ul = make_float3(pi.x * T.ginv_x, pi.y * T.ginv_y, pi.z * T.ginv_z);
@@ -260,7 +274,6 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                             fe = grid_sample(T.grid, T.gnx, T.gny, T.gnz, sd.iux, sd.iuy, ul, my_plq[k]);
                         }
                         acc.add(make_float3(-fe.x * T.ginv_x, -fe.y * T.ginv_y, -fe.z * T.ginv_z));
-                        E6[5] += (double)fe.w;
                     }
                     F[k] = acc.value();
                 }
@@ -281,20 +294,11 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     }
                 f2m = wave_max(f2m);
                 if (lane == 0) s_red[wid * 8 + 6] = f2m;
-#pragma unroll
-                for (int q = 0; q < 6; q++) {
-                    double r = wave_sum(E6[q]);
-                    if (lane == 0) s_red[wid * 8 + q] = r;
-                }
                 __syncthreads();
                 if (tid == 0) {
                     double m = 0.0;
                     for (int w = 0; w < NW; w++) m = fmax(m, s_red[w * 8 + 6]);
-                    for (int q = 0; q < 6; q++) {
-                        double acc = 0.0;
-                        for (int w = 0; w < NW; w++) acc += s_red[w * 8 + q];
-                        st.E[q] = acc;
-                    }
+                    // energies: k_energy64 at the same positions
                     st.fmax = sqrt(m);
                     A.state[s] = st;
                 }
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     }
                 }
             }
-            // block reduction of (vf, vv, ff, fmax^2) and the energy terms
+            // block reduction of (vf, vv, ff, fmax^2)
             {
                 double r0 = wave_sum(p_vf), r1 = wave_sum(p_vv), r2 = wave_sum(p_ff), r3 = wave_max(p_fm);
                 if (NW > 1) {
@@ -426,19 +430,9 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
             }
             const bool leave = done || step_in_launch >= nsteps;
             if (leave) {
-                // energies of this (last) force evaluation
-#pragma unroll
-                for (int q = 0; q < 6; q++) {
-                    double r = wave_sum(E6[q]);
-                    if (lane == 0) s_red[wid * 8 + q] = r;
-                }
-                __syncthreads();
+                // the energies of the final positions are evaluated in FP64
+                // afterwards (k_energy64)
                 if (tid == 0) {
-                    for (int q = 0; q < 6; q++) {
-                        double acc = 0.0;
-                        for (int w = 0; w < NW; w++) acc += s_red[w * 8 + q];
-                        st.E[q] = acc;
-                    }
                     st.fmax = fmax_cur;
                     A.state[s] = st;
                 }
@@ -642,15 +636,16 @@ static const void *kernel_ptr() {
 }
 
 // instantiated shapes: (threads per workgroup, atoms per thread); the
-// stencil-cache variant exists for one atom per thread
-#define SMR_SHAPES(X) X(64, 1) X(128, 1) X(256, 1) X(256, 2) X(256, 4)
+// stencil-cache variant exists for one atom per thread only
+#define SMR_SHAPES(X) X(64, 1) X(64, 2) X(64, 4) X(128, 1) X(128, 2) X(128, 4) X(256, 1) X(256, 2) X(256, 4)
 
 hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t stream) {
 #define SMR_CASE(B, P)                                                                                   \
     if (L.block == B && L.apt == P) {                                                                    \
-        if (P == 1 && L.stencil_cache)                                                                   \
-            return L.fint_lds ? launch_one<B, P, true, true>(A, L.grid, L.lds_bytes, stream)             \
-                              : launch_one<B, P, false, true>(A, L.grid, L.lds_bytes, stream);           \
+        constexpr bool SCOK = (P == 1);                                                                  \
+        if (SCOK && L.stencil_cache)                                                                     \
+            return L.fint_lds ? launch_one<B, P, true, SCOK>(A, L.grid, L.lds_bytes, stream)             \
+                              : launch_one<B, P, false, SCOK>(A, L.grid, L.lds_bytes, stream);           \
         return L.fint_lds ? launch_one<B, P, true, false>(A, L.grid, L.lds_bytes, stream)                \
                           : launch_one<B, P, false, false>(A, L.grid, L.lds_bytes, stream);              \
     }
@@ -662,12 +657,26 @@ hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t s
 const void *relax_kernel_ptr(int block, int apt, bool fint_lds, bool stencil_cache) {
 #define SMR_CASE(B, P)                                                                                   \
     if (block == B && apt == P) {                                                                        \
-        if (P == 1 && stencil_cache) return fint_lds ? kernel_ptr<B, P, true, true>() : kernel_ptr<B, P, false, true>(); \
+        constexpr bool SCOK = (P == 1);                                                                  \
+        if (SCOK && stencil_cache) return fint_lds ? kernel_ptr<B, P, true, SCOK>() : kernel_ptr<B, P, false, SCOK>(); \
+        if (!SCOK && stencil_cache) return nullptr;                                                      \
         return fint_lds ? kernel_ptr<B, P, true, false>() : kernel_ptr<B, P, false, false>();            \
     }
     SMR_SHAPES(SMR_CASE)
 #undef SMR_CASE
     return nullptr;
+}
+
+// zero the per-class work counters (a kernel, not a memset: the launch
+// sequence may be captured into a graph)
+__global__ void k_zero_i32(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero_i32(int *p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_plain_step_forces(const PlainArgs &P, int natoms, int nang, int ndih, int ninv, hipStream_t st) {

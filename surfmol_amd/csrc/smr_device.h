@@ -73,6 +73,13 @@ struct DevTables {
     // options
     float r2damp;
     int use_pairs;
+    // FP64 parameters of the energy evaluation (k_energy64): per atom bond k
+    // and l0 per neighbour slot, (R, sqrtE, sqrt(kC) Q, 0), PLQH; per term
+    // angle (K, c0, c2, c3), dihedral (V, d, n, 0), inversion (K, C0, C1, C2)
+    const double4 *bk64, *bl64, *req64, *plq64;
+    const double4 *ang64, *dih64, *inv64;
+    double ginv64_x, ginv64_y, ginv64_z;
+    double r2damp64;
 };
 
 // ---------------------------------------------------------------------------

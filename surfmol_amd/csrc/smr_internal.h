@@ -43,16 +43,81 @@ struct HostPools {
     std::vector<uint2> inv_ng;
     std::vector<float4> inv_par;
     std::vector<unsigned short> a2f_idx;
+    // FP64 parameters for the energy evaluation
+    std::vector<double4> bk64, bl64, req64, plq64, ang64, dih64, inv64;
 };
 
 struct DevPools {
     DevBuf topo, sys, ng, bk, bl, req, plq, a2f, ang_ng, ang_par, dih_ng, dih_par, inv_ng, inv_par, a2f_idx;
+    DevBuf bk64, bl64, req64, plq64, ang64, dih64, inv64;
 };
+
+// ---- size classes: systems are grouped by atom count; each class has its
+//      own launch shape and its own persistent launch ----
+constexpr int NCLASS = 5;
+constexpr int kClassMaxAtoms[NCLASS] = {64, 128, 256, 512, 1024};
+int size_class(int natoms);
+
+struct ClassPlan {
+    int cls = 0;
+    int first = 0, count = 0;                 // slice of the work list
+    int max_atoms = 0, nf_max = 0, na_max = 0, nd_max = 0, ni_max = 0;
+    LaunchShape shape;
+    size_t scratch_off = 0;                   // floats into the force-piece scratch
+};
+
+// launch-shape choice of one class remembered by the context (autotuner)
+struct TunedShape {
+    bool valid = false;
+    int block = 0, apt = 0;
+    bool fint_lds = false, pair_mode = true;
+};
+
+// shape with LDS bytes and persistent grid filled in; block == 0 if the
+// combination does not exist or does not fit the device
+LaunchShape make_shape(const smr_ctx *c, const ClassPlan &cp, int block, int apt, bool fint_lds, bool pair_mode,
+                       bool stencil_cache, bool have_grid);
+// default / tuned / overridden shape of a class
+LaunchShape class_shape(const smr_ctx *c, const ClassPlan &cp, bool have_grid, int tune_block, int tune_cache,
+                        int tune_pairs);
+// group systems into classes; the work list is ordered by class (largest
+// class first) and by falling atom count inside a class
+struct SysSize {
+    int id, natoms, nf, na, nd, ni;
+};
+void build_plans(std::vector<SysSize> sys, std::vector<ClassPlan> &plans, std::vector<int> &work);
+
+// streams of one launcher (a context's batches, or a server's worker): the
+// main stream plus three class streams, all at the default priority, and the
+// fork/join events between them
+struct StreamSet {
+    hipStream_t main = nullptr;
+    hipStream_t aux[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+    hipError_t create();
+    void destroy();
+};
+
+// the launches of one relaxation chunk: counters zeroed by a kernel, one
+// persistent launch per class, classes on separate streams (fork/join on the
+// context stream), optionally captured once into a HIP graph
+struct LaunchSet {
+    DevBuf counters;              // int[NCLASS]
+    DevBuf scratch;               // force pieces that do not fit LDS
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<unsigned char> key;
+    bool use_graph = true;
+    ~LaunchSet();
+    void drop_graph();
+};
+int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
+                 float *kernel_ms);
 
 int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, HostPools &P, TopoDesc &td);
 hipError_t upload_pools(HostPools &P, DevPools &D, hipStream_t st);
+void fill_tables(const DevPools &D, const smr_gridff *grid, const smr_ff_options &o, DevTables &T);
 void system_origin(const double *x, int n, const smr_gridff *grid, bool surface, double org[3], int &iux, int &iuy);
-LaunchShape choose_shape(const smr_ctx *c, const smr_batch *b, int nwork);
 int check_fire(const smr_fire_params *f, const char *who);
 void ctx_retain(smr_ctx *c);
 void ctx_release(smr_ctx *c);
@@ -67,8 +132,9 @@ struct smr_ctx {
     int num_cu = 0;
     size_t max_lds = 0;
     std::string arch;
-    hipStream_t stream = nullptr;
-    smr::DevBuf counter;
+    hipStream_t stream = nullptr;   // == ss.main
+    smr::StreamSet ss;              // used by the context's batches (one caller thread)
+    smr::TunedShape tuned[smr::NCLASS];
 };
 
 struct smr_gridff {
@@ -88,10 +154,12 @@ struct smr_batch {
     int nsys = 0, total_atoms = 0, max_atoms = 0, nf_max = 0, na_max = 0, nd_max = 0, ni_max = 0;
     std::vector<int> atom_off;       // [nsys+1]
     std::vector<double> origin;      // [3*nsys]
-    smr::DevBuf pos, pos0, vel, force, state, work, fint_scratch;
+    smr::DevBuf pos, pos0, vel, force, state, work;
     smr::DevBuf plain_hneigh, plain_fint, plain_e, plain_cvf;
-    smr::LaunchShape last_shape;
+    std::vector<smr::ClassPlan> plans;
+    smr::LaunchSet ls;
     int tune_block = 0;              // 0: automatic
     int tune_cache = -1;             // -1: automatic
+    int tune_pairs = -1;             // -1: automatic
     float last_kernel_ms = 0.f;
 };

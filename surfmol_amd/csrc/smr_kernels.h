@@ -25,23 +25,30 @@ struct KernelArgs {
     int lds_na, lds_nd, lds_ni;  // interaction slots reserved per LDS slot
     int steps_per_launch;
     int mode;
+    int pair_mode;           // 1: unordered-pair sweep, per-wave reaction buffers; 0: ordered pairs
 };
 
 struct LaunchShape {
     int block = 0, apt = 0, grid = 0;
     bool fint_lds = false;
     bool stencil_cache = false;   // per-lane register cache of the surface stencil
+    bool pair_mode = true;        // unordered-pair sweep (needs a reaction buffer per wave)
     size_t lds_bytes = 0;
 };
 
 // bytes of dynamic LDS for a launch shape: positions, REQ rows, hneigh,
-// pair reaction forces, reduction scratch, packed interaction slots, force pieces (optional) and
-// the 16-bit piece-gather list
-inline size_t relax_lds_bytes(int block, int lds_atoms, int nf_max, int na, int nd, int ni, bool fint_lds) {
-    size_t b = (size_t)lds_atoms * 16 * 7 + (size_t)(block / 64) * 8 * 8 + 16;
-    b += (size_t)nd * 8 + (size_t)ni * 8 + (size_t)na * 4;
+// pair reaction forces (one buffer per wave in the unordered-pair sweep),
+// reduction scratch, packed interaction slots and the 16-bit piece-gather
+// list (not for 4 atoms per thread: those tables stay in global memory), and
+// the force pieces (optional)
+inline size_t relax_lds_bytes(int block, int apt, int lds_atoms, int nf_max, int na, int nd, int ni, bool fint_lds,
+                              bool pair_mode) {
+    const int nw = block / 64;
+    size_t b = (size_t)lds_atoms * 16 * (6 + (pair_mode ? nw : 1)) + (size_t)nw * 8 * 8 + 16;
+    const bool tables = apt < 4;
+    if (tables) b += (size_t)nd * 8 + (size_t)ni * 8 + (size_t)na * 4;
     if (fint_lds) b += (size_t)nf_max * 12;
-    b += ((size_t)nf_max * 2 + 15) & ~(size_t)15;
+    if (tables) b += ((size_t)nf_max * 2 + 15) & ~(size_t)15;
     return b;
 }
 
@@ -60,6 +67,10 @@ struct PlainArgs {
 
 hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t stream);
 const void *relax_kernel_ptr(int block, int apt, bool fint_lds, bool stencil_cache);
+hipError_t launch_zero_i32(int *p, int n, hipStream_t st);
+// FP64 energies (state[s].E) of the listed systems at their current positions
+hipError_t launch_energy64(const DevTables &T, const float4 *pos, SysState *state, const int *list, int n,
+                           hipStream_t st);
 hipError_t launch_plain_step_forces(const PlainArgs &P, int natoms, int nang, int ndih, int ninv, hipStream_t st);
 hipError_t launch_plain_reduce(const PlainArgs &P, double dt, double flim, hipStream_t st);
 hipError_t launch_plain_apply(const PlainArgs &P, int natoms, float a, float b, int reset, float dt, hipStream_t st);

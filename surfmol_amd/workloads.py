@@ -91,7 +91,13 @@ def _polyhex_graph(hexes, b=B_RING):
         for a in range(6):
             i, j = ids[a], ids[(a + 1) % 6]
             edges.add((min(i, j), max(i, j)))
-    return np.array(verts), sorted(edges)
+    verts = np.array(verts)
+    # lattice neighbours that no hexagon of the set joins (they face each
+    # other across an enclosed gap) are bonded too: the induced subgraph
+    from scipy.spatial import cKDTree
+    for i, j in cKDTree(verts).query_pairs(1.05 * b):
+        edges.add((min(i, j), max(i, j)))
+    return verts, sorted(edges)
 
 
 def _outward_dir(vi, verts, nbrs):
@@ -173,35 +179,56 @@ def random_polyhex(rng: np.random.Generator, target_atoms: int) -> Molecule:
     to the target.  Vertex elements: 3-ring-coordinated vertices are C (N with
     probability 0.05); 2-coordinated vertices are C-H (0.80), C=O (0.07),
     pyridinic N (0.08) or ring O (0.05)."""
-    hexes = [(0, 0)]
-    hexset = {(0, 0)}
+    from sortedcontainers import SortedList
 
-    def count(hx):
-        verts, edges = _polyhex_graph(hx)
-        deg = np.zeros(len(verts), dtype=int)
-        for i, j in edges:
-            deg[i] += 1
-            deg[j] += 1
-        return len(verts) + int((deg == 2).sum())   # upper estimate incl. substituents
+    # incremental graph of the animal: vertex degrees, edge set, and the
+    # free neighbour positions (kept sorted: the random pick indexes them)
+    deg = {}
+    edges = set()
+    state = {"nv": 0, "ndeg2": 0}
+    hexes, hexset, frontier = [], set(), SortedList()
 
-    best = list(hexes)
-    while True:
-        c = count(hexes)
-        if c >= target_atoms:
-            if abs(count(best) - target_atoms) < abs(c - target_atoms) and len(best) >= 1 and count(best) >= 20:
-                hexes = best
-            break
-        best = list(hexes)
-        cand = set()
-        for (q, r) in hexes:
-            for dq, dr in _HEX_NEIGH:
-                h = (q + dq, r + dr)
-                if h not in hexset:
-                    cand.add(h)
-        cand = sorted(cand)
-        h = cand[int(rng.integers(len(cand)))]
+    def add_hex(h):
         hexes.append(h)
         hexset.add(h)
+        if h in frontier:
+            frontier.remove(h)
+        for dq, dr in _HEX_NEIGH:
+            nb = (h[0] + dq, h[1] + dr)
+            if nb not in hexset and nb not in frontier:
+                frontier.add(nb)
+        vs = _hex_vertices(h[0], h[1])
+        keys = [(int(round(v[0] * 1000)), int(round(v[1] * 1000))) for v in vs]
+        for k in keys:
+            if k not in deg:
+                deg[k] = 0
+                state["nv"] += 1
+        for a in range(6):
+            e = tuple(sorted((keys[a], keys[(a + 1) % 6])))
+            if e in edges:
+                continue
+            edges.add(e)
+            for k in e:
+                if deg[k] == 2:
+                    state["ndeg2"] -= 1
+                deg[k] += 1
+                if deg[k] == 2:
+                    state["ndeg2"] += 1
+
+    def count():
+        return state["nv"] + state["ndeg2"]   # upper estimate incl. substituents
+
+    add_hex((0, 0))
+    prev = None
+    while True:
+        c = count()
+        if c >= target_atoms:
+            if prev is not None and abs(prev[1] - target_atoms) < abs(c - target_atoms) and prev[1] >= 20:
+                hexes = hexes[:prev[0]]
+            break
+        prev = (len(hexes), c)
+        h = frontier[int(rng.integers(len(frontier)))]
+        add_hex(h)
     verts, edges = _polyhex_graph(hexes)
     nv = len(verts)
     deg = np.zeros(nv, dtype=int)
@@ -225,6 +252,23 @@ def random_polyhex(rng: np.random.Generator, target_atoms: int) -> Molecule:
                 ring_elems.append("N")
             else:
                 ring_elems.append("O")
+    # coves: two 2-coordinated vertices whose missing honeycomb neighbour is
+    # the same lattice point would put their substituents 0.5 A apart; such
+    # vertices carry no substituent and become pyridinic N
+    adj = [[] for _ in range(nv)]
+    for i, j in edges:
+        adj[i].append(j)
+        adj[j].append(i)
+    missing = {}
+    for vi in range(nv):
+        if deg[vi] == 2:
+            m = verts[vi] + B_RING * _outward_dir(vi, verts, adj[vi])
+            missing.setdefault((int(round(m[0] * 100)), int(round(m[1] * 100))), []).append(vi)
+    for group in missing.values():
+        if len(group) > 1:
+            for vi in group:
+                subs.pop(vi, None)
+                ring_elems[vi] = "N"
     elems, pos, bonds = _assemble(verts, edges, ring_elems, subs)
     pos[:, :2] -= pos[:, :2].mean(axis=0)
     return _jitter_and_charge(rng, elems, pos, bonds)

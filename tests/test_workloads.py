@@ -47,3 +47,20 @@ def test_slab_is_neutral_and_commensurate():
 
 def test_fs_units():
     assert math.isclose(W.fs_to_internal(W.FS_PER_TIME_UNIT), 1.0)
+
+
+def test_polyhex_has_no_clashes():
+    """Fused-ring molecules: every pair of lattice neighbours is bonded (the
+    induced honeycomb subgraph), coves carry no substituents, so no two
+    non-bonded atoms come closer than 1.3 A; every atom has 1..3 bonds."""
+    from scipy.spatial.distance import pdist, squareform
+    mols = W.uneven_batch(96, seed=W.SEED_BASE + 9)
+    for m in mols:
+        D = squareform(pdist(m.apos))
+        np.fill_diagonal(D, np.inf)
+        bonded = np.zeros_like(D, dtype=bool)
+        bonded[m.bonds[:, 0], m.bonds[:, 1]] = True
+        bonded[m.bonds[:, 1], m.bonds[:, 0]] = True
+        assert D[~bonded].min() > 1.3
+        deg = np.bincount(m.bonds.flatten(), minlength=m.natoms)
+        assert deg.min() >= 1 and deg.max() <= 3
