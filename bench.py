@@ -10,6 +10,10 @@ Metric: relaxed systems/s at Fmax < 1e-3 eV/A.
 --workload uneven: 2048 random fused-ring molecules with atom counts
   log-uniform in [20, 400], xy U(0,48) A, height U(3,5) A, on a 48 x 48 x 20 A
   GridFF at 0.1 A (480 x 480 x 200 nodes); same FIRE settings.
+--workload serve: the serving loop -- relax requests with Poisson arrivals
+  (rates --rates, req/s), sizes lognormal (median 60 atoms, ln-sigma 0.5,
+  clipped to [20, 600]), on the 48 x 48 x 20 A GridFF; reports p50/p99
+  latency and served throughput per rate (one JSON line).
 
 One "step" of this script is one complete batched relaxation of every
 system of the workload (positions restored on the device before each).
@@ -79,7 +83,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["uniform", "uneven"], default="uniform")
+    ap.add_argument("--workload", choices=["uniform", "uneven", "serve"], default="uniform")
+    ap.add_argument("--rates", type=str, default="200,1000,5000", help="serve: arrival rates [req/s]")
+    ap.add_argument("--requests", type=int, default=1000, help="serve: requests per rate")
+    ap.add_argument("--chunk-steps", type=int, default=64, help="serve: FIRE steps per persistent launch")
     ap.add_argument("--nsys", type=int, default=None)
     ap.add_argument("--max-steps", type=int, default=2000)
     ap.add_argument("--autotune", action="store_true", help="time every launch shape per size class first")
@@ -112,6 +119,9 @@ def main():
     from surfmol_amd import workloads as W
 
     alpha, rdamp = 1.5, 1.0
+    if args.workload == "serve":
+        serve(args, S, W, alpha, rdamp)
+        return
     t0 = time.time()
     mols, sys_topo, Xs, L, desc = (workload_uniform if args.workload == "uniform" else workload_uneven)(args, S, W)
     t_gen = time.time() - t0
@@ -240,6 +250,68 @@ def main():
                                          f"reference CPU path (oracle/oracle.c), single thread, {t_cpu:.1f} s"}
         out["speedup_vs_cpu"] = round(value / cpu_rate, 1) if cpu_rate > 0 else None
 
+    print(json.dumps(out))
+
+
+def serve(args, S, W, alpha, rdamp):
+    """Serving loop under Poisson arrivals; one JSON line with p50/p99 per rate."""
+    rates = [float(r) for r in args.rates.split(",") if r]
+    ctx = S.Context(0)
+    grid, spec, t_grid = grid_for(ctx, S, W, 48.0, alpha=alpha, rdamp=rdamp)
+    fire = S.FireParams(dt0=W.fs_to_internal(0.1), dt_max=W.fs_to_internal(1.0))
+    per_rate = []
+    for k, rate in enumerate(rates):
+        seed = W.SEED_BASE + 4 + 16 * k
+        sizes = W.lognormal_sizes(args.requests, median=60.0, sigma=0.5, lo=20, hi=600, seed=seed)
+        rng = np.random.Generator(np.random.PCG64(seed + 1))
+        mols = [W.random_polyhex(rng, int(n)) for n in sizes]
+        ang, xy, h = W.uniform_batch_poses(args.requests, L=48.0, hmin=3.0, hmax=5.0, seed=seed + 2)
+        X = [W.place_on_surface(m, ang[s], xy[s], h[s]) for s, m in enumerate(mols)]
+        hm = [S.Molecule.build(m.elems, X[s], m.charges, bonds=m.bonds) for s, m in enumerate(mols)]
+        arrivals = W.poisson_arrivals(rate, args.requests, seed + 3)
+        srv = S.Server(ctx, grid, fire, max_resident=4096, max_atoms=600, hbm_fraction=0.25,
+                       chunk_steps=args.chunk_steps, max_steps=args.max_steps, fconv=1e-3, rdamp=rdamp)
+        ids = []
+        t0 = time.perf_counter()
+        for s in range(args.requests):
+            wait = arrivals[s] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            ids.append(srv.submit(hm[s], X[s]))
+        srv.drain()
+        res = [srv.poll(i, positions=False) for i in ids]
+        st = srv.stats()
+        srv.close()
+        lat = np.array([r["latency"] for r in res])
+        conv = sum(1 for r in res if r["status"] == S.SMR_SYS_CONVERGED)
+        span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+        steps = np.array([r["steps"] for r in res])
+        per_rate.append({
+            "rate_req_s": rate, "requests": args.requests, "converged": conv,
+            "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(1e3 * float(lat.mean()), 3),
+            "served_per_s": round(conv / span, 1),
+            "fire_steps_mean": round(float(steps.mean()), 1),
+            "atoms_mean": round(float(np.mean(sizes)), 1),
+            "chunks": int(st["chunks"]), "peak_resident": int(st["peak_resident"]),
+            "device_busy_frac": round(st["device_s"] / span, 3),
+        })
+        print(json.dumps({"progress": per_rate[-1]}), file=sys.stderr, flush=True)
+    out = {
+        "metric": "relax request latency p50/p99 (served, continuous batching)",
+        "unit": "ms",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "serving: Poisson arrivals, request sizes lognormal (median 60 atoms, ln-sigma 0.5, "
+                               "clipped [20,600]) random fused-ring molecules on a 48x48x20 A rock-salt GridFF at "
+                               "0.1 A; FIRE dt0 0.1 fs, dt_max 1 fs, Fmax < 1e-3 eV/A, <= %d steps; %d FIRE steps "
+                               "per chunk" % (args.max_steps, args.chunk_steps),
+                   "grid_build_fit_s": round(t_grid, 3)},
+        "per_rate": per_rate,
+    }
     print(json.dumps(out))
 
 

@@ -59,6 +59,10 @@ const char *smr_version(void);
  * (crates/libs/oclff/src/lib.rs:15-23, crates/libs/oclff/src/uff.rs:18-20). */
 int smr_ctx_create(int device, smr_ctx **out);
 int smr_ctx_destroy(smr_ctx *ctx);
+/* Device facts: out[5] = (free HBM bytes, total HBM bytes, compute units,
+ * LDS bytes per workgroup, wavefront size).  No counterpart in the
+ * reference. */
+int smr_ctx_info(smr_ctx *ctx, double *out);
 
 /* ----------------------------------------------------------------- GridFF */
 typedef struct {
@@ -240,30 +244,48 @@ int smr_run_md(smr_batch *b, int niter, double dt, double fconv, double flim, do
 
 /* ----------------------------------------------------------- serving */
 typedef struct {
-    int max_resident;         /* slot count (systems on the device at once)    */
-    int max_atoms;            /* largest request accepted                      */
-    double hbm_fraction;      /* admit while free HBM > (1-hbm_fraction)*total */
-    int chunk_steps;          /* FIRE steps per persistent launch              */
-    int max_steps;            /* step cap per request                          */
-    double fconv;             /* convergence threshold [eV/A]                  */
+    int max_resident;         /* slots: systems on the device at once           */
+    int max_atoms;            /* largest request accepted (<= 1024)             */
+    double hbm_fraction;      /* device memory for resident requests, as a
+                                 fraction of the HBM free at creation (0,1]     */
+    int chunk_steps;          /* FIRE steps per persistent launch               */
+    int max_steps;            /* step cap per request                           */
+    double fconv;             /* convergence threshold [eV/A]                   */
     smr_fire_params fire;
     smr_ff_options ff;
 } smr_server_config;
 
-/* Continuous-batching relax server: requests are admitted into free slots
- * (and only while free HBM allows), relaxed by chunked persistent launches
- * that overlap with later arrivals, and retired at convergence.  No
- * counterpart in the reference (it has no serving loop). */
+/* Continuous-batching relax server.  Requests of any size (up to
+ * max_atoms) are queued by smr_server_submit; a worker thread admits them
+ * into free slots while the device arenas (sized from hbm_fraction of free
+ * HBM) hold their topology and state, relaxes every resident request by
+ * chunked persistent launches of chunk_steps FIRE steps (one launch per
+ * size class), and retires each request as soon as it converges (or hits
+ * max_steps), with FP64 energies of its final positions.  Requests that
+ * arrive while others run are admitted at the next chunk.  A request's
+ * result does not depend on what else is resident (its size class fixes
+ * its launch shape).  No counterpart in the reference (it has no serving
+ * loop); the relaxation itself is smr_relax_batched's. */
 int smr_server_create(smr_ctx *ctx, const smr_gridff *grid, const smr_server_config *cfg, smr_server **out);
-/* Submit one request (a topology + initial positions [natoms,3]); *id
- * receives a request id.  Thread-safe. */
+/* Submit one request: a topology (copied; the caller's arrays may be freed
+ * after the call) and initial positions [natoms,3].  *id receives the
+ * request id.  Thread-safe.  SMR_ERR_NOMEM if the request can never fit the
+ * arenas. */
 int smr_server_submit(smr_server *s, const smr_topology *topo, const double *apos, long long *id);
-/* Poll a request: returns SMR_OK with *done = 1 and fills result / final
- * positions ([natoms,3], may be NULL) when retired, *done = 0 otherwise. */
+/* Poll a request.  *done = 1 when it has retired: result, final positions
+ * ([natoms,3], may be NULL) and the submit / retire times (seconds on the
+ * server's steady clock) are filled and the request is forgotten (a later
+ * poll of the id fails).  *done = 0 while it is queued or resident. */
 int smr_server_poll(smr_server *s, long long id, int *done, smr_relax_result *result, double *apos_out,
                     double *t_submit, double *t_done);
 /* Block until every submitted request has retired. */
 int smr_server_drain(smr_server *s);
+/* Counters: out[10] = (submitted, retired, resident, queued, chunks,
+ * peak resident, arena bytes, arena bytes in use, atoms capacity, seconds
+ * of device time in relax launches). */
+int smr_server_stats(smr_server *s, double *out);
+/* Stop the worker after its current chunk and free everything; requests
+ * still queued or resident are dropped. */
 int smr_server_destroy(smr_server *s);
 
 #ifdef __cplusplus

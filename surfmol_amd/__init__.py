@@ -28,7 +28,8 @@ from . import _lib
 from ._lib import (DATA_DIR, SMR_SYS_CONVERGED, SMR_SYS_MAXSTEPS, SMR_SYS_NONFINITE, SMR_SYS_RUNNING, SmrError,
                    check, lib)
 
-__all__ = ["Molecule", "Context", "GridFF", "GridSpec", "Batch", "FireParams", "substrate_reqs", "SmrError", "DATA_DIR", "SMR_SYS_CONVERGED",
+__all__ = ["Molecule", "Context", "GridFF", "GridSpec", "Batch", "Server", "FireParams", "substrate_reqs", "SmrError",
+           "DATA_DIR", "SMR_SYS_CONVERGED",
            "SMR_SYS_MAXSTEPS", "SMR_SYS_NONFINITE", "SMR_SYS_RUNNING", "lib"]
 
 _P = ctypes.POINTER
@@ -141,6 +142,12 @@ class Context:
         h = ctypes.c_void_p()
         check(lib().smr_ctx_create(int(device), ctypes.byref(h)))
         self._h = h
+
+    def info(self):
+        o = (ctypes.c_double * 5)()
+        check(lib().smr_ctx_info(self._h, o))
+        return {"free_bytes": o[0], "total_bytes": o[1], "compute_units": int(o[2]), "lds_per_workgroup": int(o[3]),
+                "wavefront": int(o[4])}
 
     def close(self):
         if self._h:
@@ -347,6 +354,80 @@ class Batch:
     def close(self):
         if self._h:
             lib().smr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching relax server (smr_server_*): submit requests of any
+    size while others relax; each retires at convergence."""
+
+    def __init__(self, ctx: Context, grid: GridFF | None, fire: FireParams, max_resident=1024, max_atoms=600,
+                 hbm_fraction=0.25, chunk_steps=64, max_steps=2000, fconv=1e-3, rdamp=1.0, nonbonded=True,
+                 surface=True):
+        self.ctx = ctx
+        self.grid = grid
+        cfg = _lib.SmrServerConfig()
+        cfg.max_resident = int(max_resident)
+        cfg.max_atoms = int(max_atoms)
+        cfg.hbm_fraction = float(hbm_fraction)
+        cfg.chunk_steps = int(chunk_steps)
+        cfg.max_steps = int(max_steps)
+        cfg.fconv = float(fconv)
+        cfg.fire = fire.cstruct()
+        cfg.ff = _lib.SmrFFOptions(float(rdamp), 1 if nonbonded else 0, 1 if (surface and grid is not None) else 0)
+        h = ctypes.c_void_p()
+        check(lib().smr_server_create(ctx._h, grid._h if grid is not None else None, ctypes.byref(cfg),
+                                      ctypes.byref(h)))
+        self._h = h
+        self._natoms = {}
+
+    def submit(self, molecule: Molecule, apos) -> int:
+        x = np.ascontiguousarray(apos, dtype=np.float64).reshape(-1, 3)
+        if x.shape[0] != molecule.natoms:
+            raise ValueError(f"{x.shape[0]} positions for {molecule.natoms} atoms")
+        rid = ctypes.c_longlong()
+        check(lib().smr_server_submit(self._h, ctypes.byref(molecule.topo), _dptr(x), ctypes.byref(rid)))
+        self._natoms[rid.value] = molecule.natoms
+        return rid.value
+
+    def poll(self, rid: int, positions=True):
+        """None while the request runs; else a result dict (forgets the request)."""
+        done = ctypes.c_int()
+        res = _lib.SmrRelaxResult()
+        n = self._natoms.get(rid)
+        positions = positions and n is not None
+        x = np.zeros((n, 3)) if positions else None
+        ts, td = ctypes.c_double(), ctypes.c_double()
+        check(lib().smr_server_poll(self._h, ctypes.c_longlong(rid), ctypes.byref(done), ctypes.byref(res),
+                                    _dptr(x) if positions else None, ctypes.byref(ts), ctypes.byref(td)))
+        if not done.value:
+            return None
+        self._natoms.pop(rid, None)
+        out = _results([res])[0]
+        out.update(t_submit=ts.value, t_done=td.value, latency=td.value - ts.value)
+        if positions:
+            out["positions"] = x
+        return out
+
+    def drain(self):
+        check(lib().smr_server_drain(self._h))
+
+    def stats(self):
+        o = (ctypes.c_double * 10)()
+        check(lib().smr_server_stats(self._h, o))
+        keys = ["submitted", "retired", "resident", "queued", "chunks", "peak_resident", "arena_bytes",
+                "arena_bytes_used", "atom_capacity", "device_s"]
+        return dict(zip(keys, list(o)))
+
+    def close(self):
+        if self._h:
+            lib().smr_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -69,6 +69,7 @@ _SIGNATURES = [
     ("smr_version", c_char_p, []),
     ("smr_ctx_create", c_int, [c_int, P(c_void_p)]),
     ("smr_ctx_destroy", c_int, [c_void_p]),
+    ("smr_ctx_info", c_int, [c_void_p, P(c_double)]),
     ("smr_gridff_upload", c_int, [c_void_p, P(SmrGridDesc), P(c_float), P(c_void_p)]),
     ("smr_gridff_build", c_int, [c_void_p, P(SmrGridDesc), c_int, P(c_double), P(c_double), c_int, P(c_void_p)]),
     ("smr_gridff_download", c_int, [c_void_p, P(c_float)]),
@@ -96,6 +97,7 @@ _SIGNATURES = [
     ("smr_server_poll", c_int, [c_void_p, ctypes.c_longlong, P(c_int), P(SmrRelaxResult), P(c_double),
                                 P(c_double), P(c_double)]),
     ("smr_server_drain", c_int, [c_void_p]),
+    ("smr_server_stats", c_int, [c_void_p, P(c_double)]),
     ("smr_server_destroy", c_int, [c_void_p]),
     ("smh_last_error", c_char_p, []),
     ("smh_molecule_build", c_int, [c_char_p, c_int, P(c_char_p), P(c_double), P(c_double), c_int, P(c_int),

@@ -111,6 +111,21 @@ extern "C" int smr_ctx_destroy(smr_ctx *c) {
     return SMR_OK;
 }
 
+extern "C" int smr_ctx_info(smr_ctx *c, double *out) {
+    if (!c || !out) return fail(SMR_ERR_INVALID, "smr_ctx_info: NULL argument");
+    HIPCHK(hipSetDevice(c->device));
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    int ws = 0;
+    HIPCHK(hipDeviceGetAttribute(&ws, hipDeviceAttributeWarpSize, c->device));
+    out[0] = (double)fr;
+    out[1] = (double)tot;
+    out[2] = (double)c->num_cu;
+    out[3] = (double)c->max_lds;
+    out[4] = (double)ws;
+    return SMR_OK;
+}
+
 // ============================================================ GridFF
 static int check_desc(const smr_grid_desc *d, const char *who) {
     if (!d) return fail(SMR_ERR_INVALID, std::string(who) + ": desc == NULL");

@@ -68,6 +68,11 @@ struct PlainArgs {
 hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t stream);
 const void *relax_kernel_ptr(int block, int apt, bool fint_lds, bool stencil_cache);
 hipError_t launch_zero_i32(int *p, int n, hipStream_t st);
+// device-to-device copies described by a list (serving loop)
+struct CopyDesc {
+    unsigned long long src, dst, bytes;   // device addresses; bytes even, 2-byte aligned
+};
+hipError_t launch_copy_desc(const CopyDesc *d, int n, hipStream_t st);
 // FP64 energies (state[s].E) of the listed systems at their current positions
 hipError_t launch_energy64(const DevTables &T, const float4 *pos, SysState *state, const int *list, int n,
                            hipStream_t st);
