@@ -514,6 +514,7 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
     if (e == hipSuccess) e = b->force.alloc(sizeof(float4) * b->total_atoms);
     if (e == hipSuccess) e = b->state.alloc(sizeof(SysState) * nsys);
     if (e == hipSuccess) e = b->work.alloc(sizeof(int) * nsys);
+    if (e == hipSuccess && b->grid) e = b->stencil.alloc(sizeof(float4) * 16 * (size_t)b->total_atoms);
     if (e == hipSuccess) e = hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, c->stream);
     if (e != hipSuccess) { delete b; ctx_release(c); return hip_fail(e, "smr_batch_create: device allocation"); }
     fill_tables(b->dp, b->grid, b->opts, b->tab);
@@ -621,6 +622,7 @@ int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch,
     A.force = (float4 *)b->force.ptr;
     A.state = (SysState *)b->state.ptr;
     A.work_list = (const int *)b->work.ptr;
+    A.stencil = (float4 *)b->stencil.ptr;
     A.steps_per_launch = steps_per_launch;
     A.mode = mode;
     return launch_plans(c->ss, b->ls, b->plans, A, kernel_ms);
@@ -896,6 +898,7 @@ extern "C" int smr_batch_autotune(smr_batch *b, const smr_fire_params *fire, int
     A.force = (float4 *)b->force.ptr;
     A.state = (SysState *)b->state.ptr;
     A.work_list = (const int *)b->work.ptr;
+    A.stencil = (float4 *)b->stencil.ptr;
     A.steps_per_launch = trial_steps;
     A.mode = KMODE_FIRE;
     static const int shapes[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {128, 4}, {256, 1}, {256, 2}, {256, 4}};
@@ -917,7 +920,7 @@ extern "C" int smr_batch_autotune(smr_batch *b, const smr_fire_params *fire, int
             if (cap < p.max_atoms || cap > 2 * cap0) continue;
             for (int pm = 1; pm >= 0; pm--)
                 for (int fl = 1; fl >= 0; fl--) {
-                    LaunchShape L = make_shape(c, p, sh[0], sh[1], fl, pm, false, b->grid != nullptr);
+                    LaunchShape L = make_shape(c, p, sh[0], sh[1], fl, pm, b->tune_cache != 0, b->grid != nullptr);
                     if (L.block) cand.push_back(L);
                 }
         }

@@ -29,8 +29,7 @@ LaunchShape make_shape(const smr_ctx *c, const ClassPlan &cp, int block, int apt
     LaunchShape L;
     L.block = 0;
     if (block * apt < cp.max_atoms) return L;
-    if (stencil_cache && (apt != 1 || !have_grid)) return L;
-    const void *k = relax_kernel_ptr(block, apt, fint_lds, stencil_cache);
+    const void *k = relax_kernel_ptr(block, apt, fint_lds);
     if (!k) return L;
     const size_t lds = relax_lds_bytes(block, apt, cp.max_atoms, cp.nf_max, cp.na_max, cp.nd_max, cp.ni_max, fint_lds, pair_mode);
     if (lds > c->max_lds) return L;
@@ -41,7 +40,7 @@ LaunchShape make_shape(const smr_ctx *c, const ClassPlan &cp, int block, int apt
     L.apt = apt;
     L.fint_lds = fint_lds;
     L.pair_mode = pair_mode;
-    L.stencil_cache = stencil_cache;
+    L.stencil_cache = stencil_cache && have_grid;
     L.lds_bytes = lds;
     L.grid = (int)std::max<long long>(1, std::min<long long>(cap, cp.count));
     return L;
@@ -74,21 +73,21 @@ LaunchShape class_shape(const smr_ctx *c, const ClassPlan &cp, bool have_grid, i
                         int tune_pairs) {
     static const int def_block[NCLASS] = {64, 128, 256, 256, 256};
     static const int def_apt[NCLASS] = {1, 1, 1, 2, 4};
-    // the register stencil cache halves occupancy (256 VGPRs): off unless asked for
-    const bool cache = (tune_cache == 1) && have_grid;
+    // stored surface stencils: on unless switched off
+    const bool cache = (tune_cache != 0) && have_grid;
     if (tune_block > 0) {
         int apt = (cp.max_atoms + tune_block - 1) / tune_block;
         apt = apt <= 1 ? 1 : apt <= 2 ? 2 : 4;
-        LaunchShape L = best_variant(c, cp, tune_block, apt, cache && apt == 1, have_grid, tune_pairs);
+        LaunchShape L = best_variant(c, cp, tune_block, apt, cache, have_grid, tune_pairs);
         if (L.block) return L;
     }
     const TunedShape &t = c->tuned[cp.cls];
     if (t.valid && tune_block <= 0 && (tune_pairs < 0 || tune_pairs == (int)t.pair_mode)) {
-        LaunchShape L = make_shape(c, cp, t.block, t.apt, t.fint_lds, t.pair_mode, cache && t.apt == 1, have_grid);
+        LaunchShape L = make_shape(c, cp, t.block, t.apt, t.fint_lds, t.pair_mode, cache, have_grid);
         if (L.block) return L;
     }
     const int b = def_block[cp.cls], a = def_apt[cp.cls];
-    return best_variant(c, cp, b, a, cache && a == 1, have_grid, tune_pairs);
+    return best_variant(c, cp, b, a, cache, have_grid, tune_pairs);
 }
 
 void build_plans(std::vector<SysSize> sys, std::vector<ClassPlan> &plans, std::vector<int> &work) {
@@ -204,6 +203,7 @@ int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan
         A.lds_nd = p.nd_max;
         A.lds_ni = p.ni_max;
         A.pair_mode = p.shape.pair_mode ? 1 : 0;
+        A.stencil = p.shape.stencil_cache ? base.stencil : nullptr;
     }
     const int naux = std::min<int>(3, (int)plans.size() - 1);
     auto issue = [&]() -> hipError_t {

@@ -23,7 +23,7 @@ namespace smr {
 // ===========================================================================
 // fused persistent kernel
 // ===========================================================================
-template <int BLOCK, int APT, bool FINT_LDS, bool SCACHE>
+template <int BLOCK, int APT, bool FINT_LDS>
 __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
     constexpr int NW = BLOCK / 64;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -115,11 +115,12 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
         }
         __syncthreads();
 
-        // per-lane cache of the atom's PLQH-contracted 4x4x4 stencil; re-read
-        // from the grid only when the atom enters another cell
-        constexpr bool CACHE = SCACHE && (APT == 1);
-        float sv[CACHE ? 64 : 1];
-        int skey = -1;
+        // cell of each own atom's stored surface stencil (A.stencil): the
+        // PLQH-contracted 4x4x4 stencil is re-read from the grid only when
+        // the atom enters another cell
+        int skey[APT];
+#pragma unroll
+        for (int k = 0; k < APT; k++) skey[k] = -1;
         const int nsteps = A.steps_per_launch;
         double fmax_cur = st.fmax;
         int step_in_launch = 0;
@@ -262,14 +263,16 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     if (T.grid) {
                         float3 ul = make_float3(pi.x * T.ginv_x, pi.y * T.ginv_y, pi.z * T.ginv_z);
                         float4 fe;
-                        if constexpr (CACHE) {
+                        if (A.stencil) {
                             GridCell gc = grid_cell(T.gnx, T.gny, T.gnz, sd.iux, sd.iuy, ul);
                             const int key = gc.iz < 0 ? -2 : (gc.ix * T.gny + gc.iy) * T.gnz + gc.iz;
-                            if (key != skey) {
-                                if (gc.iz >= 0) grid_load_stencil(T.grid, T.gnx, T.gny, T.gnz, gc, my_plq[k], sv);
-                                skey = key;
+                            // the system's block of 16*N float4, element-major
+                            float4 *sc = A.stencil + (size_t)abase * 16 + i;
+                            if (key != skey[k]) {
+                                if (gc.iz >= 0) grid_store_stencil(T.grid, T.gnx, T.gny, T.gnz, gc, my_plq[k], sc, N);
+                                skey[k] = key;
                             }
-                            fe = grid_eval_stencil(sv, gc);
+                            fe = grid_eval_cached(sc, N, gc);
                         } else {
                             fe = grid_sample(T.grid, T.gnx, T.gny, T.gnz, sd.iux, sd.iuy, ul, my_plq[k]);
                         }
@@ -624,44 +627,33 @@ __global__ void k_plain_fire_apply(PlainArgs P, float a, float b, int reset, flo
 // ===========================================================================
 // host-side launch helpers
 // ===========================================================================
-template <int BLOCK, int APT, bool FL, bool SC>
+template <int BLOCK, int APT, bool FL>
 static hipError_t launch_one(const KernelArgs &A, int grid, size_t lds, hipStream_t stream) {
-    hipLaunchKernelGGL((k_relax<BLOCK, APT, FL, SC>), dim3(grid), dim3(BLOCK), lds, stream, A);
+    hipLaunchKernelGGL((k_relax<BLOCK, APT, FL>), dim3(grid), dim3(BLOCK), lds, stream, A);
     return hipGetLastError();
 }
 
-template <int BLOCK, int APT, bool FL, bool SC>
+template <int BLOCK, int APT, bool FL>
 static const void *kernel_ptr() {
-    return reinterpret_cast<const void *>(&k_relax<BLOCK, APT, FL, SC>);
+    return reinterpret_cast<const void *>(&k_relax<BLOCK, APT, FL>);
 }
 
-// instantiated shapes: (threads per workgroup, atoms per thread); the
-// stencil-cache variant exists for one atom per thread only
+// instantiated shapes: (threads per workgroup, atoms per thread)
 #define SMR_SHAPES(X) X(64, 1) X(64, 2) X(64, 4) X(128, 1) X(128, 2) X(128, 4) X(256, 1) X(256, 2) X(256, 4)
 
 hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t stream) {
 #define SMR_CASE(B, P)                                                                                   \
-    if (L.block == B && L.apt == P) {                                                                    \
-        constexpr bool SCOK = (P == 1);                                                                  \
-        if (SCOK && L.stencil_cache)                                                                     \
-            return L.fint_lds ? launch_one<B, P, true, SCOK>(A, L.grid, L.lds_bytes, stream)             \
-                              : launch_one<B, P, false, SCOK>(A, L.grid, L.lds_bytes, stream);           \
-        return L.fint_lds ? launch_one<B, P, true, false>(A, L.grid, L.lds_bytes, stream)                \
-                          : launch_one<B, P, false, false>(A, L.grid, L.lds_bytes, stream);              \
-    }
+    if (L.block == B && L.apt == P)                                                                      \
+        return L.fint_lds ? launch_one<B, P, true>(A, L.grid, L.lds_bytes, stream)                       \
+                          : launch_one<B, P, false>(A, L.grid, L.lds_bytes, stream);
     SMR_SHAPES(SMR_CASE)
 #undef SMR_CASE
     return hipErrorInvalidValue;
 }
 
-const void *relax_kernel_ptr(int block, int apt, bool fint_lds, bool stencil_cache) {
-#define SMR_CASE(B, P)                                                                                   \
-    if (block == B && apt == P) {                                                                        \
-        constexpr bool SCOK = (P == 1);                                                                  \
-        if (SCOK && stencil_cache) return fint_lds ? kernel_ptr<B, P, true, SCOK>() : kernel_ptr<B, P, false, SCOK>(); \
-        if (!SCOK && stencil_cache) return nullptr;                                                      \
-        return fint_lds ? kernel_ptr<B, P, true, false>() : kernel_ptr<B, P, false, false>();            \
-    }
+const void *relax_kernel_ptr(int block, int apt, bool fint_lds) {
+#define SMR_CASE(B, P) \
+    if (block == B && apt == P) return fint_lds ? kernel_ptr<B, P, true>() : kernel_ptr<B, P, false>();
     SMR_SHAPES(SMR_CASE)
 #undef SMR_CASE
     return nullptr;

@@ -108,7 +108,7 @@ struct Request {
 // 3 inversions per atom)
 constexpr size_t kAngPerAtom = 3, kDihPerAtom = 6, kInvPerAtom = 3;
 constexpr size_t kPiecesPerAtom = 4 * kDihPerAtom + 4 * kInvPerAtom + 3 * kAngPerAtom;
-constexpr size_t kBytesPerAtom = 3 * 16 + 5 * 16 + 8 + 4 * 32;                  // state + tables + FP64 tables
+constexpr size_t kBytesPerAtom = 3 * 16 + 5 * 16 + 8 + 4 * 32 + 256;   // state + tables + FP64 tables + stencil
 constexpr size_t kBytesPerAng = 4 + 16 + 32, kBytesPerDih = 8 + 16 + 32, kBytesPerInv = 8 + 16 + 32;
 constexpr size_t kUnitBytes = kBytesPerAtom + kAngPerAtom * kBytesPerAng + kDihPerAtom * kBytesPerDih +
                               kInvPerAtom * kBytesPerInv + kPiecesPerAtom * 2;
@@ -123,7 +123,7 @@ struct smr_server {
     LaunchSet ls;
     DevTables tab;
     FireConst fc;
-    DevBuf pos, vel, force, ng, bk, bl, req, plq, a2f, bk64, bl64, req64, plq64;
+    DevBuf pos, vel, force, stencil, ng, bk, bl, req, plq, a2f, bk64, bl64, req64, plq64;
     DevBuf ang_ng, ang_par, ang64, dih_ng, dih_par, dih64, inv_ng, inv_par, inv64, a2f_idx;
     DevBuf topo, sys, state, work, list, dstage, dout;
     RangeAlloc ra_atoms, ra_ang, ra_dih, ra_inv, ra_pc;
@@ -327,6 +327,7 @@ int run_chunk(smr_server *S) {
     A.force = (float4 *)S->force.ptr;
     A.state = (SysState *)S->state.ptr;
     A.work_list = (const int *)S->work.ptr;
+    A.stencil = (float4 *)S->stencil.ptr;
     A.steps_per_launch = S->cfg.chunk_steps;
     A.mode = KMODE_FIRE;
     float ms = 0.f;
@@ -520,6 +521,7 @@ extern "C" int smr_server_create(smr_ctx *c, const smr_gridff *grid, const smr_s
 #define ALLOC(buf, bytes) \
     if ((e = S->buf.alloc(bytes)) != hipSuccess) return fail_create(e, "smr_server_create: arena " #buf);
     ALLOC(pos, ca * sizeof(float4)) ALLOC(vel, ca * sizeof(float4)) ALLOC(force, ca * sizeof(float4))
+    ALLOC(stencil, ca * 16 * sizeof(float4))
     ALLOC(ng, ca * sizeof(int4)) ALLOC(bk, ca * sizeof(float4)) ALLOC(bl, ca * sizeof(float4))
     ALLOC(req, ca * sizeof(float4)) ALLOC(plq, ca * sizeof(float4)) ALLOC(a2f, ca * sizeof(int2))
     ALLOC(bk64, ca * sizeof(double4)) ALLOC(bl64, ca * sizeof(double4)) ALLOC(req64, ca * sizeof(double4))

@@ -305,10 +305,9 @@ __device__ __forceinline__ float4 grid_sample(const float4 *__restrict__ G, int 
     return make_float4(ox, oy, oz, oe);
 }
 
-// The same sample split in two: load the 4x4x4 stencil of one cell already
-// contracted with the atom's PLQH (v[16a+4b+c] = PLQH . C[x+a][y+b][z+c]),
-// then interpolate inside the cell.  grid_eval_stencil(grid_load_stencil())
-// performs exactly the arithmetic of grid_sample, in the same order.
+// The same sample split in two, for a per-atom stencil cache: locate the
+// cell, contract its 4x4x4 stencil with the atom's PLQH once (stored while
+// the atom stays in the cell), then interpolate inside the cell.
 struct GridCell {
     int ix, iy, iz;     // first stencil node (wrapped x,y); iz < 0: outside
     float tx, ty, tz;
@@ -336,8 +335,12 @@ __device__ __forceinline__ GridCell grid_cell(int nx, int ny, int nz, int ix0, i
     return g;
 }
 
-__device__ __forceinline__ void grid_load_stencil(const float4 *__restrict__ G, int nx, int ny, int nz, const GridCell &g,
-                                                  float4 plq, float *v) {
+// Contract the 4x4x4 stencil of cell g with the atom's PLQH and store it:
+// sc[(4a+b)*ld] = (PLQH.C[x+a][y+b][z+c], c = 0..3), 256 B per atom.  The
+// stride ld (the system's atom count) makes element k of all atoms of a
+// system contiguous, so one load instruction of a wave reads whole lines.
+__device__ __forceinline__ void grid_store_stencil(const float4 *__restrict__ G, int nx, int ny, int nz,
+                                                   const GridCell &g, float4 plq, float4 *sc, int ld) {
 #pragma unroll
     for (int a = 0; a < 4; a++) {
         int xi = g.ix + a; if (xi >= nx) xi -= nx;
@@ -345,13 +348,15 @@ __device__ __forceinline__ void grid_load_stencil(const float4 *__restrict__ G, 
         for (int b = 0; b < 4; b++) {
             int yi = g.iy + b; if (yi >= ny) yi -= ny;
             const float4 *col = G + ((size_t)(xi * ny + yi) * nz + g.iz);
-#pragma unroll
-            for (int c = 0; c < 4; c++) v[16 * a + 4 * b + c] = dotplq(plq, col[c]);
+            sc[(4 * a + b) * ld] = make_float4(dotplq(plq, col[0]), dotplq(plq, col[1]), dotplq(plq, col[2]),
+                                               dotplq(plq, col[3]));
         }
     }
 }
 
-__device__ __forceinline__ float4 grid_eval_stencil(const float *v, const GridCell &g) {
+// Interpolate inside cell g from a stored contracted stencil (the
+// contraction order of grid_sample: z, then y, then x).  Returns (dE/du, E).
+__device__ __forceinline__ float4 grid_eval_cached(const float4 *__restrict__ sc, int ld, const GridCell &g) {
     if (g.iz < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
     float4 bz = bsp_basis(g.tz), dz = bsp_dbasis(g.tz);
     float4 by = bsp_basis(g.ty), dy = bsp_dbasis(g.ty);
@@ -364,7 +369,7 @@ __device__ __forceinline__ float4 grid_eval_stencil(const float *v, const GridCe
         float e_y = 0.f, e_dy = 0.f, e_dz = 0.f;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
-            float4 cs = make_float4(v[16 * a + 4 * b], v[16 * a + 4 * b + 1], v[16 * a + 4 * b + 2], v[16 * a + 4 * b + 3]);
+            const float4 cs = sc[(4 * a + b) * ld];
             float fe = dot4(bz, cs);
             float fd = dot4(dz, cs);
             e_dy += fe * dy_[b];

@@ -155,6 +155,7 @@ struct smr_batch {
     std::vector<int> atom_off;       // [nsys+1]
     std::vector<double> origin;      // [3*nsys]
     smr::DevBuf pos, pos0, vel, force, state, work;
+    smr::DevBuf stencil;             // float4 [16 * total_atoms] stored surface stencils
     smr::DevBuf plain_hneigh, plain_fint, plain_e, plain_cvf;
     std::vector<smr::ClassPlan> plans;
     smr::LaunchSet ls;

@@ -20,6 +20,7 @@ struct KernelArgs {
     int nwork;
     int *work_counter;       // atomic fetch counter (zeroed before launch)
     float *fint_scratch;     // [grid * 3 * nf_max] when pieces do not fit LDS
+    float4 *stencil;         // [total atoms * 16] stored surface stencils (NULL: sample the grid directly)
     int nf_max;
     int lds_atoms;           // atoms reserved per LDS slot
     int lds_na, lds_nd, lds_ni;  // interaction slots reserved per LDS slot
@@ -31,7 +32,7 @@ struct KernelArgs {
 struct LaunchShape {
     int block = 0, apt = 0, grid = 0;
     bool fint_lds = false;
-    bool stencil_cache = false;   // per-lane register cache of the surface stencil
+    bool stencil_cache = true;    // per-atom stored surface stencil (re-read on a cell change)
     bool pair_mode = true;        // unordered-pair sweep (needs a reaction buffer per wave)
     size_t lds_bytes = 0;
 };
@@ -66,7 +67,7 @@ struct PlainArgs {
 };
 
 hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t stream);
-const void *relax_kernel_ptr(int block, int apt, bool fint_lds, bool stencil_cache);
+const void *relax_kernel_ptr(int block, int apt, bool fint_lds);
 hipError_t launch_zero_i32(int *p, int n, hipStream_t st);
 // device-to-device copies described by a list (serving loop)
 struct CopyDesc {

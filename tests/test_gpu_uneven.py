@@ -102,10 +102,18 @@ def test_uneven_relax_matches_fp64(ctx, surf, oracle, params):
         # converged energy (of the returned positions) against FP64 there
         assert abs(eo.sum() - r["energy"]) / m.natoms < E_ATOL_PER_ATOM, (s, eo.sum(), r["energy"])
         assert abs(eo.sum() - Ef[s].sum()) / m.natoms < E_ATOL_PER_ATOM, (s, eo.sum(), Ef[s].sum())
+        # the device's minimum is an FP64 minimum: FP64 FIRE started there
+        # converges without moving the energy by more than 1e-6 eV/atom
+        rf = oracle.relax_fire(osys, oracle.FireParams(**fp), 3000, 1e-3, apos=Xf[sl])
+        assert rf["converged"]
+        assert abs(rf["E"] - r["energy"]) / m.natoms < E_ATOL_PER_ATOM, (s, rf["E"], r["energy"])
+        # an FP64 relaxation from the same start ends in the same region of
+        # the landscape (flexible molecules have nearby minima: FP32 and FP64
+        # trajectories may pick different ones)
         ro = oracle.relax_fire(osys, oracle.FireParams(**fp), 3000, 1e-3)
         assert ro["converged"]
         print(s, m.natoms, r["steps"], ro["steps"], r["energy"], ro["E"])
-        assert abs(r["energy"] - ro["E"]) / m.natoms < 2e-5, (s, m.natoms, r["energy"], ro["E"])
+        assert abs(r["energy"] - ro["E"]) / m.natoms < 1e-3, (s, m.natoms, r["energy"], ro["E"])
 
 
 def test_uneven_same_alone_and_in_mix(ctx, surf):
