@@ -307,6 +307,9 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
         }
     }
     auto slot_ok = [&](int g) { return g >= 0 && g < 4 * N && t.neighs[g] >= 0; };
+    // hneigh slot 4*atom + k of the reference -> k*N + atom on the device
+    // (slot-major, so that the per-atom bond phase writes consecutive rows)
+    auto dslot = [&](int g) { return (unsigned)((g & 3) * N + (g >> 2)); };
     // ---- angles ----
     for (int a = 0; a < t.nangles; a++) {
         int g0 = t.ang_ngs[2 * a], g1 = t.ang_ngs[2 * a + 1];
@@ -314,7 +317,7 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
             return fail(SMR_ERR_INVALID, who + "angle " + std::to_string(a) + " has hneigh slots (" + std::to_string(g0) +
                                              "," + std::to_string(g1) + ") that are not bonds");
         const double *p = t.ang_params + 5 * a;
-        P.ang_ng.push_back((unsigned)g0 | ((unsigned)g1 << 16));
+        P.ang_ng.push_back(dslot(g0) | (dslot(g1) << 16));
         P.ang_par.push_back(make_float4((float)(p[0] * p[1]), (float)(p[0] * p[2]), (float)(p[0] * p[3]), (float)(p[0] * p[4])));
         P.ang64.push_back(make_double4(p[0], p[1], p[3], p[4]));
     }
@@ -326,7 +329,7 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
         const double *p = t.dih_params + 3 * d;
         int n = (int)p[2];
         if (n < 1 || n > 12) return fail(SMR_ERR_INVALID, who + "dihedral " + std::to_string(d) + " multiplicity " + std::to_string(n));
-        P.dih_ng.push_back(make_uint2((unsigned)g0 | ((unsigned)g1 << 16), (unsigned)g2 | ((unsigned)n << 16)));
+        P.dih_ng.push_back(make_uint2(dslot(g0) | (dslot(g1) << 16), dslot(g2) | ((unsigned)n << 16)));
         P.dih_par.push_back(make_float4((float)p[0], (float)(p[0] * p[1]), (float)(p[0] * p[1] * p[2]), 0.f));
         P.dih64.push_back(make_double4(p[0], p[1], (double)n, 0.0));
     }
@@ -336,12 +339,15 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
         if (!slot_ok(g0) || !slot_ok(g1) || !slot_ok(g2))
             return fail(SMR_ERR_INVALID, who + "inversion " + std::to_string(v) + " has an invalid hneigh slot");
         const double *p = t.inv_params + 4 * v;
-        P.inv_ng.push_back(make_uint2((unsigned)g0 | ((unsigned)g1 << 16), (unsigned)g2));
+        P.inv_ng.push_back(make_uint2(dslot(g0) | (dslot(g1) << 16), dslot(g2)));
         P.inv_par.push_back(make_float4((float)(p[0] * p[1]), (float)(p[0] * p[2]), (float)(p[0] * p[3]), 0.f));
         P.inv64.push_back(make_double4(p[0], p[1], p[2], p[3]));
     }
     // ---- a2f gather lists (Uff::map_atom_interactions, uff.rs:317-370):
-    //      dihedral pieces, then inversion pieces, then angle pieces ----
+    //      dihedral pieces, then inversion pieces, then angle pieces.  The
+    //      device stores piece k of term t at k*nterms + t within each term
+    //      block (consecutive terms write consecutive slots); the gather
+    //      order per atom is the reference's. ----
     std::vector<int> cnt(N, 0), off(N + 1, 0);
     auto chk_atom = [&](int a) { return a >= 0 && a < N; };
     for (int d = 0; d < t.ndih; d++)
@@ -367,11 +373,11 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
     std::fill(cnt.begin(), cnt.end(), 0);
     const int i0inv = 4 * t.ndih, i0ang = i0inv + 4 * t.ninv;
     for (int d = 0; d < t.ndih; d++)
-        for (int k = 0; k < 4; k++) { int a = t.dih_atoms[4 * d + k]; idx[off[a] + cnt[a]++] = (unsigned short)(4 * d + k); }
+        for (int k = 0; k < 4; k++) { int a = t.dih_atoms[4 * d + k]; idx[off[a] + cnt[a]++] = (unsigned short)(k * t.ndih + d); }
     for (int v = 0; v < t.ninv; v++)
-        for (int k = 0; k < 4; k++) { int a = t.inv_atoms[4 * v + k]; idx[off[a] + cnt[a]++] = (unsigned short)(i0inv + 4 * v + k); }
+        for (int k = 0; k < 4; k++) { int a = t.inv_atoms[4 * v + k]; idx[off[a] + cnt[a]++] = (unsigned short)(i0inv + k * t.ninv + v); }
     for (int a3 = 0; a3 < t.nangles; a3++)
-        for (int k = 0; k < 3; k++) { int a = t.ang_atoms[3 * a3 + k]; idx[off[a] + cnt[a]++] = (unsigned short)(i0ang + 3 * a3 + k); }
+        for (int k = 0; k < 3; k++) { int a = t.ang_atoms[3 * a3 + k]; idx[off[a] + cnt[a]++] = (unsigned short)(i0ang + k * t.nangles + a3); }
     for (int i = 0; i < N; i++) P.a2f.push_back(make_int2(off[i], off[i + 1] - off[i]));
     P.a2f_idx.insert(P.a2f_idx.end(), idx.begin(), idx.end());
     return SMR_OK;

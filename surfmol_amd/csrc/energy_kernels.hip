@@ -38,11 +38,11 @@ __device__ __forceinline__ d3 ld(const float4 *p, int i) {
     return mk((double)v.x, (double)v.y, (double)v.z);
 }
 
-// unit vector of hneigh slot g = 4*atom + k (atom -> its k-th neighbour)
-__device__ __forceinline__ d3 slot_dir(const float4 *pos, const int4 *ng, int g) {
-    const int a = g >> 2;
+// unit vector of device hneigh slot g = k*N + atom (atom -> its k-th neighbour)
+__device__ __forceinline__ d3 slot_dir(const float4 *pos, const int4 *ng, int g, int N) {
+    const int k = g / N;
+    const int a = g - k * N;
     const int4 n = ng[a];
-    const int k = g & 3;
     const int b = k == 0 ? n.x : k == 1 ? n.y : k == 2 ? n.z : n.w;
     d3 d = sub(ld(pos, b), ld(pos, a));
     return mul(d, 1.0 / sqrt(dot(d, d)));
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_energy64(DevTables T, const float4 *__r
     // angles: E = K (c0 + c2 cos 2t + c3 cos 3t)
     for (int a = tid; a < td.nangles; a += blockDim.x) {
         const unsigned g = T.ang_ng[td.ang_off + a];
-        const d3 qij = slot_dir(P, NG, (int)(g & 0xffffu)), qkj = slot_dir(P, NG, (int)(g >> 16));
+        const d3 qij = slot_dir(P, NG, (int)(g & 0xffffu), N), qkj = slot_dir(P, NG, (int)(g >> 16), N);
         const double4 p = T.ang64[td.ang_off + a];
         const d3 h = add(qij, qkj);
         const double c = 0.5 * (dot(h, h) - 2.0);
@@ -98,8 +98,8 @@ __global__ __launch_bounds__(256) void k_energy64(DevTables T, const float4 *__r
     // dihedrals: E = V (1 + d cos n phi)
     for (int q = tid; q < td.ndih; q += blockDim.x) {
         const uint2 g = T.dih_ng[td.dih_off + q];
-        const d3 h12 = slot_dir(P, NG, (int)(g.x & 0xffffu)), h32 = slot_dir(P, NG, (int)(g.x >> 16));
-        const d3 h43 = slot_dir(P, NG, (int)(g.y & 0xffffu));
+        const d3 h12 = slot_dir(P, NG, (int)(g.x & 0xffffu), N), h32 = slot_dir(P, NG, (int)(g.x >> 16), N);
+        const d3 h43 = slot_dir(P, NG, (int)(g.y & 0xffffu), N);
         const double4 p = T.dih64[td.dih_off + q];
         const d3 n123 = cross(h12, h32), n234 = cross(h43, h32);
         const double n1 = dot(n123, n123), n2 = dot(n234, n234);
@@ -118,8 +118,8 @@ __global__ __launch_bounds__(256) void k_energy64(DevTables T, const float4 *__r
     // inversions: E = K (C0 + C1 c + C2 cos 2w)
     for (int q = tid; q < td.ninv; q += blockDim.x) {
         const uint2 g = T.inv_ng[td.inv_off + q];
-        const d3 h21 = slot_dir(P, NG, (int)(g.x & 0xffffu)), h31 = slot_dir(P, NG, (int)(g.x >> 16));
-        const d3 h41 = slot_dir(P, NG, (int)g.y);
+        const d3 h21 = slot_dir(P, NG, (int)(g.x & 0xffffu), N), h31 = slot_dir(P, NG, (int)(g.x >> 16), N);
+        const d3 h41 = slot_dir(P, NG, (int)g.y, N);
         const double4 p = T.inv64[td.inv_off + q];
         d3 n123 = cross(h21, h31);
         const double nl = sqrt(dot(n123, n123));

@@ -20,6 +20,20 @@
 
 namespace smr {
 
+// Workgroup barrier of the fused kernel.  A one-wave workgroup needs no
+// hardware barrier: its lanes run in lockstep and its LDS accesses complete
+// in issue order, so a wavefront-scope fence (a code-motion barrier that
+// emits no waits) orders the phases; wider workgroups use s_barrier.
+template <int NW>
+__device__ __forceinline__ void block_sync() {
+    if constexpr (NW == 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        __syncthreads();
+    }
+}
+
 // ===========================================================================
 // fused persistent kernel
 // ===========================================================================
@@ -57,9 +71,9 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
     while (true) {
         // ---- fetch the next system from the work list ----
         if (tid == 0) s_next[0] = atomicAdd(A.work_counter, 1);
-        __syncthreads();
+        block_sync<NW>();
         const int wi = s_next[0];
-        __syncthreads();
+        block_sync<NW>();
         if (wi >= A.nwork) break;
         const int s = A.work_list[wi];
         const SysDesc sd = T.sys[s];
@@ -113,7 +127,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
             t_ang = T.ang_ng + td.ang_off; t_dih = T.dih_ng + td.dih_off;
             t_inv = T.inv_ng + td.inv_off; t_a2f = T.a2f_idx + td.a2f_off;
         }
-        __syncthreads();
+        block_sync<NW>();
 
         // cell of each own atom's stored surface stencil (A.stencil): the
         // PLQH-contracted 4x4x4 stencil is re-read from the grid only when
@@ -142,16 +156,16 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
 #pragma unroll
                     for (int sl = 0; sl < 4; sl++) {
                         if (ngs[sl] < 0) {
-                            s_h[4 * i + sl] = make_float4(0.f, 0.f, 0.f, 1e6f);
+                            s_h[sl * N + i] = make_float4(0.f, 0.f, 0.f, 1e6f);
                             continue;
                         }
                         float4 hn;
                         (void)bond_term(pi, f3(s_pos[ngs[sl]]), bks[sl], bls[sl], hn, fb[k]);
-                        s_h[4 * i + sl] = hn;
+                        s_h[sl * N + i] = hn;
                     }
                 }
             }
-            __syncthreads();
+            block_sync<NW>();
             // ---------------- phase B: angles, dihedrals, inversions ----------------
             {
                 const int i0inv = 4 * td.ndih, i0ang = i0inv + 4 * td.ninv;
@@ -159,35 +173,35 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     const unsigned g = t_ang[a];
                     float3 fpi, fpj, fpk;
                     (void)angle_term(s_h[g & 0xffffu], s_h[g >> 16], T.ang_par[td.ang_off + a], fpi, fpj, fpk);
-                    const int p = i0ang + 3 * a;
+                    const int p = i0ang + a, q = td.nangles;
                     fint_x[p] = fpi.x; fint_y[p] = fpi.y; fint_z[p] = fpi.z;
-                    fint_x[p + 1] = fpj.x; fint_y[p + 1] = fpj.y; fint_z[p + 1] = fpj.z;
-                    fint_x[p + 2] = fpk.x; fint_y[p + 2] = fpk.y; fint_z[p + 2] = fpk.z;
+                    fint_x[p + q] = fpj.x; fint_y[p + q] = fpj.y; fint_z[p + q] = fpj.z;
+                    fint_x[p + 2 * q] = fpk.x; fint_y[p + 2 * q] = fpk.y; fint_z[p + 2 * q] = fpk.z;
                 }
                 for (int d = tid; d < td.ndih; d += BLOCK) {
                     const uint2 g = t_dih[d];
                     float3 f1, f2, f3v, f4;
                     (void)dihedral_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y & 0xffffu], T.dih_par[td.dih_off + d],
                                             (int)(g.y >> 16), f1, f2, f3v, f4);
-                    const int p = 4 * d;
+                    const int p = d, q = td.ndih;
                     fint_x[p] = f1.x; fint_y[p] = f1.y; fint_z[p] = f1.z;
-                    fint_x[p + 1] = f2.x; fint_y[p + 1] = f2.y; fint_z[p + 1] = f2.z;
-                    fint_x[p + 2] = f3v.x; fint_y[p + 2] = f3v.y; fint_z[p + 2] = f3v.z;
-                    fint_x[p + 3] = f4.x; fint_y[p + 3] = f4.y; fint_z[p + 3] = f4.z;
+                    fint_x[p + q] = f2.x; fint_y[p + q] = f2.y; fint_z[p + q] = f2.z;
+                    fint_x[p + 2 * q] = f3v.x; fint_y[p + 2 * q] = f3v.y; fint_z[p + 2 * q] = f3v.z;
+                    fint_x[p + 3 * q] = f4.x; fint_y[p + 3 * q] = f4.y; fint_z[p + 3 * q] = f4.z;
                 }
                 for (int v = tid; v < td.ninv; v += BLOCK) {
                     const uint2 g = t_inv[v];
                     float3 f1, f2, f3v, f4;
                     (void)inversion_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y], T.inv_par[td.inv_off + v], f1, f2,
                                              f3v, f4);
-                    const int p = i0inv + 4 * v;
+                    const int p = i0inv + v, q = td.ninv;
                     fint_x[p] = f1.x; fint_y[p] = f1.y; fint_z[p] = f1.z;
-                    fint_x[p + 1] = f2.x; fint_y[p + 1] = f2.y; fint_z[p + 1] = f2.z;
-                    fint_x[p + 2] = f3v.x; fint_y[p + 2] = f3v.y; fint_z[p + 2] = f3v.z;
-                    fint_x[p + 3] = f4.x; fint_y[p + 3] = f4.y; fint_z[p + 3] = f4.z;
+                    fint_x[p + q] = f2.x; fint_y[p + q] = f2.y; fint_z[p + q] = f2.z;
+                    fint_x[p + 2 * q] = f3v.x; fint_y[p + 2 * q] = f3v.y; fint_z[p + 2 * q] = f3v.z;
+                    fint_x[p + 3 * q] = f4.x; fint_y[p + 3 * q] = f4.y; fint_z[p + 3 * q] = f4.z;
                 }
             }
-            __syncthreads();
+            block_sync<NW>();
             // ---------------- phase C0: pairs, each unordered pair once ----------------
             // On sweep d the thread of atom i pairs it with j = (i+d) mod N.
             // For fixed d this map is a permutation, so within one wave (all
@@ -212,13 +226,24 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     const int4 ng = my_ng[k];
                     KahanF3 pacc;
                     pacc.init(make_float3(0.f, 0.f, 0.f));
+                    // positions and REQ rows do not change during the sweep:
+                    // the next partner's rows are loaded before this pair's
+                    // reaction store, hiding their LDS latency
                     int j = own ? i : 0;
+                    int jn = (j + 1 == N) ? 0 : j + 1;
+                    float4 pj = s_pos[jn], rj = s_req[jn];
                     for (int d = 1; d <= nd; d++) {
-                        j = (j + 1 == N) ? 0 : j + 1;
+                        j = jn;
+                        const float4 pc = pj, rc = rj;
+                        jn = (j + 1 == N) ? 0 : j + 1;
+                        if (d < nd) {
+                            pj = s_pos[jn];
+                            rj = s_req[jn];
+                        }
                         const bool active = own && !(even && d == nd && i >= nd);
                         if (active && !(j == ng.x || j == ng.y || j == ng.z || j == ng.w)) {
                             float3 f;
-                            (void)ljq_pair(f3(s_pos[j]) - f3(pi4), ri, s_req[j], T.r2damp, f);
+                            (void)ljq_pair(f3(pc) - f3(pi4), ri, rc, T.r2damp, f);
                             pacc.add(f);
                             float4 fj = fpw[j];
                             fj.x -= f.x; fj.y -= f.y; fj.z -= f.z;
@@ -227,7 +252,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     }
                     pf[k] = pacc.value();
                 }
-                __syncthreads();
+                block_sync<NW>();
             }
             // ---------------- phase C: gather pieces + pairs + surface ----------------
             float3 F[APT];
@@ -242,10 +267,21 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                         acc.add(pf[k]);
                         for (int w = 0; w < NW; w++) acc.add(f3(s_fp[w * A.lds_atoms + i]));
                     }
+                    // pieces four at a time: the index and piece loads of a
+                    // group are independent, so their LDS latencies overlap;
+                    // the adds keep the list order
                     const unsigned short *idx = t_a2f + my_a2f[k].x;
-                    for (int q = 0; q < my_a2f[k].y; q++) {
-                        const int p = idx[q];
-                        acc.add(make_float3(fint_x[p], fint_y[p], fint_z[p]));
+                    const int cnt = my_a2f[k].y;
+                    for (int q = 0; q < cnt; q += 4) {
+                        int pp[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) pp[u] = idx[min(q + u, cnt - 1)];
+                        float3 pv[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) pv[u] = make_float3(fint_x[pp[u]], fint_y[pp[u]], fint_z[pp[u]]);
+#pragma unroll
+                        for (int u = 0; u < 4; u++)
+                            if (q + u < cnt) acc.add(pv[u]);
                     }
                     const float4 pi4 = s_pos[i];
                     const float3 pi = f3(pi4);
@@ -297,7 +333,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     }
                 f2m = wave_max(f2m);
                 if (lane == 0) s_red[wid * 8 + 6] = f2m;
-                __syncthreads();
+                block_sync<NW>();
                 if (tid == 0) {
                     double m = 0.0;
                     for (int w = 0; w < NW; w++) m = fmax(m, s_red[w * 8 + 6]);
@@ -305,7 +341,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     st.fmax = sqrt(m);
                     A.state[s] = st;
                 }
-                __syncthreads();
+                block_sync<NW>();
                 break;
             }
             // per-atom partial sums (FP64)
@@ -360,13 +396,13 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                         s_red[wid * 8 + 0] = r0; s_red[wid * 8 + 1] = r1;
                         s_red[wid * 8 + 2] = r2; s_red[wid * 8 + 3] = r3;
                     }
-                    __syncthreads();
+                    block_sync<NW>();
                     r0 = r1 = r2 = r3 = 0.0;
                     for (int w = 0; w < NW; w++) {
                         r0 += s_red[w * 8 + 0]; r1 += s_red[w * 8 + 1];
                         r2 += s_red[w * 8 + 2]; r3 = fmax(r3, s_red[w * 8 + 3]);
                     }
-                    __syncthreads();
+                    block_sync<NW>();
                 }
                 p_vf = r0; p_vv = r1; p_ff = r2; p_fm = r3;
             }
@@ -401,7 +437,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
             }
             // move (x += v*dt_old), own atoms only; every lane has finished
             // reading positions for this step's pairs before any is moved
-            __syncthreads();
+            block_sync<NW>();
 #pragma unroll
             for (int k = 0; k < APT; k++) {
                 const int i = tid + k * BLOCK;
@@ -447,10 +483,10 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                         A.vel[abase + i] = make_float4(my_v[k].x, my_v[k].y, my_v[k].z, 0.f);
                     }
                 }
-                __syncthreads();
+                block_sync<NW>();
                 break;
             }
-            __syncthreads();
+            block_sync<NW>();
         }
     }
 }
@@ -471,10 +507,10 @@ __global__ void k_plain_bonds(PlainArgs P) {
     float3 f = make_float3(0.f, 0.f, 0.f);
     float eb = 0.f;
     for (int sl = 0; sl < 4; sl++) {
-        if (ngs[sl] < 0) { P.hneigh[4 * i + sl] = make_float4(0.f, 0.f, 0.f, 1e6f); continue; }
+        if (ngs[sl] < 0) { P.hneigh[sl * td.natoms + i] = make_float4(0.f, 0.f, 0.f, 1e6f); continue; }
         float4 hn;
         eb += bond_term(pi, f3(P.pos[ngs[sl]]), bks[sl], bls[sl], hn, f);
-        P.hneigh[4 * i + sl] = hn;
+        P.hneigh[sl * td.natoms + i] = hn;
     }
     P.force[i] = make_float4(f.x, f.y, f.z, 0.f);
     P.eatom[i] = eb;
@@ -487,10 +523,10 @@ __global__ void k_plain_angles(PlainArgs P) {
     const unsigned g = P.tab.ang_ng[td.ang_off + a];
     float3 fpi, fpj, fpk;
     float e = angle_term(P.hneigh[g & 0xffffu], P.hneigh[g >> 16], P.tab.ang_par[td.ang_off + a], fpi, fpj, fpk);
-    const int p = 4 * td.ndih + 4 * td.ninv + 3 * a;
+    const int p = 4 * td.ndih + 4 * td.ninv + a, q = td.nangles;
     P.fint[p] = make_float4(fpi.x, fpi.y, fpi.z, e);
-    P.fint[p + 1] = make_float4(fpj.x, fpj.y, fpj.z, 0.f);
-    P.fint[p + 2] = make_float4(fpk.x, fpk.y, fpk.z, 0.f);
+    P.fint[p + q] = make_float4(fpj.x, fpj.y, fpj.z, 0.f);
+    P.fint[p + 2 * q] = make_float4(fpk.x, fpk.y, fpk.z, 0.f);
 }
 
 __global__ void k_plain_dihedrals(PlainArgs P) {
@@ -501,11 +537,11 @@ __global__ void k_plain_dihedrals(PlainArgs P) {
     float3 f1, f2, f3v, f4;
     float e = dihedral_term(P.hneigh[g.x & 0xffffu], P.hneigh[g.x >> 16], P.hneigh[g.y & 0xffffu],
                             P.tab.dih_par[td.dih_off + d], (int)(g.y >> 16), f1, f2, f3v, f4);
-    const int p = 4 * d;
+    const int p = d, q = td.ndih;
     P.fint[p] = make_float4(f1.x, f1.y, f1.z, e);
-    P.fint[p + 1] = make_float4(f2.x, f2.y, f2.z, 0.f);
-    P.fint[p + 2] = make_float4(f3v.x, f3v.y, f3v.z, 0.f);
-    P.fint[p + 3] = make_float4(f4.x, f4.y, f4.z, 0.f);
+    P.fint[p + q] = make_float4(f2.x, f2.y, f2.z, 0.f);
+    P.fint[p + 2 * q] = make_float4(f3v.x, f3v.y, f3v.z, 0.f);
+    P.fint[p + 3 * q] = make_float4(f4.x, f4.y, f4.z, 0.f);
 }
 
 __global__ void k_plain_inversions(PlainArgs P) {
@@ -516,11 +552,11 @@ __global__ void k_plain_inversions(PlainArgs P) {
     float3 f1, f2, f3v, f4;
     float e = inversion_term(P.hneigh[g.x & 0xffffu], P.hneigh[g.x >> 16], P.hneigh[g.y], P.tab.inv_par[td.inv_off + v],
                              f1, f2, f3v, f4);
-    const int p = 4 * td.ndih + 4 * v;
+    const int p = 4 * td.ndih + v, q = td.ninv;
     P.fint[p] = make_float4(f1.x, f1.y, f1.z, e);
-    P.fint[p + 1] = make_float4(f2.x, f2.y, f2.z, 0.f);
-    P.fint[p + 2] = make_float4(f3v.x, f3v.y, f3v.z, 0.f);
-    P.fint[p + 3] = make_float4(f4.x, f4.y, f4.z, 0.f);
+    P.fint[p + q] = make_float4(f2.x, f2.y, f2.z, 0.f);
+    P.fint[p + 2 * q] = make_float4(f3v.x, f3v.y, f3v.z, 0.f);
+    P.fint[p + 3 * q] = make_float4(f4.x, f4.y, f4.z, 0.f);
 }
 
 // assemble + pairs + surface (assembleForces_UFF + getNonBond_GridFF_Bspline)
@@ -592,9 +628,9 @@ __global__ void k_plain_fire_reduce(PlainArgs P, double dt, double flim) {
         r[8] += P.epair[i];
         r[9] += (double)P.esurf[i];
     }
-    for (int a = tid; a < td.nangles; a += blockDim.x) r[5] += (double)P.fint[4 * td.ndih + 4 * td.ninv + 3 * a].w;
-    for (int d = tid; d < td.ndih; d += blockDim.x) r[6] += (double)P.fint[4 * d].w;
-    for (int v = tid; v < td.ninv; v += blockDim.x) r[7] += (double)P.fint[4 * td.ndih + 4 * v].w;
+    for (int a = tid; a < td.nangles; a += blockDim.x) r[5] += (double)P.fint[4 * td.ndih + 4 * td.ninv + a].w;
+    for (int d = tid; d < td.ndih; d += blockDim.x) r[6] += (double)P.fint[d].w;
+    for (int v = tid; v < td.ninv; v += blockDim.x) r[7] += (double)P.fint[4 * td.ndih + v].w;
     for (int q = 0; q < 11; q++) r[q] = (q == 3) ? wave_max(r[q]) : wave_sum(r[q]);
     if (lane == 0)
         for (int q = 0; q < 11; q++) red[wid][q] = r[q];
