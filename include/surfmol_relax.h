@@ -131,6 +131,20 @@ typedef struct {
                                  opencl/UFF.cl:1023-1204: 1-2 exclusions,
                                  E0 = sqrtEi*sqrtEj); 0: off                   */
     int surface;              /* 1: GridFF surface term; 0: off                */
+    double inter_cutoff;      /* 0: the systems are independent molecules.  > 0:
+                                 the batch is one coupled layer -- pairs of atoms
+                                 of different systems interact (LJ + Coulomb, no
+                                 exclusions) when closer than inter_cutoff [A],
+                                 found through a device cell list, and one FIRE
+                                 state drives all systems (the broad-phase culling
+                                 of NonBondedFF::eval_broad, molff/src/nonbonded.rs:
+                                 251-324; spacc/src/uniform_grid.rs:87-116).      */
+    double inter_switch;      /* coupled layer: 0 = hard cutoff (the reference's
+                                 culling); w > 0 = each intermolecular pair energy
+                                 is multiplied by S = 1 - 3x^2 + 2x^3,
+                                 x = (r - (inter_cutoff - w)) / w, over the last w
+                                 below the cutoff, so energy and forces stay
+                                 continuous when pairs cross it                    */
 } smr_ff_options;
 
 /* Create a batch of nsys systems.  Systems reference one of ntopo

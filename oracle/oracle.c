@@ -60,6 +60,15 @@ typedef struct {
     double gp0[3];
     double ginv_step[3];
     double alpha_morse;       /* GFFParams.y (UFF.cl:1681) */
+    /* coupled layer of molecules (GPU convention only): pairs of atoms of
+     * different molecules interact only within the hard cutoff, r^2 < rcut_inter2
+     * (NonBondedFF::eval_broad culls pairs beyond rc, nonbonded.rs:251-324);
+     * pairs inside one molecule always interact. NULL = one molecule. */
+    const int *mol_id;        /* [natoms] */
+    double rcut_inter2;
+    int layer_cells;          /* 1: find intermolecular pairs through a cell grid (same terms) */
+    double rswitch_inter;     /* > 0: intermolecular pair energies are switched off smoothly over
+                                 [rc - rswitch_inter, rc] by S(x) = 1 - 3x^2 + 2x^3; 0: hard cut */
 } orc_system;
 
 typedef struct {
@@ -325,10 +334,117 @@ static inline double orc_ljqh_cpu(v3 dp, const double *reqh, double r2damp, v3 *
  *   GPU convention: getNonBond (opencl/UFF.cl:1023-1204), non-PBC branch.
  *   CPU convention: NonBondedFF::eval / eval_nb_atom (nonbonded.rs:199-243).
  * Adds forces into fapos, returns the energy.                               */
+/* Intermolecular pair of a coupled layer within the cutoff: getLJQH, times
+ * the switch S(r) when a switching width is set (E S; force (fr S + E S'/r) dp). */
+static inline double orc_inter_pair(const orc_system *s, v3 dp, const double *ri, const double *rj, double r2damp,
+                                    v3 *f) {
+    const double e = orc_ljqh_gpu(dp, ri[0] + rj[0], ri[1] * rj[1], ri[2] * rj[2], r2damp, f);
+    if (s->rswitch_inter <= 0.0) return e;
+    const double rc = sqrt(s->rcut_inter2), ron = rc - s->rswitch_inter, r2 = v3_dot(dp, dp);
+    if (r2 <= ron * ron) return e;
+    const double r = sqrt(r2), w = s->rswitch_inter, x = (r - ron) / w;
+    const double S = 1.0 - x * x * (3.0 - 2.0 * x), dS_r = 6.0 * x * (x - 1.0) / w / r;
+    *f = v3_add(v3_mul(*f, S), v3_mul(dp, e * dS_r));
+    return e * S;
+}
+
+/* Coupled layer, GPU convention: intramolecular pairs as getNonBond (all
+ * pairs of the molecule but 1-2 neighbours), intermolecular pairs within the
+ * cutoff found through a uniform cell grid of cell size >= rc (the broad
+ * phase of NonBondedFF::eval_broad, nonbonded.rs:251-324, with the cell grid
+ * of spacc/src/uniform_grid.rs:87-116).  Molecules are contiguous atom
+ * ranges (mol_id non-decreasing).  Same terms as the brute-force loop, in
+ * another summation order.                                                 */
+static double orc_nb_eval_layer(const orc_system *s, const double *apos, double *fapos) {
+    const int na = s->natoms;
+    const double r2damp = s->rdamp * s->rdamp, rc2 = s->rcut_inter2, rc = sqrt(rc2);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < na; i++)
+        for (int k = 0; k < 3; k++) {
+            if (apos[3 * i + k] < lo[k]) lo[k] = apos[3 * i + k];
+            if (apos[3 * i + k] > hi[k]) hi[k] = apos[3 * i + k];
+        }
+    double h = rc;
+    int n[3];
+    long long ncell;
+    for (;;) {
+        ncell = 1;
+        for (int k = 0; k < 3; k++) {
+            n[k] = (int)((hi[k] - lo[k]) / h) + 1;
+            ncell *= n[k];
+        }
+        if (ncell <= (1 << 22)) break;
+        h *= 2.0;
+    }
+    int *cell = (int *)malloc(sizeof(int) * (size_t)na), *start = (int *)calloc((size_t)ncell + 1, sizeof(int));
+    int *order = (int *)malloc(sizeof(int) * (size_t)na), *m0 = (int *)malloc(sizeof(int) * (size_t)na);
+    int *m1 = (int *)malloc(sizeof(int) * (size_t)na);
+    for (int i = 0; i < na; i++) {
+        int c[3];
+        for (int k = 0; k < 3; k++) c[k] = (int)((apos[3 * i + k] - lo[k]) / h);
+        cell[i] = (c[2] * n[1] + c[1]) * n[0] + c[0];
+        start[cell[i] + 1]++;
+    }
+    for (long long q = 0; q < ncell; q++) start[q + 1] += start[q];
+    {
+        int *fill = (int *)malloc(sizeof(int) * (size_t)ncell);
+        memcpy(fill, start, sizeof(int) * (size_t)ncell);
+        for (int i = 0; i < na; i++) order[fill[cell[i]]++] = i;
+        free(fill);
+    }
+    /* atom ranges of the molecules */
+    for (int i = 0; i < na;) {
+        int j = i;
+        while (j < na && s->mol_id[j] == s->mol_id[i]) j++;
+        for (int q = i; q < j; q++) { m0[q] = i; m1[q] = j; }
+        i = j;
+    }
+    double etot = 0.0;
+    for (int ia = 0; ia < na; ia++) {
+        v3 pi = v3_ld(apos + 3 * ia);
+        const double *ri = s->reqs + 4 * ia;
+        const int *ng = s->neighs + 4 * ia;
+        v3 fe = {0, 0, 0};
+        double e = 0.0;
+        for (int ja = m0[ia]; ja < m1[ia]; ja++) {
+            if (ja == ia || ja == ng[0] || ja == ng[1] || ja == ng[2] || ja == ng[3]) continue;
+            const double *rj = s->reqs + 4 * ja;
+            v3 f;
+            e += orc_ljqh_gpu(v3_sub(v3_ld(apos + 3 * ja), pi), ri[0] + rj[0], ri[1] * rj[1], ri[2] * rj[2], r2damp, &f);
+            fe = v3_add(fe, f);
+        }
+        const int cx = cell[ia] % n[0], cy = (cell[ia] / n[0]) % n[1], cz = cell[ia] / (n[0] * n[1]);
+        for (int z = cz - 1; z <= cz + 1; z++) {
+            if (z < 0 || z >= n[2]) continue;
+            for (int y = cy - 1; y <= cy + 1; y++) {
+                if (y < 0 || y >= n[1]) continue;
+                for (int x = cx - 1; x <= cx + 1; x++) {
+                    if (x < 0 || x >= n[0]) continue;
+                    const int cc = (z * n[1] + y) * n[0] + x;
+                    for (int q = start[cc]; q < start[cc + 1]; q++) {
+                        const int ja = order[q];
+                        if (s->mol_id[ja] == s->mol_id[ia]) continue;
+                        v3 dp = v3_sub(v3_ld(apos + 3 * ja), pi);
+                        if (v3_dot(dp, dp) >= rc2) continue;
+                        v3 f;
+                        e += orc_inter_pair(s, dp, ri, s->reqs + 4 * ja, r2damp, &f);
+                        fe = v3_add(fe, f);
+                    }
+                }
+            }
+        }
+        etot += e;
+        v3_st(fapos + 3 * ia, v3_add(v3_ld(fapos + 3 * ia), fe));
+    }
+    free(cell); free(start); free(order); free(m0); free(m1);
+    return etot;
+}
+
 double orc_nb_eval(const orc_system *s, const double *apos, double *fapos) {
     int na = s->natoms;
     double r2damp = s->rdamp * s->rdamp;
     double etot = 0.0;
+    if (s->nb_mode == ORC_NB_GPU && s->mol_id && s->layer_cells) return orc_nb_eval_layer(s, apos, fapos);
     if (s->nb_mode == ORC_NB_GPU) {
         for (int ia = 0; ia < na; ia++) {
             v3 pi = v3_ld(apos + 3 * ia);
@@ -342,8 +458,13 @@ double orc_nb_eval(const orc_system *s, const double *apos, double *fapos) {
                 const double *rj = s->reqs + 4 * ja;
                 v3 dp = v3_sub(v3_ld(apos + 3 * ja), pi);
                 v3 f;
-                /* mixREQ_arithmetic: R0=Ri+Rj, E0=Ei*Ej (pre-sqrt), Q=Qi*Qj (UFF.cl:57-65) */
-                e += orc_ljqh_gpu(dp, ri[0] + rj[0], ri[1] * rj[1], ri[2] * rj[2], r2damp, &f);
+                if (s->mol_id && s->mol_id[ja] != s->mol_id[ia]) {
+                    if (v3_dot(dp, dp) >= s->rcut_inter2) continue;
+                    e += orc_inter_pair(s, dp, ri, rj, r2damp, &f);
+                } else {
+                    /* mixREQ_arithmetic: R0=Ri+Rj, E0=Ei*Ej (pre-sqrt), Q=Qi*Qj (UFF.cl:57-65) */
+                    e += orc_ljqh_gpu(dp, ri[0] + rj[0], ri[1] * rj[1], ri[2] * rj[2], r2damp, &f);
+                }
                 fe = v3_add(fe, f);
             }
             etot += e;

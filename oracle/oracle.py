@@ -132,6 +132,8 @@ class OrcSystem(ctypes.Structure):
         ("gnx", ctypes.c_int), ("gny", ctypes.c_int), ("gnz", ctypes.c_int),
         ("gp0", ctypes.c_double * 3), ("ginv_step", ctypes.c_double * 3),
         ("alpha_morse", ctypes.c_double),
+        ("mol_id", ctypes.POINTER(ctypes.c_int)), ("rcut_inter2", ctypes.c_double), ("layer_cells", ctypes.c_int),
+        ("rswitch_inter", ctypes.c_double),
     ]
 
 
@@ -700,6 +702,10 @@ class OracleSystem:
     grid_p0: tuple = (0.0, 0.0, 0.0)
     grid_step: tuple = (1.0, 1.0, 1.0)
     alpha_morse: float = 1.5
+    mol_id: np.ndarray | None = None      # coupled layer: molecule of each atom
+    rcut_inter: float = 0.0               # coupled layer: cutoff of pairs between molecules
+    layer_cells: bool = False             # coupled layer: cell-grid pair search (same terms, O(N))
+    rswitch_inter: float = 0.0            # coupled layer: width of the smooth switch below the cutoff
     _keep: list = field(default_factory=list)
 
     @property
@@ -750,6 +756,13 @@ class OracleSystem:
                 s.gp0[k] = self.grid_p0[k]
                 s.ginv_step[k] = 1.0 / self.grid_step[k]
             s.alpha_morse = self.alpha_morse
+        if self.mol_id is not None:
+            m = _i32(self.mol_id, (-1,))
+            self._keep.append(m)
+            s.mol_id = _pi(m)
+            s.rcut_inter2 = self.rcut_inter * self.rcut_inter
+            s.layer_cells = 1 if self.layer_cells else 0
+            s.rswitch_inter = self.rswitch_inter
         return s
 
 
@@ -780,6 +793,24 @@ def system_from_geometry(elems, apos, params: Params, charges=None, rcut_bond=1.
         bon_params=bp, ang_params=ap, dih_params=dp, inv_params=ip,
         ang_ngs=bake_angle_neighs(angles, nl), dih_ngs=bake_dihedral_neighs(dihedrals, nl),
         inv_ngs=bake_inversion_neighs(inversions, nl), reqs=reqs, **kw)
+
+
+def layer_from_molecules(mols, positions, params: Params, rcut_inter: float, **kw) -> OracleSystem:
+    """A coupled layer: the union of several molecules as one system whose
+    pairs between different molecules interact within rcut_inter only (the
+    GPU pair convention inside each molecule).  mols: objects with elems,
+    bonds, charges; positions: one [n,3] array per molecule."""
+    elems, apos, bonds, charges, mol_id = [], [], [], [], []
+    off = 0
+    for k, (m, x) in enumerate(zip(mols, positions)):
+        elems += list(m.elems)
+        apos.append(np.asarray(x, dtype=np.float64))
+        bonds += [(int(a) + off, int(b) + off) for a, b in np.asarray(m.bonds).reshape(-1, 2)]
+        charges.append(np.asarray(m.charges, dtype=np.float64))
+        mol_id += [k] * len(m.elems)
+        off += len(m.elems)
+    return system_from_geometry(elems, np.concatenate(apos), params, charges=np.concatenate(charges), bonds=bonds,
+                                nb_mode=NB_GPU, mol_id=np.array(mol_id, dtype=np.int32), rcut_inter=rcut_inter, **kw)
 
 
 # ---------------------------------------------------------------------------

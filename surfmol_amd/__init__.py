@@ -13,6 +13,9 @@ Python host mirror of the reference's set-up + relax API over the C ABI of
                           (MolWorld::eval_forces), relax_batched (FIRE,
                           persistent kernel), relax_plain (one launch sequence
                           per step, host FIRE), run_md (MolWorld::run_md).
+                          With inter_cutoff > 0 the systems form one coupled
+                          layer (intermolecular pairs through a cell list).
+* ``Server``           -- continuous-batching relax server.
 
 All compute runs in HIP kernels; importing works without a GPU, creating a
 Context needs one.
@@ -258,7 +261,7 @@ class Batch:
     """A set of systems on the device (shared topologies, per-system state)."""
 
     def __init__(self, ctx: Context, molecules, sys_topo, apos, grid: GridFF | None = None, rdamp=1.0,
-                 nonbonded=True, surface=True):
+                 nonbonded=True, surface=True, inter_cutoff=0.0, inter_switch=0.0):
         self.ctx = ctx
         self.grid = grid
         self.molecules = list(molecules)
@@ -268,7 +271,8 @@ class Batch:
         nat = sum(self.molecules[t].natoms for t in st)
         if x.shape[0] != nat:
             raise ValueError(f"apos has {x.shape[0]} atoms, systems need {nat}")
-        opts = _lib.SmrFFOptions(float(rdamp), 1 if nonbonded else 0, 1 if (surface and grid is not None) else 0)
+        opts = _lib.SmrFFOptions(float(rdamp), 1 if nonbonded else 0, 1 if (surface and grid is not None) else 0,
+                                 float(inter_cutoff), float(inter_switch))
         h = ctypes.c_void_p()
         check(lib().smr_batch_create(ctx._h, grid._h if grid is not None else None, len(self.molecules), topos,
                                      len(st), _iptr(st), _dptr(x), ctypes.byref(opts), ctypes.byref(h)))
@@ -380,7 +384,8 @@ class Server:
         cfg.max_steps = int(max_steps)
         cfg.fconv = float(fconv)
         cfg.fire = fire.cstruct()
-        cfg.ff = _lib.SmrFFOptions(float(rdamp), 1 if nonbonded else 0, 1 if (surface and grid is not None) else 0)
+        cfg.ff = _lib.SmrFFOptions(float(rdamp), 1 if nonbonded else 0, 1 if (surface and grid is not None) else 0,
+                                   0.0, 0.0)
         h = ctypes.c_void_p()
         check(lib().smr_server_create(ctx._h, grid._h if grid is not None else None, ctypes.byref(cfg),
                                       ctypes.byref(h)))

@@ -425,19 +425,29 @@ void fill_tables(const DevPools &D, const smr_gridff *grid, const smr_ff_options
         T.ginv64_x = 1.0 / grid->desc.step[0];
         T.ginv64_y = 1.0 / grid->desc.step[1];
         T.ginv64_z = 1.0 / grid->desc.step[2];
+        T.ostep64_x = grid->desc.step[0];
+        T.ostep64_y = grid->desc.step[1];
+        T.ostep64_z = grid->desc.step[2];
     } else {
         T.grid = nullptr;
         T.gnx = T.gny = T.gnz = 1;
         T.ginv_x = T.ginv_y = T.ginv_z = 1.f;
         T.ginv64_x = T.ginv64_y = T.ginv64_z = 1.0;
+        T.ostep64_x = T.ostep64_y = T.ostep64_z = 1.0;
     }
+    T.ostep_x = (float)T.ostep64_x;
+    T.ostep_y = (float)T.ostep64_y;
+    T.ostep_z = (float)T.ostep64_z;
     T.r2damp = (float)(o.rdamp * o.rdamp);
     T.use_pairs = o.nonbonded ? 1 : 0;
 }
 
 // system origin: grid-node aligned in x,y (so the surface lookup offset is an
-// exact integer), the grid's z origin in z; centroid-aligned without a grid.
-void system_origin(const double *x, int n, const smr_gridff *grid, bool surface, double org[3], int &iux, int &iuy) {
+// exact integer), the grid's z origin in z; without a grid, the centroid
+// rounded to whole Angstroms.  g = the origin in origin-grid steps
+// (org = p0 + g*step, step = the grid step or 1 A), unwrapped.
+void system_origin(const double *x, int n, const smr_gridff *grid, bool surface, double org[3], int &iux, int &iuy,
+                   int g[3]) {
     double c[3] = {0, 0, 0};
     for (int i = 0; i < n; i++)
         for (int k = 0; k < 3; k++) c[k] += x[3 * i + k];
@@ -451,8 +461,14 @@ void system_origin(const double *x, int n, const smr_gridff *grid, bool surface,
         org[2] = d.p0[2];
         iux = (int)(((jx % d.nx) + d.nx) % d.nx);
         iuy = (int)(((jy % d.ny) + d.ny) % d.ny);
+        g[0] = (int)jx;
+        g[1] = (int)jy;
+        g[2] = 0;
     } else {
-        for (int k = 0; k < 3; k++) org[k] = std::round(c[k]);
+        for (int k = 0; k < 3; k++) {
+            org[k] = std::round(c[k]);
+            g[k] = (int)org[k];
+        }
         iux = iuy = 0;
     }
 }
@@ -466,6 +482,129 @@ void refresh_shapes(smr_batch *b) {
 
 }  // namespace smr
 
+namespace smr {
+
+// Coupled layer set-up: per-atom system ids and REQ rows, the cell grid over
+// the initial positions (cells >= the cutoff, 1 A margin; atoms that leave
+// the box are binned into its edge cells, which keeps the 27-cell search
+// exact), cell-list buffers and the shared FIRE state.
+int layer_setup(smr_batch *b, const double *apos) {
+    const double rc = b->opts.inter_cutoff;
+    const int na = b->total_atoms, ns = b->nsys;
+    std::vector<int> asys(na);
+    std::vector<float4> areq(na);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    const double p0[3] = {b->grid ? b->grid->desc.p0[0] : 0.0, b->grid ? b->grid->desc.p0[1] : 0.0,
+                          b->grid ? b->grid->desc.p0[2] : 0.0};
+    for (int s = 0; s < ns; s++) {
+        const TopoDesc &td = b->hp.topo[b->hp.sys[s].topo];
+        for (int i = 0; i < td.natoms; i++) {
+            const int a = b->atom_off[s] + i;
+            asys[a] = s;
+            areq[a] = b->hp.req[td.atom_off + i];
+            for (int k = 0; k < 3; k++) {
+                // the origin-grid frame of the kernels: global position - p0
+                const double x = apos[3 * (size_t)a + k] - p0[k];
+                lo[k] = std::min(lo[k], x);
+                hi[k] = std::max(hi[k], x);
+            }
+        }
+    }
+    // atoms that leave the box are binned into its edge cells (exact for the
+    // 27-cell search), so a small margin suffices
+    const double margin = 1.0;
+    double h = rc * (1.0 + 1e-4) + 1e-3;
+    int n[3];
+    for (;;) {
+        long long tot = 1;
+        for (int k = 0; k < 3; k++) {
+            n[k] = std::max(1, (int)std::ceil((hi[k] - lo[k] + 2.0 * margin) / h));
+            tot *= n[k];
+        }
+        if (tot <= (1 << 20)) break;
+        h *= 1.5;   // a very sparse layer: fewer, larger cells
+    }
+    CellGrid &C = b->cells;
+    memset(&C, 0, sizeof(C));
+    C.ncx = n[0]; C.ncy = n[1]; C.ncz = n[2];
+    C.nox = n[0] > 1 ? 3 : 1;
+    C.noy = n[1] > 1 ? 3 : 1;
+    C.noz = n[2] > 1 ? 3 : 1;
+    C.lox = (float)(lo[0] - margin);
+    C.loy = (float)(lo[1] - margin);
+    C.loz = (float)(lo[2] - margin);
+    C.inv_h = (float)(1.0 / h);
+    C.rc2 = (float)(rc * rc);
+    C.rc2_64 = rc * rc;
+    const double w = b->opts.inter_switch;
+    C.w64 = w;
+    C.ron64 = rc - w;
+    C.ron = (float)(rc - w);
+    C.ron2 = w > 0.0 ? (float)((rc - w) * (rc - w)) : C.rc2;
+    C.inv_w = w > 0.0 ? (float)(1.0 / w) : 0.f;
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    hipStream_t st = b->ctx->stream;
+    hipError_t e = b->atom_sys.upload(asys.data(), sizeof(int) * na, st);
+    if (e == hipSuccess) e = b->req_atom.upload(areq.data(), sizeof(float4) * na, st);
+    if (e == hipSuccess) e = b->cell_count.alloc(sizeof(int) * ncell);
+    if (e == hipSuccess) e = hipMemsetAsync(b->cell_count.ptr, 0, sizeof(int) * ncell, st);
+    if (e == hipSuccess) e = b->cell_fill.alloc(sizeof(int) * ncell);
+    if (e == hipSuccess) e = b->cell_start.alloc(sizeof(int) * (ncell + 1));
+    if (e == hipSuccess) e = b->cell_sidx.alloc(sizeof(int) * na);
+    if (e == hipSuccess) e = b->cell_spos.alloc(sizeof(float4) * na);
+    if (e == hipSuccess) e = b->cell_sreq.alloc(sizeof(float4) * na);
+    if (e == hipSuccess) e = b->atom_cell.alloc(sizeof(int) * na);
+    if (e == hipSuccess) e = b->cell_part.alloc(sizeof(float4) * 27 * (size_t)na);
+    if (e == hipSuccess) e = b->slot_of.alloc(sizeof(int) * na);
+    if (e == hipSuccess) e = b->partials.alloc(sizeof(double4) * ns);
+    if (e == hipSuccess) e = b->layer_state.alloc(sizeof(LayerState));
+    if (e == hipSuccess) e = hipMemsetAsync(b->layer_state.ptr, 0, sizeof(LayerState), st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "smr_batch_create: layer buffers");
+    C.spos = (const float4 *)b->cell_spos.ptr;
+    C.sreq = (const float4 *)b->cell_sreq.ptr;
+    C.start = (const int *)b->cell_start.ptr;
+    C.part = (const float4 *)b->cell_part.ptr;
+    C.slot_of = (const int *)b->slot_of.ptr;
+    C.natoms = na;
+    b->layer = true;
+    return SMR_OK;
+}
+
+LayerArgs layer_args(smr_batch *b) {
+    LayerArgs L;
+    memset(&L, 0, sizeof(L));
+    L.tab = b->tab;
+    L.pos = (float4 *)b->pos.ptr;
+    L.vel = (float4 *)b->vel.ptr;
+    L.force = (const float4 *)b->force.ptr;
+    L.atom_sys = (const int *)b->atom_sys.ptr;
+    L.req_atom = (const float4 *)b->req_atom.ptr;
+    L.natoms = b->total_atoms;
+    L.nsys = b->nsys;
+    L.count = (int *)b->cell_count.ptr;
+    L.fill = (int *)b->cell_fill.ptr;
+    L.start = (int *)b->cell_start.ptr;
+    L.sidx = (int *)b->cell_sidx.ptr;
+    L.spos = (float4 *)b->cell_spos.ptr;
+    L.sreq = (float4 *)b->cell_sreq.ptr;
+    L.atom_cell = (int *)b->atom_cell.ptr;
+    L.part = (float4 *)b->cell_part.ptr;
+    L.slot_of = (int *)b->slot_of.ptr;
+    L.cells = b->cells;
+    L.layer = (LayerState *)b->layer_state.ptr;
+    return L;
+}
+
+// (re)build the cell list from the current positions, no move
+int layer_cells(smr_batch *b) {
+    hipError_t e = launch_layer_cells(layer_args(b), false, b->ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "layer cell list");
+    return SMR_OK;
+}
+
+}  // namespace smr
+
 extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, const smr_topology *topos, int nsys,
                                 const int *sys_topo, const double *apos, const smr_ff_options *opts, smr_batch **out) {
     if (!c || !out || !topos || !sys_topo || !apos || !opts)
@@ -474,6 +613,12 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
     if (ntopo <= 0 || nsys <= 0) return fail(SMR_ERR_INVALID, "smr_batch_create: ntopo and nsys must be > 0");
     if (opts->surface && !grid) return fail(SMR_ERR_INVALID, "smr_batch_create: surface requested but grid == NULL");
     if (!(opts->rdamp >= 0.0)) return fail(SMR_ERR_INVALID, "smr_batch_create: rdamp must be >= 0");
+    if (!(opts->inter_cutoff >= 0.0) || opts->inter_cutoff > 1000.0)
+        return fail(SMR_ERR_INVALID, "smr_batch_create: inter_cutoff must be in [0, 1000] A");
+    if (opts->inter_cutoff > 0.0 && !opts->nonbonded)
+        return fail(SMR_ERR_INVALID, "smr_batch_create: a coupled layer (inter_cutoff > 0) needs nonbonded = 1");
+    if (!(opts->inter_switch >= 0.0) || (opts->inter_switch > 0.0 && opts->inter_switch >= opts->inter_cutoff))
+        return fail(SMR_ERR_INVALID, "smr_batch_create: inter_switch must be in [0, inter_cutoff)");
     HIPCHK(hipSetDevice(c->device));
     auto *b = new smr_batch();
     b->ctx = c;
@@ -510,8 +655,13 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
         SysDesc sd;
         sd.topo = sys_topo[s];
         sd.atom_base = b->atom_off[s];
+        int g[3];
         system_origin(apos + 3 * (size_t)b->atom_off[s], b->hp.topo[sd.topo].natoms, b->grid, b->grid != nullptr,
-                      &b->origin[3 * s], sd.iux, sd.iuy);
+                      &b->origin[3 * s], sd.iux, sd.iuy, g);
+        sd.gx = g[0];
+        sd.gy = g[1];
+        sd.gz = g[2];
+        sd.pad = 0;
         b->hp.sys.push_back(sd);
     }
     hipError_t e = upload_pools(b->hp, b->dp, c->stream);
@@ -519,8 +669,11 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
     if (e == hipSuccess) e = b->vel.alloc(sizeof(float4) * b->total_atoms);
     if (e == hipSuccess) e = b->force.alloc(sizeof(float4) * b->total_atoms);
     if (e == hipSuccess) e = b->state.alloc(sizeof(SysState) * nsys);
+    if (e == hipSuccess) e = hipMemsetAsync(b->state.ptr, 0, sizeof(SysState) * nsys, c->stream);
     if (e == hipSuccess) e = b->work.alloc(sizeof(int) * nsys);
     if (e == hipSuccess && b->grid) e = b->stencil.alloc(sizeof(float4) * 16 * (size_t)b->total_atoms);
+    if (e == hipSuccess) e = b->stencil_key.alloc(sizeof(int) * (size_t)b->total_atoms);
+    if (e == hipSuccess) e = hipMemsetAsync(b->stencil_key.ptr, 0xff, sizeof(int) * (size_t)b->total_atoms, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, c->stream);
     if (e != hipSuccess) { delete b; ctx_release(c); return hip_fail(e, "smr_batch_create: device allocation"); }
     fill_tables(b->dp, b->grid, b->opts, b->tab);
@@ -541,6 +694,10 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
     e = hipMemcpyAsync(b->work.ptr, wl.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) { delete b; ctx_release(c); return hip_fail(e, "smr_batch_create: upload"); }
+    if (opts->inter_cutoff > 0.0) {
+        rc = layer_setup(b, apos);
+        if (rc) { delete b; ctx_release(c); return rc; }
+    }
     *out = b;
     return SMR_OK;
 }
@@ -629,6 +786,8 @@ int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch,
     A.state = (SysState *)b->state.ptr;
     A.work_list = (const int *)b->work.ptr;
     A.stencil = (float4 *)b->stencil.ptr;
+    A.stencil_key = (int *)b->stencil_key.ptr;
+    if (b->layer) A.cells = b->cells;
     A.steps_per_launch = steps_per_launch;
     A.mode = mode;
     return launch_plans(c->ss, b->ls, b->plans, A, kernel_ms);
@@ -639,6 +798,8 @@ int eval_energies64(smr_batch *b) {
     hipStream_t st = b->ctx->stream;
     hipError_t e = launch_energy64(b->tab, (const float4 *)b->pos.ptr, (SysState *)b->state.ptr,
                                    (const int *)b->work.ptr, b->nsys, st);
+    if (e == hipSuccess && b->layer)
+        e = launch_energy64_inter(layer_args(b), (SysState *)b->state.ptr, (const int *)b->work.ptr, b->nsys, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "k_energy64");
     return SMR_OK;
@@ -681,6 +842,106 @@ FireConst make_fire_const(const smr_fire_params *f, int max_steps, double fconv)
 
 }  // namespace smr
 
+namespace smr {
+
+// FIRE on a coupled layer: steps of [force launches (per size class) ->
+// move + cell-list rebuild] captured S at a time into a HIP graph; the host
+// replays the graph until the shared FIRE state leaves RUNNING.
+int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double fconv, smr_relax_result *res) {
+    smr_ctx *c = b->ctx;
+    hipStream_t st = c->stream;
+    constexpr int S = 16;
+    LayerState L0;
+    memset(&L0, 0, sizeof(L0));
+    L0.dt = fire->dt0;
+    L0.alpha = fire->alpha0;
+    L0.status = SMR_SYS_RUNNING;
+    L0.nsys = b->nsys;
+    std::vector<SysState> h(b->nsys);
+    for (auto &x : h) {
+        memset(&x, 0, sizeof(x));
+        x.dt = fire->dt0;
+        x.alpha = fire->alpha0;
+        x.status = SMR_SYS_RUNNING;
+    }
+    HIPCHK(hipMemcpyAsync(b->layer_state.ptr, &L0, sizeof(L0), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->state.ptr, h.data(), sizeof(SysState) * h.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, st));
+    int rc = layer_cells(b);
+    if (rc) return rc;
+    KernelArgs A;
+    memset(&A, 0, sizeof(A));
+    A.tab = b->tab;
+    A.fire = make_fire_const(fire, max_steps, fconv);
+    A.pos = (float4 *)b->pos.ptr;
+    A.vel = (float4 *)b->vel.ptr;
+    A.force = (float4 *)b->force.ptr;
+    A.state = (SysState *)b->state.ptr;
+    A.work_list = (const int *)b->work.ptr;
+    A.stencil = (float4 *)b->stencil.ptr;
+    A.stencil_key = (int *)b->stencil_key.ptr;
+    A.cells = b->cells;
+    A.layer = (LayerState *)b->layer_state.ptr;
+    A.partials = (double4 *)b->partials.ptr;
+    A.steps_per_launch = 1;
+    A.mode = KMODE_LAYER;
+    std::vector<KernelArgs> args;
+    rc = prepare_plans(b->ls, b->plans, A, args);
+    if (rc) return rc;
+    const LayerArgs LA = layer_args(b);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t ei = hipSuccess;
+    for (int k = 0; k < S && ei == hipSuccess; k++) {
+        ei = issue_plans(c->ss, b->ls, b->plans, args);
+        if (ei == hipSuccess) ei = launch_layer_cells(LA, true, st);
+    }
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (ei != hipSuccess || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(ei != hipSuccess ? ei : e, "capture of the layer steps");
+    }
+    e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return hip_fail(e, "hipGraphInstantiate (layer steps)");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    LayerState hs = L0;
+    const int max_rep = max_steps / S + 2;
+    for (int r = 0; r < max_rep && e == hipSuccess; r++) {
+        e = hipGraphLaunch(gexec, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(&hs, b->layer_state.ptr, sizeof(hs), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (hs.status != SMR_SYS_RUNNING) break;
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&b->last_kernel_ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(gexec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "layer relaxation");
+    rc = eval_energies64(b);
+    if (rc) return rc;
+    rc = read_results(b, res);
+    if (rc) return rc;
+    const int status = hs.status == SMR_SYS_RUNNING ? SMR_SYS_MAXSTEPS : hs.status;
+    for (int s = 0; s < b->nsys; s++) {
+        res[s].steps = hs.steps;
+        res[s].status = status;
+        res[s].fmax = hs.fmax;
+    }
+    return SMR_OK;
+}
+
+}  // namespace smr
+
 extern "C" int smr_batch_eval_forces(smr_batch *b, double *forces, double *energies) {
     if (!b || !forces || !energies) return fail(SMR_ERR_INVALID, "smr_batch_eval_forces: NULL argument");
     HIPCHK(hipSetDevice(b->ctx->device));
@@ -688,7 +949,9 @@ extern "C" int smr_batch_eval_forces(smr_batch *b, double *forces, double *energ
     std::vector<SysState> keep(b->nsys);
     HIPCHK(hipMemcpy(keep.data(), b->state.ptr, sizeof(SysState) * keep.size(), hipMemcpyDeviceToHost));
     FireConst fc = make_fire_const(nullptr, 1, 0.0);
-    int rc = run_fused(b, KMODE_EVAL, fc, 1, false, 0, 0, nullptr);
+    int rc = b->layer ? layer_cells(b) : SMR_OK;
+    if (rc) return rc;
+    rc = run_fused(b, KMODE_EVAL, fc, 1, false, 0, 0, nullptr);
     if (rc) return rc;
     rc = eval_energies64(b);
     if (rc) return rc;
@@ -714,6 +977,7 @@ extern "C" int smr_relax_batched(smr_batch *b, const smr_fire_params *fire, int 
     if (rc) return rc;
     if (max_steps <= 0 || !(fconv > 0.0)) return fail(SMR_ERR_INVALID, "smr_relax_batched: need max_steps > 0, fconv > 0");
     HIPCHK(hipSetDevice(b->ctx->device));
+    if (b->layer) return layer_relax(b, fire, max_steps, fconv, results);
     HIPCHK(hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, b->ctx->stream));
     FireConst fc = make_fire_const(fire, max_steps, fconv);
     rc = run_fused(b, KMODE_FIRE, fc, max_steps, true, fire->dt0, fire->alpha0, &b->last_kernel_ms);
@@ -726,6 +990,7 @@ extern "C" int smr_relax_batched(smr_batch *b, const smr_fire_params *fire, int 
 extern "C" int smr_run_md(smr_batch *b, int niter, double dt, double fconv, double flim, double damping, int *steps) {
     if (!b || !steps) return fail(SMR_ERR_INVALID, "smr_run_md: NULL argument");
     if (niter <= 0 || !(dt > 0.0) || !(flim > 0.0)) return fail(SMR_ERR_INVALID, "smr_run_md: need niter > 0, dt > 0, flim > 0");
+    if (b->layer) return fail(SMR_ERR_UNSUPPORTED, "smr_run_md: not available for a coupled layer (use smr_relax_batched)");
     HIPCHK(hipSetDevice(b->ctx->device));
     FireConst fc = make_fire_const(nullptr, niter, fconv);
     fc.md_dt = dt;
@@ -747,6 +1012,7 @@ extern "C" int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fi
     if (isys < 0 || isys >= b->nsys)
         return fail(SMR_ERR_INVALID, "smr_relax_plain: isys " + std::to_string(isys) + " not in [0," + std::to_string(b->nsys) + ")");
     if (max_steps <= 0 || !(fconv > 0.0)) return fail(SMR_ERR_INVALID, "smr_relax_plain: need max_steps > 0, fconv > 0");
+    if (b->layer) return fail(SMR_ERR_UNSUPPORTED, "smr_relax_plain: not available for a coupled layer");
     smr_ctx *c = b->ctx;
     HIPCHK(hipSetDevice(c->device));
     hipStream_t st = c->stream;

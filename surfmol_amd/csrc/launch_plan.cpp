@@ -159,12 +159,8 @@ void put(std::vector<unsigned char> &k, const T &v) {
 }
 }  // namespace
 
-int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
-                 float *kernel_ms) {
-    if (plans.empty()) {
-        if (kernel_ms) *kernel_ms = 0.f;
-        return SMR_OK;
-    }
+int prepare_plans(LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
+                  std::vector<KernelArgs> &args) {
     hipError_t e;
     // force-piece scratch of the classes whose pieces do not fit LDS
     std::vector<size_t> off(plans.size(), 0);
@@ -186,9 +182,9 @@ int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan
         ls.drop_graph();
         if ((e = ls.counters.alloc(sizeof(int) * NCLASS)) != hipSuccess) return hip_fail(e, "work counters");
     }
-    // byte copies keep the caller's zeroed padding, so the graph key below
-    // compares only meaningful bytes
-    std::vector<KernelArgs> args(plans.size());
+    // byte copies keep the caller's zeroed padding, so a graph key compares
+    // only meaningful bytes
+    args.assign(plans.size(), KernelArgs());
     for (size_t q = 0; q < plans.size(); q++) {
         const ClassPlan &p = plans[q];
         KernelArgs &A = args[q];
@@ -205,25 +201,41 @@ int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan
         A.pair_mode = p.shape.pair_mode ? 1 : 0;
         A.stencil = p.shape.stencil_cache ? base.stencil : nullptr;
     }
+    return SMR_OK;
+}
+
+hipError_t issue_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans,
+                       const std::vector<KernelArgs> &args) {
     const int naux = std::min<int>(3, (int)plans.size() - 1);
-    auto issue = [&]() -> hipError_t {
-        hipError_t r = launch_zero_i32((int *)ls.counters.ptr, NCLASS, ss.main);
-        if (r != hipSuccess) return r;
-        if (naux > 0) {
-            if ((r = hipEventRecord(ss.fork, ss.main)) != hipSuccess) return r;
-            for (int a = 0; a < naux; a++)
-                if ((r = hipStreamWaitEvent(ss.aux[a], ss.fork, 0)) != hipSuccess) return r;
-        }
-        for (size_t q = 0; q < plans.size(); q++) {
-            hipStream_t st = q == 0 ? ss.main : ss.aux[(q - 1) % 3];
-            if ((r = launch_relax(args[q], plans[q].shape, st)) != hipSuccess) return r;
-        }
-        for (int a = 0; a < naux; a++) {
-            if ((r = hipEventRecord(ss.join[a], ss.aux[a])) != hipSuccess) return r;
-            if ((r = hipStreamWaitEvent(ss.main, ss.join[a], 0)) != hipSuccess) return r;
-        }
-        return hipSuccess;
-    };
+    hipError_t r = launch_zero_i32((int *)ls.counters.ptr, NCLASS, ss.main);
+    if (r != hipSuccess) return r;
+    if (naux > 0) {
+        if ((r = hipEventRecord(ss.fork, ss.main)) != hipSuccess) return r;
+        for (int a = 0; a < naux; a++)
+            if ((r = hipStreamWaitEvent(ss.aux[a], ss.fork, 0)) != hipSuccess) return r;
+    }
+    for (size_t q = 0; q < plans.size(); q++) {
+        hipStream_t st = q == 0 ? ss.main : ss.aux[(q - 1) % 3];
+        if ((r = launch_relax(args[q], plans[q].shape, st)) != hipSuccess) return r;
+    }
+    for (int a = 0; a < naux; a++) {
+        if ((r = hipEventRecord(ss.join[a], ss.aux[a])) != hipSuccess) return r;
+        if ((r = hipStreamWaitEvent(ss.main, ss.join[a], 0)) != hipSuccess) return r;
+    }
+    return hipSuccess;
+}
+
+int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
+                 float *kernel_ms) {
+    if (plans.empty()) {
+        if (kernel_ms) *kernel_ms = 0.f;
+        return SMR_OK;
+    }
+    hipError_t e;
+    std::vector<KernelArgs> args;
+    int rc = prepare_plans(ls, plans, base, args);
+    if (rc) return rc;
+    auto issue = [&]() -> hipError_t { return issue_plans(ss, ls, plans, args); };
     if (ls.use_graph) {
         std::vector<unsigned char> key;
         for (size_t q = 0; q < plans.size(); q++) {

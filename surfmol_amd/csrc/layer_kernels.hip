@@ -1,0 +1,292 @@
+// layer_kernels.hip -- coupled layer of molecules: cell list, FIRE move and
+// intermolecular FP64 energies.
+//
+// A coupled layer is a batch whose systems interact: pairs of atoms of
+// different molecules closer than the cutoff add LJ + Coulomb (the broad-
+// phase culling of NonBondedFF::eval_broad, molff/src/nonbonded.rs:251-324,
+// with the uniform cell grid of spacc/src/uniform_grid.rs:87-116), and one
+// FIRE state drives every atom.  One step is
+//   k_relax (KMODE_LAYER)  forces of every molecule, v += F dt, per-molecule
+//                          FIRE sums; the last molecule to finish reduces
+//                          them and takes the FIRE decision
+//   k_layer_move_count     apply the decision (mix v, x += v dt), bin atoms
+//   k_cells_scan           cell starts (exclusive scan), clears the decision
+//   k_cells_scatter        atoms into their cells
+//   k_cells_sort           order each cell by atom index, gather the sorted
+//                          positions / REQ rows
+//   k_layer_inter          intermolecular forces: one workgroup per (cell,
+//                          neighbour cell), one lane per atom of the cell; one
+//                          partial force per atom and neighbour cell that the
+//                          next force launch adds in a fixed order
+// All kernels are launched every step (no host decision); once the layer has
+// converged the force launch returns at once and the move is a no-op.
+#include "smr_device.h"
+#include "smr_kernels.h"
+
+namespace smr {
+
+namespace {
+
+__device__ __forceinline__ int cell_of(const CellGrid &C, float gx, float gy, float gz) {
+    int cx = (int)floorf((gx - C.lox) * C.inv_h);
+    int cy = (int)floorf((gy - C.loy) * C.inv_h);
+    int cz = (int)floorf((gz - C.loz) * C.inv_h);
+    cx = min(max(cx, 0), C.ncx - 1);
+    cy = min(max(cy, 0), C.ncy - 1);
+    cz = min(max(cz, 0), C.ncz - 1);
+    return (cz * C.ncy + cy) * C.ncx + cx;
+}
+
+}  // namespace
+
+// Apply the FIRE decision of the step (if one is pending), then bin the atom.
+__global__ void k_layer_move_count(LayerArgs L, int move) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= L.natoms) return;
+    float4 p = L.pos[a];
+    if (move && L.layer->pending_move) {
+        const LayerState &S = *L.layer;
+        float3 v;
+        if (S.reset) {
+            v = make_float3(0.f, 0.f, 0.f);
+        } else {
+            const float4 v4 = L.vel[a], F = L.force[a];
+            v = make_float3(v4.x * S.mix_a + F.x * S.mix_b, v4.y * S.mix_a + F.y * S.mix_b, v4.z * S.mix_a + F.z * S.mix_b);
+        }
+        L.vel[a] = make_float4(v.x, v.y, v.z, 0.f);
+        p.x += v.x * S.dt_move;
+        p.y += v.y * S.dt_move;
+        p.z += v.z * S.dt_move;
+        L.pos[a] = p;
+    }
+    const SysDesc sd = L.tab.sys[L.atom_sys[a]];
+    const int c = cell_of(L.cells, p.x + (float)sd.gx * L.tab.ostep_x, p.y + (float)sd.gy * L.tab.ostep_y,
+                          p.z + (float)sd.gz * L.tab.ostep_z);
+    L.atom_cell[a] = c;
+    atomicAdd(&L.count[c], 1);
+}
+
+// Exclusive scan of the cell counts (one workgroup of 1024 threads): each
+// thread scans a contiguous chunk, chunk totals are scanned in LDS.  Zeroes
+// the counts for the next step and clears the applied decision.
+__global__ __launch_bounds__(1024) void k_cells_scan(LayerArgs L) {
+    __shared__ int part[1024];
+    const int n = L.cells.ncx * L.cells.ncy * L.cells.ncz;
+    const int t = threadIdx.x;
+    const int chunk = (n + 1023) / 1024;
+    const int c0 = min(n, t * chunk), c1 = min(n, c0 + chunk);
+    int sum = 0;
+    for (int c = c0; c < c1; c++) sum += L.count[c];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;   // exclusive prefix of this chunk
+    for (int c = c0; c < c1; c++) {
+        const int k = L.count[c];
+        L.start[c] = run;
+        L.fill[c] = run;
+        L.count[c] = 0;
+        run += k;
+    }
+    if (t == 1023) L.start[n] = part[1023];
+    if (t == 0) L.layer->pending_move = 0;
+}
+
+__global__ void k_cells_scatter(LayerArgs L) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= L.natoms) return;
+    const int slot = atomicAdd(&L.fill[L.atom_cell[a]], 1);
+    L.sidx[slot] = a;
+}
+
+// One workgroup per cell: order the cell's atoms by index (bitonic network
+// in LDS over 256 slots; a longer cell is sorted serially), then gather
+// positions (w = system) and REQ rows in that order.
+__global__ __launch_bounds__(256) void k_cells_sort(LayerArgs L) {
+    __shared__ int key[256];
+    const int c = blockIdx.x, t = threadIdx.x;
+    const int st = L.start[c], n = L.start[c + 1] - st;
+    if (n == 0) return;
+    if (n <= 256) {
+        key[t] = t < n ? L.sidx[st + t] : 0x7fffffff;
+        __syncthreads();
+        for (int k = 2; k <= 256; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const int o = t ^ j;
+                if (o > t) {
+                    const int a = key[t], b = key[o];
+                    if ((a > b) == ((t & k) == 0)) {
+                        key[t] = b;
+                        key[o] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        if (t < n) {
+            const int a = key[t];
+            L.sidx[st + t] = a;
+            L.slot_of[a] = st + t;
+            const float4 p = L.pos[a];
+            L.spos[st + t] = make_float4(p.x, p.y, p.z, __int_as_float(L.atom_sys[a]));
+            L.sreq[st + t] = L.req_atom[a];
+        }
+    } else if (t == 0) {
+        // a crowded cell: insertion sort and gather by one thread
+        for (int q = st + 1; q < st + n; q++) {
+            const int v = L.sidx[q];
+            int r = q - 1;
+            while (r >= st && L.sidx[r] > v) {
+                L.sidx[r + 1] = L.sidx[r];
+                r--;
+            }
+            L.sidx[r + 1] = v;
+        }
+        for (int q = st; q < st + n; q++) {
+            const int a = L.sidx[q];
+            L.slot_of[a] = q;
+            const float4 p = L.pos[a];
+            L.spos[q] = make_float4(p.x, p.y, p.z, __int_as_float(L.atom_sys[a]));
+            L.sreq[q] = L.req_atom[a];
+        }
+    }
+}
+
+// Intermolecular forces on the atoms of cell c from the atoms of its
+// neighbour cell number o (0..K-1, K = 27, or 9 / 3 / 1 when the grid is a
+// single cell thick along some axis): partners of other molecules within
+// the cutoff, in slot order, Kahan-summed.  Writes part[o][slot] for every
+// atom of the cell (zero when the neighbour cell lies outside the grid).
+__global__ __launch_bounds__(64) void k_layer_inter(LayerArgs L) {
+    const CellGrid &C = L.cells;
+    const DevTables &T = L.tab;
+    const int K = C.nox * C.noy * C.noz;
+    const int c = blockIdx.x / K, o = blockIdx.x - K * c;
+    const int s0 = C.start[c], n = C.start[c + 1] - s0;
+    if (n == 0) return;
+    const int cx = c % C.ncx, cy = (c / C.ncx) % C.ncy, cz = c / (C.ncx * C.ncy);
+    // neighbour offsets only along axes with more than one cell
+    const int x = cx + (C.nox == 3 ? o % 3 - 1 : 0);
+    const int y = cy + (C.noy == 3 ? (o / C.nox) % 3 - 1 : 0);
+    const int z = cz + (C.noz == 3 ? o / (C.nox * C.noy) - 1 : 0);
+    const bool inside = x >= 0 && x < C.ncx && y >= 0 && y < C.ncy && z >= 0 && z < C.ncz;
+    const int cc = inside ? (z * C.ncy + y) * C.ncx + x : 0;
+    const int q0 = inside ? C.start[cc] : 0, q1 = inside ? C.start[cc + 1] : 0;
+    float4 *out = L.part + (size_t)o * L.natoms;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const float4 pi = C.spos[s0 + t];
+        const float4 ri = C.sreq[s0 + t];
+        const int si = __float_as_int(pi.w);
+        const SysDesc di = T.sys[si];
+        KahanF3 acc;
+        acc.init(make_float3(0.f, 0.f, 0.f));
+        int last = -1;
+        float dgx = 0.f, dgy = 0.f, dgz = 0.f;
+        for (int q = q0; q < q1; q++) {
+            const float4 pj = C.spos[q];
+            const int sj = __float_as_int(pj.w);
+            if (sj == si) continue;
+            if (sj != last) {
+                const SysDesc dj = T.sys[sj];
+                dgx = (float)(dj.gx - di.gx) * T.ostep_x;
+                dgy = (float)(dj.gy - di.gy) * T.ostep_y;
+                dgz = (float)(dj.gz - di.gz) * T.ostep_z;
+                last = sj;
+            }
+            const float3 dp = make_float3((pj.x - pi.x) + dgx, (pj.y - pi.y) + dgy, (pj.z - pi.z) + dgz);
+            const float r2 = dot3(dp, dp);
+            if (r2 >= C.rc2) continue;
+            float3 f;
+            const float e = ljq_pair(dp, ri, C.sreq[q], T.r2damp, f);
+            if (r2 > C.ron2) {
+                // switching shell: E S, force (fr S + E S'/r) dp
+                const float r = sqrtf(r2), x = (r - C.ron) * C.inv_w;
+                const float S = 1.f - x * x * (3.f - 2.f * x), dS_r = 6.f * x * (x - 1.f) * C.inv_w / r;
+                f = f * S + dp * (e * dS_r);
+            }
+            acc.add(f);
+        }
+        const float3 v = acc.value();
+        out[s0 + t] = make_float4(v.x, v.y, v.z, 0.f);
+    }
+}
+
+// FP64 intermolecular pair energies of the listed systems (added to E[4]):
+// every partner of another system within the cutoff, counted at this
+// system's end (each pair appears once in each of its two systems).
+__global__ __launch_bounds__(256) void k_energy64_inter(LayerArgs L, SysState *state, const int *list) {
+    const int s = list[blockIdx.x];
+    const DevTables &T = L.tab;
+    const SysDesc sd = T.sys[s];
+    const TopoDesc td = T.topo[sd.topo];
+    const CellGrid &C = L.cells;
+    const double rc2 = C.rc2_64;
+    double e = 0.0;
+    for (int i = threadIdx.x; i < td.natoms; i += blockDim.x) {
+        const int a = sd.atom_base + i;
+        const float4 pi = L.pos[a];
+        const double4 ri = T.req64[td.atom_off + i];
+        const int c = L.atom_cell[a];
+        const int cx = c % C.ncx, cy = (c / C.ncx) % C.ncy, cz = c / (C.ncx * C.ncy);
+        for (int z = max(cz - 1, 0); z <= min(cz + 1, C.ncz - 1); z++)
+            for (int y = max(cy - 1, 0); y <= min(cy + 1, C.ncy - 1); y++)
+                for (int x = max(cx - 1, 0); x <= min(cx + 1, C.ncx - 1); x++) {
+                    const int cc = (z * C.ncy + y) * C.ncx + x;
+                    for (int q = C.start[cc]; q < C.start[cc + 1]; q++) {
+                        const float4 pj = C.spos[q];
+                        const int sj = __float_as_int(pj.w);
+                        if (sj == s) continue;
+                        const SysDesc dj = T.sys[sj];
+                        const double dx = ((double)pj.x - (double)pi.x) + (double)(dj.gx - sd.gx) * T.ostep64_x;
+                        const double dy = ((double)pj.y - (double)pi.y) + (double)(dj.gy - sd.gy) * T.ostep64_y;
+                        const double dz = ((double)pj.z - (double)pi.z) + (double)(dj.gz - sd.gz) * T.ostep64_z;
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 >= rc2) continue;
+                        const int aj = L.sidx[q];
+                        const int lj = aj - T.sys[sj].atom_base;
+                        const double4 rj = T.req64[T.topo[dj.topo].atom_off + lj];
+                        const double R0 = ri.x + rj.x, E0 = ri.y * rj.y, kQ = ri.z * rj.z;
+                        const double u2 = R0 * R0 / r2;
+                        const double u6 = u2 * u2 * u2;
+                        double eij = (u6 - 2.0) * u6 * E0 + kQ / sqrt(r2 + T.r2damp64);
+                        if (C.w64 > 0.0 && r2 > C.ron64 * C.ron64) {
+                            const double x = (sqrt(r2) - C.ron64) / C.w64;
+                            eij *= 1.0 - x * x * (3.0 - 2.0 * x);
+                        }
+                        e += eij;
+                    }
+                }
+    }
+    __shared__ double red[4];
+    const double r = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) acc += red[w];
+        state[s].E[4] += acc;
+    }
+}
+
+hipError_t launch_layer_cells(const LayerArgs &L, bool move, hipStream_t st) {
+    const int nb = (L.natoms + 255) / 256;
+    const int ncells = L.cells.ncx * L.cells.ncy * L.cells.ncz;
+    hipLaunchKernelGGL(k_layer_move_count, dim3(nb), dim3(256), 0, st, L, move ? 1 : 0);
+    hipLaunchKernelGGL(k_cells_scan, dim3(1), dim3(1024), 0, st, L);
+    hipLaunchKernelGGL(k_cells_scatter, dim3(nb), dim3(256), 0, st, L);
+    hipLaunchKernelGGL(k_cells_sort, dim3(ncells), dim3(256), 0, st, L);
+    hipLaunchKernelGGL(k_layer_inter, dim3(ncells * L.cells.nox * L.cells.noy * L.cells.noz), dim3(64), 0, st, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy64_inter(const LayerArgs &L, SysState *state, const int *list, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_energy64_inter, dim3(n), dim3(256), 0, st, L, state, list);
+    return hipGetLastError();
+}
+
+}  // namespace smr

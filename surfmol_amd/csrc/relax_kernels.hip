@@ -34,6 +34,63 @@ __device__ __forceinline__ void block_sync() {
     }
 }
 
+// The shared FIRE decision of a coupled layer, taken by the workgroup that
+// finished the step's last molecule: reduce the per-molecule sums in a fixed
+// order, then the FIRE update of relax_pentacene_uff.rs:162-181 (as in
+// k_relax) and the status; the move is applied by k_layer_move_count.
+template <int BLOCK>
+__device__ void layer_decide(const KernelArgs &A, double *s_red) {
+    constexpr int NW = BLOCK / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    LayerState *L = A.layer;
+    __threadfence();   // acquire: the other molecules' partial sums
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int q = tid; q < L->nsys; q += BLOCK) {
+        const double4 p = A.partials[q];
+        a0 += p.x; a1 += p.y; a2 += p.z; a3 = fmax(a3, p.w);
+    }
+    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2); a3 = wave_max(a3);
+    if (lane == 0) {
+        s_red[wid * 8 + 4] = a0; s_red[wid * 8 + 5] = a1;
+        s_red[wid * 8 + 6] = a2; s_red[wid * 8 + 7] = a3;
+    }
+    block_sync<NW>();
+    if (tid == 0) {
+        double vf = 0.0, vv = 0.0, ff = 0.0, fm = 0.0;
+        for (int w = 0; w < NW; w++) {
+            vf += s_red[w * 8 + 4]; vv += s_red[w * 8 + 5]; ff += s_red[w * 8 + 6]; fm = fmax(fm, s_red[w * 8 + 7]);
+        }
+        const FireConst &P = A.fire;
+        const double dt_old = L->dt;
+        const bool nonfinite = !(isfinite(vf) && isfinite(vv) && isfinite(ff));
+        if (vf > 0.0 && ff > 1e-30 && vv > 1e-30) {
+            L->mix_a = (float)(1.0 - L->alpha);
+            L->mix_b = (float)(L->alpha * sqrt(vv) / sqrt(ff));
+            L->reset = 0;
+            L->npos++;
+            if (L->npos > P.n_min) {
+                L->dt = fmin(L->dt * P.f_inc, P.dt_max);
+                L->alpha *= P.f_alpha;
+            }
+        } else {
+            L->npos = 0;
+            L->dt *= P.f_dec;
+            L->alpha = P.alpha0;
+            L->mix_a = L->mix_b = 0.f;
+            L->reset = 1;
+        }
+        L->dt_move = (float)dt_old;
+        L->steps++;
+        L->fmax = sqrt(fm);
+        if (nonfinite) L->status = SMR_SYS_NONFINITE;
+        else if (L->fmax < P.fconv) L->status = SMR_SYS_CONVERGED;
+        else if (L->steps >= P.max_steps) L->status = SMR_SYS_MAXSTEPS;
+        L->pending_move = 1;
+        L->ticket = 0;
+        __threadfence();
+    }
+}
+
 // ===========================================================================
 // fused persistent kernel
 // ===========================================================================
@@ -68,6 +125,8 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
     const unsigned short *t_a2f;
     float *fint_x, *fint_y, *fint_z;
 
+    // a coupled layer that has converged (or stopped) does no more work
+    if (A.mode == KMODE_LAYER && A.layer->status != SMR_SYS_RUNNING) return;
     while (true) {
         // ---- fetch the next system from the work list ----
         if (tid == 0) s_next[0] = atomicAdd(A.work_counter, 1);
@@ -79,7 +138,8 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
         const SysDesc sd = T.sys[s];
         const TopoDesc td = T.topo[sd.topo];
         SysState st = A.state[s];
-        if (st.status != SMR_SYS_RUNNING) continue;
+        // a finished system is not stepped again (forces are always evaluated)
+        if (A.mode != KMODE_EVAL && st.status != SMR_SYS_RUNNING) continue;
         const int N = td.natoms;
         const int abase = sd.atom_base;
         if (FINT_LDS) {
@@ -134,7 +194,10 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
         // the atom enters another cell
         int skey[APT];
 #pragma unroll
-        for (int k = 0; k < APT; k++) skey[k] = -1;
+        for (int k = 0; k < APT; k++) {
+            const int i = tid + k * BLOCK;
+            skey[k] = (A.stencil_key && i < N) ? A.stencil_key[abase + i] : -1;
+        }
         const int nsteps = A.steps_per_launch;
         double fmax_cur = st.fmax;
         int step_in_launch = 0;
@@ -296,6 +359,18 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                             acc.add(f);
                         }
                     }
+                    if (A.cells.spos) {
+                        // coupled layer: intermolecular forces of the
+                        // neighbour cells (k_layer_inter), in cell order
+                        const CellGrid &C = A.cells;
+                        const int slot = C.slot_of[abase + i];
+                        // (cells without partners contribute exact zeros, skipped)
+                        const int K = C.nox * C.noy * C.noz;
+                        for (int o = 0; o < K; o++) {
+                            const float4 pf4 = C.part[(size_t)o * C.natoms + slot];
+                            if (pf4.x != 0.f || pf4.y != 0.f || pf4.z != 0.f) acc.add(f3(pf4));
+                        }
+                    }
                     if (T.grid) {
                         float3 ul = make_float3(pi.x * T.ginv_x, pi.y * T.ginv_y, pi.z * T.ginv_z);
                         float4 fe;
@@ -322,7 +397,10 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
 #pragma unroll
                 for (int k = 0; k < APT; k++) {
                     const int i = tid + k * BLOCK;
-                    if (i < N) A.force[abase + i] = make_float4(F[k].x, F[k].y, F[k].z, 0.f);
+                    if (i < N) {
+                        A.force[abase + i] = make_float4(F[k].x, F[k].y, F[k].z, 0.f);
+                        if (A.stencil_key) A.stencil_key[abase + i] = skey[k];
+                    }
                 }
                 double f2m = 0.0;
 #pragma unroll
@@ -346,8 +424,8 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
             }
             // per-atom partial sums (FP64)
             double p_vf = 0.0, p_vv = 0.0, p_ff = 0.0, p_fm = 0.0;
-            if (A.mode == KMODE_FIRE) {
-                const float dtf = (float)st.dt;
+            if (A.mode == KMODE_FIRE || A.mode == KMODE_LAYER) {
+                const float dtf = (float)(A.mode == KMODE_LAYER ? A.layer->dt : st.dt);
                 const double flim = A.fire.flim;
 #pragma unroll
                 for (int k = 0; k < APT; k++) {
@@ -389,6 +467,41 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                 }
             }
             // block reduction of (vf, vv, ff, fmax^2)
+            if (A.mode == KMODE_LAYER) {
+                // coupled layer: clamped forces and updated velocities go to
+                // memory, this molecule's sums to its partial slot; the last
+                // molecule of the step takes the shared FIRE decision
+#pragma unroll
+                for (int k = 0; k < APT; k++) {
+                    const int i = tid + k * BLOCK;
+                    if (i < N) {
+                        A.force[abase + i] = make_float4(F[k].x, F[k].y, F[k].z, 0.f);
+                        A.vel[abase + i] = make_float4(my_v[k].x, my_v[k].y, my_v[k].z, 0.f);
+                        if (A.stencil_key) A.stencil_key[abase + i] = skey[k];
+                    }
+                }
+                double r0 = wave_sum(p_vf), r1 = wave_sum(p_vv), r2 = wave_sum(p_ff), r3 = wave_max(p_fm);
+                if (lane == 0) {
+                    s_red[wid * 8 + 0] = r0; s_red[wid * 8 + 1] = r1;
+                    s_red[wid * 8 + 2] = r2; s_red[wid * 8 + 3] = r3;
+                }
+                block_sync<NW>();
+                if (tid == 0) {
+                    r0 = r1 = r2 = r3 = 0.0;
+                    for (int w = 0; w < NW; w++) {
+                        r0 += s_red[w * 8 + 0]; r1 += s_red[w * 8 + 1];
+                        r2 += s_red[w * 8 + 2]; r3 = fmax(r3, s_red[w * 8 + 3]);
+                    }
+                    A.partials[s] = make_double4(r0, r1, r2, r3);
+                    __threadfence();
+                    const unsigned t = atomicAdd(&A.layer->ticket, 1u);
+                    s_next[1] = (t == (unsigned)(A.layer->nsys - 1)) ? 1 : 0;
+                }
+                block_sync<NW>();
+                if (s_next[1]) layer_decide<BLOCK>(A, s_red);
+                block_sync<NW>();
+                break;
+            }
             {
                 double r0 = wave_sum(p_vf), r1 = wave_sum(p_vv), r2 = wave_sum(p_ff), r3 = wave_max(p_fm);
                 if (NW > 1) {
@@ -479,6 +592,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                 for (int k = 0; k < APT; k++) {
                     const int i = tid + k * BLOCK;
                     if (i < N) {
+                        if (A.stencil_key) A.stencil_key[abase + i] = skey[k];
                         A.pos[abase + i] = s_pos[i];
                         A.vel[abase + i] = make_float4(my_v[k].x, my_v[k].y, my_v[k].z, 0.f);
                     }

@@ -108,7 +108,7 @@ struct Request {
 // 3 inversions per atom)
 constexpr size_t kAngPerAtom = 3, kDihPerAtom = 6, kInvPerAtom = 3;
 constexpr size_t kPiecesPerAtom = 4 * kDihPerAtom + 4 * kInvPerAtom + 3 * kAngPerAtom;
-constexpr size_t kBytesPerAtom = 3 * 16 + 5 * 16 + 8 + 4 * 32 + 256;   // state + tables + FP64 tables + stencil
+constexpr size_t kBytesPerAtom = 3 * 16 + 5 * 16 + 8 + 4 * 32 + 256 + 4;   // state, tables, FP64 tables, stencil
 constexpr size_t kBytesPerAng = 4 + 16 + 32, kBytesPerDih = 8 + 16 + 32, kBytesPerInv = 8 + 16 + 32;
 constexpr size_t kUnitBytes = kBytesPerAtom + kAngPerAtom * kBytesPerAng + kDihPerAtom * kBytesPerDih +
                               kInvPerAtom * kBytesPerInv + kPiecesPerAtom * 2;
@@ -123,7 +123,7 @@ struct smr_server {
     LaunchSet ls;
     DevTables tab;
     FireConst fc;
-    DevBuf pos, vel, force, stencil, ng, bk, bl, req, plq, a2f, bk64, bl64, req64, plq64;
+    DevBuf pos, vel, force, stencil, skey, ng, bk, bl, req, plq, a2f, bk64, bl64, req64, plq64;
     DevBuf ang_ng, ang_par, ang64, dih_ng, dih_par, dih64, inv_ng, inv_par, inv64, a2f_idx;
     DevBuf topo, sys, state, work, list, dstage, dout;
     RangeAlloc ra_atoms, ra_ang, ra_dih, ra_inv, ra_pc;
@@ -263,6 +263,7 @@ int upload(smr_server *S, const std::vector<Request *> &admit) {
         sd.atom_base = (int)r->a_off;
         sd.iux = r->iux;
         sd.iuy = r->iuy;
+        sd.gx = sd.gy = sd.gz = sd.pad = 0;
         SysState st;
         memset(&st, 0, sizeof(st));
         st.dt = S->cfg.fire.dt0;
@@ -282,6 +283,9 @@ int upload(smr_server *S, const std::vector<Request *> &admit) {
         g.add(r->pos0.data(), N * sizeof(float4), at<float4>(S->pos, a));
         zeros.assign(N, make_float4(0.f, 0.f, 0.f, 0.f));
         g.add(zeros.data(), N * sizeof(float4), at<float4>(S->vel, a));
+        // the slot's stored surface stencils belong to the previous request
+        std::vector<int> nokey(N, -1);
+        g.add(nokey.data(), N * sizeof(int), at<int>(S->skey, a));
         g.add(h.ang_ng.data(), td.nangles * sizeof(unsigned), at<unsigned>(S->ang_ng, r->an_off));
         g.add(h.ang_par.data(), td.nangles * sizeof(float4), at<float4>(S->ang_par, r->an_off));
         g.add(h.ang64.data(), td.nangles * sizeof(double4), at<double4>(S->ang64, r->an_off));
@@ -328,6 +332,7 @@ int run_chunk(smr_server *S) {
     A.state = (SysState *)S->state.ptr;
     A.work_list = (const int *)S->work.ptr;
     A.stencil = (float4 *)S->stencil.ptr;
+    A.stencil_key = (int *)S->skey.ptr;
     A.steps_per_launch = S->cfg.chunk_steps;
     A.mode = KMODE_FIRE;
     float ms = 0.f;
@@ -487,6 +492,8 @@ extern "C" int smr_server_create(smr_ctx *c, const smr_gridff *grid, const smr_s
     int rc = check_fire(&cfg->fire, "smr_server_create");
     if (rc) return rc;
     if (cfg->ff.surface && !grid) return fail(SMR_ERR_INVALID, "smr_server_create: surface requested but grid == NULL");
+    if (cfg->ff.inter_cutoff != 0.0)
+        return fail(SMR_ERR_UNSUPPORTED, "smr_server_create: requests are independent molecules (inter_cutoff must be 0)");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     size_t free_b = 0, total_b = 0;
@@ -521,7 +528,7 @@ extern "C" int smr_server_create(smr_ctx *c, const smr_gridff *grid, const smr_s
 #define ALLOC(buf, bytes) \
     if ((e = S->buf.alloc(bytes)) != hipSuccess) return fail_create(e, "smr_server_create: arena " #buf);
     ALLOC(pos, ca * sizeof(float4)) ALLOC(vel, ca * sizeof(float4)) ALLOC(force, ca * sizeof(float4))
-    ALLOC(stencil, ca * 16 * sizeof(float4))
+    ALLOC(stencil, ca * 16 * sizeof(float4)) ALLOC(skey, ca * sizeof(int))
     ALLOC(ng, ca * sizeof(int4)) ALLOC(bk, ca * sizeof(float4)) ALLOC(bl, ca * sizeof(float4))
     ALLOC(req, ca * sizeof(float4)) ALLOC(plq, ca * sizeof(float4)) ALLOC(a2f, ca * sizeof(int2))
     ALLOC(bk64, ca * sizeof(double4)) ALLOC(bl64, ca * sizeof(double4)) ALLOC(req64, ca * sizeof(double4))
@@ -616,7 +623,8 @@ extern "C" int smr_server_submit(smr_server *S, const smr_topology *topo, const 
         return fail(SMR_ERR_NOMEM, "smr_server_submit: the request's tables exceed the server's arenas");
     }
     const int N = t.natoms;
-    system_origin(apos, N, S->grid, S->grid != nullptr, r->org, r->iux, r->iuy);
+    int g[3];
+    system_origin(apos, N, S->grid, S->grid != nullptr, r->org, r->iux, r->iuy, g);
     r->pos0.resize(N);
     for (int i = 0; i < N; i++) {
         const double x = apos[3 * i] - r->org[0], y = apos[3 * i + 1] - r->org[1], z = apos[3 * i + 2] - r->org[2];

@@ -31,6 +31,8 @@ struct SysDesc {
     int topo;        // index into TopoDesc[]
     int atom_base;   // into the state arrays (pos/vel/force)
     int iux, iuy;    // grid-node offset of the system origin, wrapped into [0,nx),[0,ny)
+    int gx, gy, gz;  // unwrapped origin offset in origin-grid steps (org = p0 + g*step)
+    int pad;
 };
 
 // FIRE / MD state kept per system across launches
@@ -70,6 +72,9 @@ struct DevTables {
     const float4 *grid;    // [nx*ny*nz] z-fastest (P, L, Q, 0); NULL = no surface
     int gnx, gny, gnz;
     float ginv_x, ginv_y, ginv_z;
+    // origin grid of the per-system local frames (the GridFF nodes, or 1 A)
+    float ostep_x, ostep_y, ostep_z;
+    double ostep64_x, ostep64_y, ostep64_z;
     // options
     float r2damp;
     int use_pairs;

@@ -113,11 +113,17 @@ struct LaunchSet {
 };
 int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
                  float *kernel_ms);
+// the two halves of launch_plans, for callers that build their own sequence
+int prepare_plans(LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
+                  std::vector<KernelArgs> &args);
+hipError_t issue_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans,
+                       const std::vector<KernelArgs> &args);
 
 int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, HostPools &P, TopoDesc &td);
 hipError_t upload_pools(HostPools &P, DevPools &D, hipStream_t st);
 void fill_tables(const DevPools &D, const smr_gridff *grid, const smr_ff_options &o, DevTables &T);
-void system_origin(const double *x, int n, const smr_gridff *grid, bool surface, double org[3], int &iux, int &iuy);
+void system_origin(const double *x, int n, const smr_gridff *grid, bool surface, double org[3], int &iux, int &iuy,
+                   int g[3]);
 int check_fire(const smr_fire_params *f, const char *who);
 void ctx_retain(smr_ctx *c);
 void ctx_release(smr_ctx *c);
@@ -156,6 +162,13 @@ struct smr_batch {
     std::vector<double> origin;      // [3*nsys]
     smr::DevBuf pos, pos0, vel, force, state, work;
     smr::DevBuf stencil;             // float4 [16 * total_atoms] stored surface stencils
+    smr::DevBuf stencil_key;         // int [total_atoms] cell of each stored stencil
+    // coupled layer (opts.inter_cutoff > 0)
+    bool layer = false;
+    smr::DevBuf atom_sys, req_atom, cell_count, cell_fill, cell_start, cell_sidx, cell_spos, cell_sreq, atom_cell;
+    smr::DevBuf cell_part, slot_of;
+    smr::DevBuf partials, layer_state;
+    smr::CellGrid cells;
     smr::DevBuf plain_hneigh, plain_fint, plain_e, plain_cvf;
     std::vector<smr::ClassPlan> plans;
     smr::LaunchSet ls;

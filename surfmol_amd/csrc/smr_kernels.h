@@ -7,7 +7,39 @@
 
 namespace smr {
 
-enum { KMODE_FIRE = 0, KMODE_EVAL = 1, KMODE_MD = 2 };
+enum { KMODE_FIRE = 0, KMODE_EVAL = 1, KMODE_MD = 2, KMODE_LAYER = 3 };
+
+// Cell list of a coupled layer (atoms sorted by cell, by atom index inside a
+// cell), rebuilt every step.  Cells are at least as large as the cutoff, so
+// the 27 neighbouring cells hold every partner within it.
+struct CellGrid {
+    const float4 *spos;      // [natoms] local xyz of the atom, w = its system (int bits)
+    const float4 *sreq;      // [natoms] REQ rows in the same order
+    const int *start;        // [ncells + 1]
+    const float4 *part;      // [K][natoms] intermolecular force of each atom (by sorted slot) per neighbour cell
+    int nox, noy, noz;       // neighbour offsets per axis (3, or 1 for a single-cell axis); K = nox*noy*noz
+    const int *slot_of;      // [natoms] sorted slot of each atom
+    int natoms;
+    int ncx, ncy, ncz;
+    float lox, loy, loz;     // lower corner in the origin-grid frame
+    float inv_h;             // 1 / cell size
+    float rc2;               // cutoff squared (force test, FP32)
+    double rc2_64;           // cutoff squared (energy test, FP64)
+    float ron, ron2, inv_w;  // switch: start, start squared, 1 / width (ron2 = rc2: hard cutoff)
+    double ron64, w64;
+};
+
+// FIRE state of a coupled layer (one for all its systems) and the decision
+// of the last step for the move kernel.
+struct LayerState {
+    double dt, alpha;
+    int npos, steps, status, nsys;
+    double fmax;
+    float mix_a, mix_b, dt_move;
+    int reset;
+    int pending_move;        // the decision of this step still has to be applied
+    unsigned ticket;         // systems finished in the running force launch
+};
 
 struct KernelArgs {
     DevTables tab;
@@ -27,6 +59,10 @@ struct KernelArgs {
     int steps_per_launch;
     int mode;
     int pair_mode;           // 1: unordered-pair sweep, per-wave reaction buffers; 0: ordered pairs
+    int *stencil_key;        // [total atoms] cell of each stored stencil (-1: none), kept across launches
+    CellGrid cells;          // coupled layer: intermolecular partners (spos == NULL: none)
+    LayerState *layer;       // coupled layer: shared FIRE state (KMODE_LAYER)
+    double4 *partials;       // coupled layer: per-system (v.F, v.v, F.F, max F^2)
 };
 
 struct LaunchShape {
@@ -77,6 +113,24 @@ hipError_t launch_copy_desc(const CopyDesc *d, int n, hipStream_t st);
 // FP64 energies (state[s].E) of the listed systems at their current positions
 hipError_t launch_energy64(const DevTables &T, const float4 *pos, SysState *state, const int *list, int n,
                            hipStream_t st);
+// coupled layer: cell-list build, move, inter-molecular FP64 energies
+struct LayerArgs {
+    DevTables tab;
+    float4 *pos, *vel;
+    const float4 *force;
+    const int *atom_sys;     // [natoms]
+    const float4 *req_atom;  // [natoms]
+    int natoms, nsys;
+    int *count, *fill, *start, *sidx;   // [ncells], [ncells], [ncells+1], [natoms]
+    float4 *spos, *sreq;
+    int *atom_cell;          // [natoms]
+    float4 *part;            // [K][natoms]
+    int *slot_of;            // [natoms]
+    CellGrid cells;
+    LayerState *layer;
+};
+hipError_t launch_layer_cells(const LayerArgs &L, bool move, hipStream_t st);
+hipError_t launch_energy64_inter(const LayerArgs &L, SysState *state, const int *list, int n, hipStream_t st);
 hipError_t launch_plain_step_forces(const PlainArgs &P, int natoms, int nang, int ndih, int ninv, hipStream_t st);
 hipError_t launch_plain_reduce(const PlainArgs &P, double dt, double flim, hipStream_t st);
 hipError_t launch_plain_apply(const PlainArgs &P, int natoms, float a, float b, int reset, float dt, hipStream_t st);

@@ -10,6 +10,7 @@ CPU baseline.
 * ``random_polyhex``   -- random planar fused 6-ring graphs with C/N/O/H.
 * ``rocksalt_slab``    -- 2-layer rock-salt (001) slab, +-0.9 e.
 * ``place_on_surface`` -- in-plane rotation + xy offset + height.
+* ``layer_lattice``    -- a coupled monolayer of PTCDA-like molecules.
 * ``poisson_arrivals`` -- Poisson request stream for the serving loop.
 
 Geometric conventions (chosen here, stated once):
@@ -327,6 +328,26 @@ def uniform_batch_poses(nsys: int, L: float = 24.0, hmin: float = 3.0, hmax: flo
     xy = rng.uniform(0.0, L, (nsys, 2))
     h = rng.uniform(hmin, hmax, nsys)
     return ang, xy, h
+
+
+def layer_lattice(nx: int = 16, ny: int = 32, spacing: float = 11.0, seed: int = SEED_BASE + 3,
+                  hmin: float = 3.0, hmax: float = 4.0, jitter_deg: float = 2.0, origin=(5.5, 5.5)):
+    """A monolayer of PTCDA-like molecules on an nx x ny square lattice.
+
+    At 11 A spacing a single orientation overlaps neighbours (closest
+    contacts 1.6 A), so the molecules alternate 0 / 90 degrees on the two
+    checkerboard sublattices (herringbone-like; closest contacts 2.2 A), with
+    a seeded rotation jitter of +-jitter_deg and heights U(hmin, hmax).
+    Returns (molecule, [positions [38,3] per lattice site])."""
+    mol = ptcda_like(SEED_BASE)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    X = []
+    for i in range(nx):
+        for j in range(ny):
+            th = (0.0 if (i + j) % 2 == 0 else 90.0) + rng.uniform(-jitter_deg, jitter_deg)
+            h = rng.uniform(hmin, hmax)
+            X.append(place_on_surface(mol, math.radians(th), (origin[0] + spacing * i, origin[1] + spacing * j), h))
+    return mol, X
 
 
 def poisson_arrivals(rate: float, n: int, seed: int):

@@ -1,0 +1,110 @@
+"""Coupled layer on the GPU: intermolecular pairs within a cutoff through the
+device cell list, one FIRE state for all molecules.
+
+Checked against the FP64 oracle of the same layer (pairs of different
+molecules within the cutoff, brute force): per-atom forces within 1e-5 of
+max|F| (the cutoff test runs in FP32 on the device; a pair within ~1e-6 A of
+the cutoff could fall on the other side), energies within 1e-6 eV/atom; the
+relaxed layer is an FP64 minimum; repeated relaxations are bit-identical.
+Both the reference's hard cutoff and the smooth switch (energy times
+S = 1 - 3x^2 + 2x^3 over the last w below the cutoff) are checked.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_force_err
+from surface_setup import ALPHA, RDAMP, fire_params_internal, oracle_grid, small_grid_spec
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def surf(ctx, oracle, params):
+    import surfmol_amd as S
+    spec = small_grid_spec()
+    E, C, (el, pos, q, reqs) = oracle_grid(oracle, params, spec)
+    gs = S.GridSpec(spec["nx"], spec["ny"], spec["nz"], spec["p0"], spec["step"], ALPHA, RDAMP)
+    g = S.GridFF.build(ctx, gs, pos, reqs, npbc=2)
+    return dict(spec=spec, grid=g, C=g.coefficients().astype(np.float64))
+
+
+def _layer(ctx, surf, oracle, params, nx, ny, rc, spacing=11.0, seed=7, surface=True, w=0.0):
+    import surfmol_amd as S
+    from surfmol_amd import workloads as W
+    mol, X = W.layer_lattice(nx, ny, spacing=spacing, seed=seed)
+    hm = S.Molecule.build(mol.elems, X[0], mol.charges, bonds=mol.bonds)
+    b = S.Batch(ctx, [hm], np.zeros(len(X), dtype=np.int32), np.concatenate(X), grid=surf["grid"], rdamp=RDAMP,
+                surface=surface, inter_cutoff=rc, inter_switch=w)
+    spec = surf["spec"]
+    kw = {}
+    if surface:
+        kw = dict(grid=surf["C"], grid_ns=(spec["nx"], spec["ny"], spec["nz"]), grid_p0=spec["p0"],
+                  grid_step=spec["step"], alpha_morse=spec["alpha"])
+    osys = oracle.layer_from_molecules([mol] * len(X), X, params, rc, rdamp=RDAMP, rswitch_inter=w, **kw)
+    return mol, X, b, osys
+
+
+@pytest.mark.parametrize("rc,w", [(6.0, 0.0), (12.0, 0.0), (12.0, 2.0)])
+def test_layer_eval_forces(ctx, surf, oracle, params, rc, w):
+    mol, X, b, osys = _layer(ctx, surf, oracle, params, 4, 3, rc, w=w)
+    F, E = b.eval_forces()
+    Xd = b.positions()
+    fo, eo = oracle.eval_forces(osys, Xd)
+    err = rel_force_err(F, fo)
+    print("rc", rc, "force err", err)
+    assert err < 1e-5
+    n = len(Xd)
+    # per-molecule energies add up to the layer's (intermolecular pairs
+    # counted once at each end, as getNonBond counts every pair twice)
+    dE = np.abs(E.sum(axis=0) - eo)
+    assert (dE / n < 1e-6).all(), (E.sum(axis=0), eo)
+
+
+def test_layer_without_neighbours_is_independent(ctx, surf, oracle, params):
+    """Molecules 40 A apart: the layer equals the independent batch."""
+    import surfmol_amd as S
+    from surfmol_amd import workloads as W
+    mol, X = W.layer_lattice(2, 2, spacing=40.0, seed=9)
+    hm = S.Molecule.build(mol.elems, X[0], mol.charges, bonds=mol.bonds)
+    args = (ctx, [hm], np.zeros(4, dtype=np.int32), np.concatenate(X))
+    bl = S.Batch(*args, grid=surf["grid"], rdamp=RDAMP, inter_cutoff=12.0)
+    bi = S.Batch(*args, grid=surf["grid"], rdamp=RDAMP)
+    Fl, El = bl.eval_forces()
+    Fi, Ei = bi.eval_forces()
+    assert np.array_equal(Fl, Fi)
+    assert np.abs(El - Ei).max() < 1e-12
+
+
+def test_layer_relax_is_fp64_minimum(ctx, surf, oracle, params):
+    import surfmol_amd as S
+    # the smooth switch: with a hard cutoff, pairs crossing it make the
+    # forces jump by ~4e-3 eV/A and FIRE may stall above Fmax = 1e-3
+    mol, X, b, osys = _layer(ctx, surf, oracle, params, 3, 3, 12.0, w=2.0)
+    fp = fire_params_internal()
+    res = b.relax_batched(S.FireParams(**fp), 3000, 1e-3)
+    assert all(r["status"] == S.SMR_SYS_CONVERGED for r in res), res[0]
+    assert len({r["steps"] for r in res}) == 1          # one FIRE state
+    Xf = b.positions()
+    fo, eo = oracle.eval_forces(osys, Xf)
+    n = len(Xf)
+    assert np.linalg.norm(fo, axis=1).max() < 2e-3
+    Etot = sum(r["energy"] for r in res)
+    assert abs(eo.sum() - Etot) / n < 1e-6, (eo.sum(), Etot)
+    rf = oracle.relax_fire(osys, oracle.FireParams(**fp), 3000, 1e-3, apos=Xf)
+    assert rf["converged"] and abs(rf["E"] - Etot) / n < 1e-6, (rf["E"], Etot)
+    print("layer steps", res[0]["steps"], "E", Etot)
+
+
+def test_layer_relax_is_reproducible(ctx, surf, oracle, params):
+    import surfmol_amd as S
+    mol, X, b, osys = _layer(ctx, surf, oracle, params, 4, 4, 12.0, w=2.0)
+    fp = S.FireParams(**fire_params_internal())
+    r1 = b.relax_batched(fp, 600, 1e-3)
+    x1 = b.positions()
+    for _ in range(2):
+        b.reset()
+        b.eval_forces()
+        r2 = b.relax_batched(fp, 600, 1e-3)
+        assert np.array_equal(b.positions(), x1)
+        assert [r["energy"] for r in r1] == [r["energy"] for r in r2]
+        assert r1[0]["steps"] == r2[0]["steps"]

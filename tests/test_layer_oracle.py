@@ -1,0 +1,133 @@
+"""The FP64 oracle of a coupled layer (molecules interacting through pairs
+within a cutoff) against independent checks:
+
+* molecules farther apart than the cutoff do not interact: the layer's
+  forces and energies are the sum of the separate molecules';
+* the intermolecular part equals a brute-force double loop over all atom
+  pairs of different molecules within the cutoff (the reference's own check
+  of its broad phase, molff/tests/test_broad_phase.rs:79-106, is this
+  equivalence);
+* Newton's third law: the intermolecular forces sum to zero, and the energy
+  counts every intermolecular pair at both ends.
+"""
+import numpy as np
+
+from surfmol_amd import workloads as W
+from surface_setup import RDAMP
+
+KC = 14.3996448915
+
+
+def _layer(oracle, params, nx, ny, spacing, rc, seed=5, w=0.0):
+    mol, X = W.layer_lattice(nx, ny, spacing=spacing, seed=seed)
+    sysL = oracle.layer_from_molecules([mol] * len(X), X, params, rc, rdamp=RDAMP, rswitch_inter=w)
+    singles = [oracle.system_from_geometry(mol.elems, x, params, charges=mol.charges, bonds=mol.bonds,
+                                           nb_mode=oracle.NB_GPU, rdamp=RDAMP) for x in X]
+    return mol, X, sysL, singles
+
+
+def _brute_inter(mol, X, reqs_one, rc, w=0.0):
+    """Forces and (doubly counted) energy of all pairs of atoms of different
+    molecules closer than rc: a plain double loop; with w > 0 each pair
+    energy is multiplied by S = 1 - 3x^2 + 2x^3, x = (r - (rc - w)) / w, over
+    the last w below rc."""
+    n = mol.natoms
+    A = np.concatenate(X)
+    R = np.tile(reqs_one, (len(X), 1))
+    F = np.zeros_like(A)
+    E = 0.0
+    for i in range(len(A)):
+        for j in range(len(A)):
+            if i // n == j // n:
+                continue
+            d = A[j] - A[i]
+            r2 = d @ d
+            if r2 >= rc * rc:
+                continue
+            R0 = R[i, 0] + R[j, 0]
+            E0 = R[i, 1] * R[j, 1]
+            q = KC * R[i, 2] * R[j, 2]
+            u6 = (R0 * R0 / r2) ** 3
+            ec = q / np.sqrt(r2 + RDAMP * RDAMP)
+            e = E0 * (u6 * u6 - 2 * u6) + ec
+            dEdr_over_r = E0 * (-12 * u6 * u6 + 12 * u6) / r2 - ec / (r2 + RDAMP * RDAMP)
+            r = np.sqrt(r2)
+            if w > 0 and r > rc - w:
+                x = (r - (rc - w)) / w
+                S = 1 - 3 * x * x + 2 * x ** 3
+                dS = (-6 * x + 6 * x * x) / w
+                dEdr_over_r = dEdr_over_r * S + e * dS / r
+                e = e * S
+            E += e
+            F[i] += dEdr_over_r * d
+    return F, E
+
+
+def test_far_molecules_do_not_interact(oracle, params):
+    mol, X, sysL, singles = _layer(oracle, params, 2, 2, 40.0, 12.0)
+    F, E = oracle.eval_forces(sysL)
+    Fs = np.concatenate([oracle.eval_forces(s)[0] for s in singles])
+    Es = sum(oracle.eval_forces(s)[1] for s in singles)
+    assert np.abs(F - Fs).max() < 1e-12
+    assert np.abs(E - Es).max() < 1e-9
+
+
+def test_inter_part_is_brute_force_within_cutoff(oracle, params):
+    for rc, w in ((6.0, 0.0), (12.0, 0.0), (12.0, 2.0)):
+        mol, X, sysL, singles = _layer(oracle, params, 3, 2, 11.0, rc, w=w)
+        F, E = oracle.eval_forces(sysL)
+        Fs = np.concatenate([oracle.eval_forces(s)[0] for s in singles])
+        Es = sum(oracle.eval_forces(s)[1] for s in singles)
+        reqs_one = singles[0].reqs
+        Fb, Eb = _brute_inter(mol, X, reqs_one, rc, w)
+        scale = np.abs(Fb).max()
+        assert scale > 1e-3          # the molecules do interact
+        assert np.abs((F - Fs) - Fb).max() < 1e-9 * max(1.0, scale), rc
+        # only the pair term changes; every intermolecular pair is counted at both ends
+        assert np.abs(E[:4] - Es[:4]).max() < 1e-9 and abs(E[5] - Es[5]) < 1e-12
+        assert abs((E[4] - Es[4]) - Eb) < 1e-9, (E[4] - Es[4], Eb)
+        # Newton's third law for the intermolecular forces
+        assert np.abs(Fb.sum(axis=0)).max() < 1e-9
+
+
+def test_cutoff_changes_only_far_pairs(oracle, params):
+    mol, X, sysA, _ = _layer(oracle, params, 2, 2, 11.0, 40.0)
+    sysB = oracle.layer_from_molecules([mol] * len(X), X, params, 1e6, rdamp=RDAMP)
+    FA, EA = oracle.eval_forces(sysA)
+    FB, EB = oracle.eval_forces(sysB)
+    # 2 x 2 molecules at 11 A: every pair is within 40 A, so the cutoff is inactive
+    assert np.abs(FA - FB).max() < 1e-12 and np.abs(EA - EB).max() < 1e-12
+
+
+def test_cell_search_equals_brute_force(oracle, params):
+    """The oracle's O(N) cell-grid pair search (used for the CPU timing of
+    large layers) gives the brute-force loop's forces and energies."""
+    for rc in (6.0, 12.0):
+        mol, X, sysL, _ = _layer(oracle, params, 3, 3, 11.0, rc)
+        F, E = oracle.eval_forces(sysL)
+        sysL.layer_cells = True
+        Fc, Ec = oracle.eval_forces(sysL)
+        assert np.abs(F - Fc).max() < 1e-10 * max(1.0, np.abs(F).max())
+        assert np.abs(E - Ec).max() < 1e-9
+
+
+def test_switched_layer_energy_is_smooth(oracle, params):
+    """With the switch, forces are minus the gradient of the energy also for
+    pairs inside the switching shell (central differences, FP64); at the
+    cutoff the pair energy and force vanish continuously."""
+    mol, X, sysL, _ = _layer(oracle, params, 2, 2, 11.0, 9.0, w=3.0)
+    x0 = np.concatenate(X)
+    F, _ = oracle.nb_eval(sysL, x0)
+    h = 1e-5
+    rng = np.random.Generator(np.random.PCG64(3))
+    for a in rng.choice(len(x0), 12, replace=False):
+        for k in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[a, k] += h
+            xm[a, k] -= h
+            d = -(oracle.nb_eval(sysL, xp)[1] - oracle.nb_eval(sysL, xm)[1]) / (2 * h)
+            # the GPU-convention pair energy counts every pair twice
+            assert abs(d / 2 - F[a, k]) < 1e-5 * max(1.0, abs(F[a, k])), (a, k, d / 2, F[a, k])
+    # switch shape: S(rc - w) = 1, S(rc) = 0, S' = 0 at both ends
+    for x, S in ((0.0, 1.0), (1.0, 0.0)):
+        assert abs(1 - 3 * x * x + 2 * x ** 3 - S) < 1e-15 and abs(-6 * x + 6 * x * x) < 1e-15
