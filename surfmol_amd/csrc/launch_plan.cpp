@@ -31,7 +31,7 @@ LaunchShape make_shape(const smr_ctx *c, const ClassPlan &cp, int block, int apt
     if (block * apt < cp.max_atoms) return L;
     const void *k = relax_kernel_ptr(block, apt, fint_lds);
     if (!k) return L;
-    const size_t lds = relax_lds_bytes(block, apt, cp.max_atoms, cp.nf_max, cp.na_max, cp.nd_max, cp.ni_max, fint_lds, pair_mode);
+    const size_t lds = relax_lds_bytes(block, apt, relax_lds_atoms(block, cp.max_atoms), cp.nf_max, cp.na_max, cp.nd_max, cp.ni_max, fint_lds, pair_mode);
     if (lds > c->max_lds) return L;
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, block, lds) != hipSuccess || occ < 1) occ = 1;
@@ -55,7 +55,7 @@ static LaunchShape best_variant(const smr_ctx *c, const ClassPlan &cp, int block
     const size_t lds_cap = std::min<size_t>(c->max_lds, 64 * 1024);
     for (int pm = 1; pm >= 0; pm--) {
         if ((pairs == 0 && pm == 1) || (pairs == 1 && pm == 0)) continue;
-        const size_t with = relax_lds_bytes(block, apt, cp.max_atoms, cp.nf_max, cp.na_max, cp.nd_max, cp.ni_max, true, pm);
+        const size_t with = relax_lds_bytes(block, apt, relax_lds_atoms(block, cp.max_atoms), cp.nf_max, cp.na_max, cp.nd_max, cp.ni_max, true, pm);
         const bool fl = with <= lds_cap;
         LaunchShape L = make_shape(c, cp, block, apt, fl, pm, cache, have_grid);
         if (L.block) return L;
@@ -194,7 +194,7 @@ int prepare_plans(LaunchSet &ls, const std::vector<ClassPlan> &plans, const Kern
         A.work_counter = (int *)ls.counters.ptr + p.cls;
         A.fint_scratch = (float *)ls.scratch.ptr + off[q];
         A.nf_max = std::max(1, p.nf_max);
-        A.lds_atoms = p.max_atoms;
+        A.lds_atoms = relax_lds_atoms(p.shape.block, p.max_atoms);
         A.lds_na = p.na_max;
         A.lds_nd = p.nd_max;
         A.lds_ni = p.ni_max;

@@ -91,12 +91,59 @@ __device__ void layer_decide(const KernelArgs &A, double *s_red) {
     }
 }
 
+// Refresh the register stencils of the atoms of one wave that entered a new
+// cell (need = true; the wave calls this converged).  The changed atoms are
+// served nslot (<= 4) at a time: each of 16 lanes loads one 4-node z column of
+// an atom's 4x4x4 stencil, contracts it with that atom's PLQH and passes it
+// through the wave's LDS staging area (16*nslot float4), from which the
+// atom's own lane picks up its 16 values.  A wave then waits for one round of
+// grid loads per nslot changed atoms instead of one per x plane of every
+// changed atom; the values equal grid_contract_stencil's.
+__device__ __forceinline__ void refresh_stencils(const DevTables &T, bool need, const GridCell &gc, int key,
+                                                 float4 plq, float4 (&rs)[16], int &rkey, float4 *stage, int nslot,
+                                                 int lane) {
+    unsigned long long pend = __builtin_amdgcn_ballot_w64(need);
+    const int q = lane >> 4, col = lane & 15;
+    while (pend) {
+        // owner of slot q: the q-th pending lane, if any
+        const unsigned long long m1 = pend & (pend - 1), m2 = m1 & (m1 - 1), m3 = m2 & (m2 - 1);
+        const unsigned long long mq = q == 0 ? pend : q == 1 ? m1 : q == 2 ? m2 : m3;
+        const bool serve = q < nslot && mq != 0ull;
+        const int owner = serve ? __builtin_ctzll(mq) : lane;
+        const int oix = __shfl(gc.ix, owner, 64), oiy = __shfl(gc.iy, owner, 64), oiz = __shfl(gc.iz, owner, 64);
+        const float4 op = make_float4(__shfl(plq.x, owner, 64), __shfl(plq.y, owner, 64), __shfl(plq.z, owner, 64), 0.f);
+        if (serve) {
+            int xi = oix + (col >> 2); if (xi >= T.gnx) xi -= T.gnx;
+            int yi = oiy + (col & 3); if (yi >= T.gny) yi -= T.gny;
+            const float4 *c = T.grid + ((size_t)(xi * T.gny + yi) * T.gnz + oiz);
+            stage[q * 16 + col] = make_float4(dotplq(op, c[0]), dotplq(op, c[1]), dotplq(op, c[2]), dotplq(op, c[3]));
+        }
+        block_sync<1>();
+        const int nserved = min(nslot, __builtin_popcountll(pend));
+        const int rank = __builtin_popcountll(pend & ((1ull << lane) - 1ull));
+        if (need && rank < nserved) {
+#pragma unroll
+            for (int e = 0; e < 16; e++) rs[e] = stage[rank * 16 + e];
+            rkey = key;
+            need = false;
+        }
+        block_sync<1>();
+        for (int t = 0; t < nserved; t++) pend &= pend - 1ull;
+    }
+}
+
 // ===========================================================================
 // fused persistent kernel
 // ===========================================================================
+// One atom per thread: at most 256 registers, so that two waves share each
+// SIMD (the LDS of a <= 64-atom system admits two per SIMD as well); wider
+// shapes keep the compiler's choice (their per-thread atom arrays would
+// spill).
 template <int BLOCK, int APT, bool FINT_LDS>
-__global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
+__global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A) {
     constexpr int NW = BLOCK / 64;
+    // one atom per thread: each atom's surface stencil lives in registers
+    constexpr bool RSTEN = (APT == 1);
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
@@ -198,6 +245,33 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
             const int i = tid + k * BLOCK;
             skey[k] = (A.stencil_key && i < N) ? A.stencil_key[abase + i] : -1;
         }
+        // one atom per thread: the contracted stencil is held in registers
+        // for the whole launch (rkey = its cell, -1: none); the stored copy
+        // is read once here and written back when the system leaves
+        float4 rs[RSTEN ? 16 : 1];
+        int rkey = -1;
+        if constexpr (RSTEN) {
+            if (A.stencil && tid < N && skey[0] != -1) {
+                rkey = skey[0];
+                if (rkey >= 0) {
+                    const float4 *sc = A.stencil + (size_t)abase * 16 + tid;
+#pragma unroll
+                    for (int e = 0; e < 16; e++) rs[e] = sc[e * N];
+                }
+            }
+        }
+        auto persist_stencil = [&]() {
+            if constexpr (RSTEN) {
+                if (A.stencil && tid < N && rkey != skey[0]) {
+                    if (rkey >= 0) {
+                        float4 *sc = A.stencil + (size_t)abase * 16 + tid;
+#pragma unroll
+                        for (int e = 0; e < 16; e++) sc[e * N] = rs[e];
+                    }
+                    skey[0] = rkey;
+                }
+            }
+        };
         const int nsteps = A.steps_per_launch;
         double fmax_cur = st.fmax;
         int step_in_launch = 0;
@@ -318,6 +392,24 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                 block_sync<NW>();
             }
             // ---------------- phase C: gather pieces + pairs + surface ----------------
+            // surface force of the own atom from its register stencil (one
+            // atom per thread; the whole wave takes part in the refresh)
+            float4 fe_r = make_float4(0.f, 0.f, 0.f, 0.f);
+            if constexpr (RSTEN) {
+                if (T.grid) {
+                    const bool own = tid < N;
+                    const float3 pi = own ? f3(s_pos[tid]) : make_float3(0.f, 0.f, 0.f);
+                    const float3 ul = make_float3(pi.x * T.ginv_x, pi.y * T.ginv_y, pi.z * T.ginv_z);
+                    const GridCell gc = grid_cell(T.gnx, T.gny, T.gnz, sd.iux, sd.iuy, ul);
+                    const int key = gc.iz < 0 ? -2 : (gc.ix * T.gny + gc.iy) * T.gnz + gc.iz;
+                    if (own && key != rkey && gc.iz < 0) rkey = key;   // outside the grid: zero surface force
+                    const bool need = own && key != rkey;
+                    // staging: 4 slots per wave in the hneigh area (the host
+                    // reserves >= 16 atoms per wave), idle until the next bonds
+                    refresh_stencils(T, need, gc, key, my_plq[0], rs, rkey, s_h + wid * 64, 4, lane);
+                    if (own) fe_r = grid_eval_stencil([&](int e) { return rs[e]; }, gc);
+                }
+            }
             float3 F[APT];
 #pragma unroll
             for (int k = 0; k < APT; k++) {
@@ -374,7 +466,9 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     if (T.grid) {
                         float3 ul = make_float3(pi.x * T.ginv_x, pi.y * T.ginv_y, pi.z * T.ginv_z);
                         float4 fe;
-                        if (A.stencil) {
+                        if constexpr (RSTEN) {
+                            fe = fe_r;
+                        } else if (A.stencil) {
                             GridCell gc = grid_cell(T.gnx, T.gny, T.gnz, sd.iux, sd.iuy, ul);
                             const int key = gc.iz < 0 ? -2 : (gc.ix * T.gny + gc.iy) * T.gnz + gc.iz;
                             // the system's block of 16*N float4, element-major
@@ -399,6 +493,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     const int i = tid + k * BLOCK;
                     if (i < N) {
                         A.force[abase + i] = make_float4(F[k].x, F[k].y, F[k].z, 0.f);
+                        if (k == 0) persist_stencil();
                         if (A.stencil_key) A.stencil_key[abase + i] = skey[k];
                     }
                 }
@@ -477,6 +572,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                     if (i < N) {
                         A.force[abase + i] = make_float4(F[k].x, F[k].y, F[k].z, 0.f);
                         A.vel[abase + i] = make_float4(my_v[k].x, my_v[k].y, my_v[k].z, 0.f);
+                        if (k == 0) persist_stencil();
                         if (A.stencil_key) A.stencil_key[abase + i] = skey[k];
                     }
                 }
@@ -592,6 +688,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(KernelArgs A) {
                 for (int k = 0; k < APT; k++) {
                     const int i = tid + k * BLOCK;
                     if (i < N) {
+                        if (k == 0) persist_stencil();
                         if (A.stencil_key) A.stencil_key[abase + i] = skey[k];
                         A.pos[abase + i] = s_pos[i];
                         A.vel[abase + i] = make_float4(my_v[k].x, my_v[k].y, my_v[k].z, 0.f);

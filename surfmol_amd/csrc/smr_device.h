@@ -286,7 +286,7 @@ __device__ __forceinline__ float4 grid_sample(const float4 *__restrict__ G, int 
     float by_[4] = {by.x, by.y, by.z, by.w}, dy_[4] = {dy.x, dy.y, dy.z, dy.w};
     float bx_[4] = {bx.x, bx.y, bx.z, bx.w}, dx_[4] = {dx.x, dx.y, dx.z, dx.w};
     float ox = 0.f, oy = 0.f, oz = 0.f, oe = 0.f;
-#pragma unroll
+#pragma unroll 1
     for (int a = 0; a < 4; a++) {
         int xi = ix + a; if (xi >= nx) xi -= nx;
         float e_y = 0.f, e_dy = 0.f, e_dz = 0.f;
@@ -346,7 +346,7 @@ __device__ __forceinline__ GridCell grid_cell(int nx, int ny, int nz, int ix0, i
 // system contiguous, so one load instruction of a wave reads whole lines.
 __device__ __forceinline__ void grid_store_stencil(const float4 *__restrict__ G, int nx, int ny, int nz,
                                                    const GridCell &g, float4 plq, float4 *sc, int ld) {
-#pragma unroll
+#pragma unroll 1
     for (int a = 0; a < 4; a++) {
         int xi = g.ix + a; if (xi >= nx) xi -= nx;
 #pragma unroll
@@ -359,9 +359,31 @@ __device__ __forceinline__ void grid_store_stencil(const float4 *__restrict__ G,
     }
 }
 
-// Interpolate inside cell g from a stored contracted stencil (the
-// contraction order of grid_sample: z, then y, then x).  Returns (dE/du, E).
-__device__ __forceinline__ float4 grid_eval_cached(const float4 *__restrict__ sc, int ld, const GridCell &g) {
+// Contract the 4x4x4 stencil of cell g with the atom's PLQH into registers
+// (the same values grid_store_stencil writes).  The scheduling barrier after
+// each x plane keeps at most 16 node loads in flight, so the contraction does
+// not raise the kernel's register count.
+__device__ __forceinline__ void grid_contract_stencil(const float4 *__restrict__ G, int nx, int ny, int nz,
+                                                      const GridCell &g, float4 plq, float4 (&rs)[16]) {
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        int xi = g.ix + a; if (xi >= nx) xi -= nx;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            int yi = g.iy + b; if (yi >= ny) yi -= ny;
+            const float4 *col = G + ((size_t)(xi * ny + yi) * nz + g.iz);
+            rs[4 * a + b] = make_float4(dotplq(plq, col[0]), dotplq(plq, col[1]), dotplq(plq, col[2]),
+                                        dotplq(plq, col[3]));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Interpolate inside cell g from a contracted stencil, element (4a+b) given
+// by get(4a+b) (the contraction order of grid_sample: z, then y, then x).
+// Returns (dE/du, E).
+template <class Get>
+__device__ __forceinline__ float4 grid_eval_stencil(Get get, const GridCell &g) {
     if (g.iz < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
     float4 bz = bsp_basis(g.tz), dz = bsp_dbasis(g.tz);
     float4 by = bsp_basis(g.ty), dy = bsp_dbasis(g.ty);
@@ -374,7 +396,7 @@ __device__ __forceinline__ float4 grid_eval_cached(const float4 *__restrict__ sc
         float e_y = 0.f, e_dy = 0.f, e_dz = 0.f;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
-            const float4 cs = sc[(4 * a + b) * ld];
+            const float4 cs = get(4 * a + b);
             float fe = dot4(bz, cs);
             float fd = dot4(dz, cs);
             e_dy += fe * dy_[b];
@@ -387,6 +409,11 @@ __device__ __forceinline__ float4 grid_eval_cached(const float4 *__restrict__ sc
         oe += bx_[a] * e_y;
     }
     return make_float4(ox, oy, oz, oe);
+}
+
+// the same from a stored stencil sc[(4a+b)*ld]
+__device__ __forceinline__ float4 grid_eval_cached(const float4 *__restrict__ sc, int ld, const GridCell &g) {
+    return grid_eval_stencil([&](int e) { return sc[e * ld]; }, g);
 }
 
 // ---------------------------------------------------------------------------

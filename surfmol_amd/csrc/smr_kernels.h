@@ -54,7 +54,7 @@ struct KernelArgs {
     float *fint_scratch;     // [grid * 3 * nf_max] when pieces do not fit LDS
     float4 *stencil;         // [total atoms * 16] stored surface stencils (NULL: sample the grid directly)
     int nf_max;
-    int lds_atoms;           // atoms reserved per LDS slot
+    int lds_atoms;           // atoms reserved per LDS slot (relax_lds_atoms: >= 16 per wave)
     int lds_na, lds_nd, lds_ni;  // interaction slots reserved per LDS slot
     int steps_per_launch;
     int mode;
@@ -72,6 +72,11 @@ struct LaunchShape {
     bool pair_mode = true;        // unordered-pair sweep (needs a reaction buffer per wave)
     size_t lds_bytes = 0;
 };
+
+// atoms reserved per LDS slot: at least 16 per wave, so that the hneigh area
+// (4 float4 per atom) also holds each wave's surface-stencil staging slots
+// (4 atoms x 16 float4, used while hneigh is idle)
+inline int relax_lds_atoms(int block, int max_atoms) { return max_atoms > 16 * (block / 64) ? max_atoms : 16 * (block / 64); }
 
 // bytes of dynamic LDS for a launch shape: positions, REQ rows, hneigh,
 // pair reaction forces (one buffer per wave in the unordered-pair sweep),
