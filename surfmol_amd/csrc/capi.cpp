@@ -380,6 +380,22 @@ int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, H
         for (int k = 0; k < 3; k++) { int a = t.ang_atoms[3 * a3 + k]; idx[off[a] + cnt[a]++] = (unsigned short)(i0ang + k * t.nangles + a3); }
     for (int i = 0; i < N; i++) P.a2f.push_back(make_int2(off[i], off[i + 1] - off[i]));
     P.a2f_idx.insert(P.a2f_idx.end(), idx.begin(), idx.end());
+    // the inverse: the atom-major position of every piece, per term
+    std::vector<unsigned> dst(td.nf, 0);
+    for (int q = 0; q < off[N]; q++) dst[idx[q]] = (unsigned)q;
+    auto pk = [&](int p0, int p1) { return dst[p0] | (dst[p1] << 16); };
+    for (int d = 0; d < t.ndih; d++) {
+        const int n = t.ndih;
+        P.dih_dst.push_back(make_uint2(pk(d, n + d), pk(2 * n + d, 3 * n + d)));
+    }
+    for (int v = 0; v < t.ninv; v++) {
+        const int n = t.ninv;
+        P.inv_dst.push_back(make_uint2(pk(i0inv + v, i0inv + n + v), pk(i0inv + 2 * n + v, i0inv + 3 * n + v)));
+    }
+    for (int a3 = 0; a3 < t.nangles; a3++) {
+        const int n = t.nangles;
+        P.ang_dst.push_back(make_uint2(pk(i0ang + a3, i0ang + n + a3), dst[i0ang + 2 * n + a3]));
+    }
     return SMR_OK;
 }
 
@@ -387,7 +403,7 @@ hipError_t upload_pools(HostPools &P, DevPools &D, hipStream_t st) {
     hipError_t e;
 #define UP(name) if ((e = D.name.upload(P.name.data(), P.name.size() * sizeof(P.name[0]), st)) != hipSuccess) return e;
     UP(topo) UP(sys) UP(ng) UP(bk) UP(bl) UP(req) UP(plq) UP(a2f) UP(ang_ng) UP(ang_par) UP(dih_ng) UP(dih_par)
-    UP(inv_ng) UP(inv_par) UP(a2f_idx) UP(bk64) UP(bl64) UP(req64) UP(plq64) UP(ang64) UP(dih64) UP(inv64)
+    UP(inv_ng) UP(inv_par) UP(a2f_idx) UP(ang_dst) UP(dih_dst) UP(inv_dst) UP(bk64) UP(bl64) UP(req64) UP(plq64) UP(ang64) UP(dih64) UP(inv64)
 #undef UP
     return hipSuccess;
 }
@@ -408,6 +424,9 @@ void fill_tables(const DevPools &D, const smr_gridff *grid, const smr_ff_options
     T.inv_ng = (const uint2 *)D.inv_ng.ptr;
     T.inv_par = (const float4 *)D.inv_par.ptr;
     T.a2f_idx = (const unsigned short *)D.a2f_idx.ptr;
+    T.ang_dst = (const uint2 *)D.ang_dst.ptr;
+    T.dih_dst = (const uint2 *)D.dih_dst.ptr;
+    T.inv_dst = (const uint2 *)D.inv_dst.ptr;
     T.bk64 = (const double4 *)D.bk64.ptr;
     T.bl64 = (const double4 *)D.bl64.ptr;
     T.req64 = (const double4 *)D.req64.ptr;

@@ -91,6 +91,16 @@ __device__ void layer_decide(const KernelArgs &A, double *s_red) {
     }
 }
 
+// Value of lane (lane+1) mod (last+1) for lanes 0..last: the first last+1
+// lanes rotated by one towards lane 0 (DPP wave_shl:1, then lane `last`
+// takes lane 0's value).  Lanes above `last` receive garbage.
+__device__ __forceinline__ float wave_rot1(float x, int last, int lane) {
+    const int xi = __float_as_int(x);
+    const int y = __builtin_amdgcn_update_dpp(0, xi, 0x130, 0xf, 0xf, true);
+    const int x0 = __builtin_amdgcn_readlane(xi, 0);
+    return __int_as_float(lane == last ? x0 : y);
+}
+
 // Refresh the register stencils of the atoms of one wave that entered a new
 // cell (need = true; the wave calls this converged).  The changed atoms are
 // served nslot (<= 4) at a time: each of 16 lanes loads one 4-node z column of
@@ -144,6 +154,9 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
     constexpr int NW = BLOCK / 64;
     // one atom per thread: each atom's surface stencil lives in registers
     constexpr bool RSTEN = (APT == 1);
+    // one wave, one atom per thread: the pair sweep carries reaction forces
+    // in registers that move one lane per sweep
+    constexpr bool ROT = (BLOCK == 64 && APT == 1);
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
@@ -157,19 +170,20 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
     const int NWP = A.pair_mode ? NW : 1;                    // one reaction buffer per wave
     double *s_red = reinterpret_cast<double *>(s_fp + NWP * A.lds_atoms);  // [NW*8]
     int *s_next = reinterpret_cast<int *>(s_red + NW * 8);
-    // interaction slots and the piece-gather list: staged in LDS, except for
+    // interaction slots and piece destinations: staged in LDS, except for
     // the widest shapes (4 atoms per thread, up to 1024 atoms), whose tables
     // are read through the caches instead so that the atom data fits
     constexpr bool TABLES_LDS = (APT < 4);
     const int tn_d = TABLES_LDS ? A.lds_nd : 0, tn_i = TABLES_LDS ? A.lds_ni : 0, tn_a = TABLES_LDS ? A.lds_na : 0;
     uint2 *s_dih = reinterpret_cast<uint2 *>(s_next + 4);                // packed slots
     uint2 *s_inv = s_dih + tn_d;
-    unsigned *s_ang = reinterpret_cast<unsigned *>(s_inv + tn_i);
+    uint2 *s_dihd = s_inv + tn_i;                                       // piece destinations
+    uint2 *s_invd = s_dihd + tn_d;
+    uint2 *s_angd = s_invd + tn_i;
+    unsigned *s_ang = reinterpret_cast<unsigned *>(s_angd + tn_a);
     float *s_fx = reinterpret_cast<float *>(s_ang + tn_a);
-    unsigned short *s_a2f = reinterpret_cast<unsigned short *>(s_fx + (FINT_LDS ? 3 * A.nf_max : 0));
-    const uint2 *t_dih, *t_inv;
+    const uint2 *t_dih, *t_inv, *t_dihd, *t_invd, *t_angd;
     const unsigned *t_ang;
-    const unsigned short *t_a2f;
     float *fint_x, *fint_y, *fint_z;
 
     // a coupled layer that has converged (or stopped) does no more work
@@ -223,16 +237,24 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                 my_v[k] = make_float3(0.f, 0.f, 0.f);
             }
         }
-        // interaction slots and the piece-gather list stay in LDS for all steps
+        // interaction slots and piece destinations stay in LDS for all steps
         if constexpr (TABLES_LDS) {
-            for (int q = tid; q < td.nangles; q += BLOCK) s_ang[q] = T.ang_ng[td.ang_off + q];
-            for (int q = tid; q < td.ndih; q += BLOCK) s_dih[q] = T.dih_ng[td.dih_off + q];
-            for (int q = tid; q < td.ninv; q += BLOCK) s_inv[q] = T.inv_ng[td.inv_off + q];
-            for (int q = tid; q < td.nf; q += BLOCK) s_a2f[q] = T.a2f_idx[td.a2f_off + q];
-            t_ang = s_ang; t_dih = s_dih; t_inv = s_inv; t_a2f = s_a2f;
+            for (int q = tid; q < td.nangles; q += BLOCK) {
+                s_ang[q] = T.ang_ng[td.ang_off + q];
+                s_angd[q] = T.ang_dst[td.ang_off + q];
+            }
+            for (int q = tid; q < td.ndih; q += BLOCK) {
+                s_dih[q] = T.dih_ng[td.dih_off + q];
+                s_dihd[q] = T.dih_dst[td.dih_off + q];
+            }
+            for (int q = tid; q < td.ninv; q += BLOCK) {
+                s_inv[q] = T.inv_ng[td.inv_off + q];
+                s_invd[q] = T.inv_dst[td.inv_off + q];
+            }
+            t_ang = s_ang; t_dih = s_dih; t_inv = s_inv; t_angd = s_angd; t_dihd = s_dihd; t_invd = s_invd;
         } else {
-            t_ang = T.ang_ng + td.ang_off; t_dih = T.dih_ng + td.dih_off;
-            t_inv = T.inv_ng + td.inv_off; t_a2f = T.a2f_idx + td.a2f_off;
+            t_ang = T.ang_ng + td.ang_off; t_dih = T.dih_ng + td.dih_off; t_inv = T.inv_ng + td.inv_off;
+            t_angd = T.ang_dst + td.ang_off; t_dihd = T.dih_dst + td.dih_off; t_invd = T.inv_dst + td.inv_off;
         }
         block_sync<NW>();
 
@@ -277,7 +299,7 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
         int step_in_launch = 0;
         for (;;) {
             // ---------------- phase A: bonds + hneigh ----------------
-            if (A.pair_mode)
+            if (!ROT && A.pair_mode)
                 for (int w = 0; w < NWP; w++)
                     for (int q = tid; q < N; q += BLOCK) s_fp[w * A.lds_atoms + q] = make_float4(0.f, 0.f, 0.f, 0.f);
             float3 fb[APT];
@@ -304,38 +326,32 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
             }
             block_sync<NW>();
             // ---------------- phase B: angles, dihedrals, inversions ----------------
+            // each force piece goes straight to its atom-major position (the
+            // order in which the atom's gather adds it)
             {
-                const int i0inv = 4 * td.ndih, i0ang = i0inv + 4 * td.ninv;
+                auto put = [&](unsigned p, float3 f) { fint_x[p] = f.x; fint_y[p] = f.y; fint_z[p] = f.z; };
                 for (int a = tid; a < td.nangles; a += BLOCK) {
                     const unsigned g = t_ang[a];
+                    const uint2 dd = t_angd[a];
                     float3 fpi, fpj, fpk;
                     (void)angle_term(s_h[g & 0xffffu], s_h[g >> 16], T.ang_par[td.ang_off + a], fpi, fpj, fpk);
-                    const int p = i0ang + a, q = td.nangles;
-                    fint_x[p] = fpi.x; fint_y[p] = fpi.y; fint_z[p] = fpi.z;
-                    fint_x[p + q] = fpj.x; fint_y[p + q] = fpj.y; fint_z[p + q] = fpj.z;
-                    fint_x[p + 2 * q] = fpk.x; fint_y[p + 2 * q] = fpk.y; fint_z[p + 2 * q] = fpk.z;
+                    put(dd.x & 0xffffu, fpi); put(dd.x >> 16, fpj); put(dd.y & 0xffffu, fpk);
                 }
                 for (int d = tid; d < td.ndih; d += BLOCK) {
                     const uint2 g = t_dih[d];
+                    const uint2 dd = t_dihd[d];
                     float3 f1, f2, f3v, f4;
                     (void)dihedral_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y & 0xffffu], T.dih_par[td.dih_off + d],
                                             (int)(g.y >> 16), f1, f2, f3v, f4);
-                    const int p = d, q = td.ndih;
-                    fint_x[p] = f1.x; fint_y[p] = f1.y; fint_z[p] = f1.z;
-                    fint_x[p + q] = f2.x; fint_y[p + q] = f2.y; fint_z[p + q] = f2.z;
-                    fint_x[p + 2 * q] = f3v.x; fint_y[p + 2 * q] = f3v.y; fint_z[p + 2 * q] = f3v.z;
-                    fint_x[p + 3 * q] = f4.x; fint_y[p + 3 * q] = f4.y; fint_z[p + 3 * q] = f4.z;
+                    put(dd.x & 0xffffu, f1); put(dd.x >> 16, f2); put(dd.y & 0xffffu, f3v); put(dd.y >> 16, f4);
                 }
                 for (int v = tid; v < td.ninv; v += BLOCK) {
                     const uint2 g = t_inv[v];
+                    const uint2 dd = t_invd[v];
                     float3 f1, f2, f3v, f4;
                     (void)inversion_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y], T.inv_par[td.inv_off + v], f1, f2,
                                              f3v, f4);
-                    const int p = i0inv + v, q = td.ninv;
-                    fint_x[p] = f1.x; fint_y[p] = f1.y; fint_z[p] = f1.z;
-                    fint_x[p + q] = f2.x; fint_y[p + q] = f2.y; fint_z[p + q] = f2.z;
-                    fint_x[p + 2 * q] = f3v.x; fint_y[p + 2 * q] = f3v.y; fint_z[p + 2 * q] = f3v.z;
-                    fint_x[p + 3 * q] = f4.x; fint_y[p + 3 * q] = f4.y; fint_z[p + 3 * q] = f4.z;
+                    put(dd.x & 0xffffu, f1); put(dd.x >> 16, f2); put(dd.y & 0xffffu, f3v); put(dd.y >> 16, f4);
                 }
             }
             block_sync<NW>();
@@ -349,7 +365,53 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
             float3 pf[APT];
 #pragma unroll
             for (int k = 0; k < APT; k++) pf[k] = make_float3(0.f, 0.f, 0.f);
-            if (A.pair_mode && T.use_pairs) {
+            float3 rr = make_float3(0.f, 0.f, 0.f);   // ROT: reaction forces on the own atom
+            if (ROT && A.pair_mode && T.use_pairs) {
+                // The same sweep with the reaction sum of atom j = (i+d) mod N
+                // held by lane i: on sweep d+1 atom (i+d+1) mod N is the
+                // target of lane i+1 on sweep d, so the accumulators rotate
+                // by one lane per sweep (each atom's reaction terms are added
+                // in sweep order, as in the LDS variant).  Afterwards lane
+                // (i - nd) mod N holds atom i's sum.
+                const bool own = tid < N;
+                const int nd = N >> 1;
+                const bool even = (N & 1) == 0;
+                const float4 pi4 = own ? s_pos[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 ri = own ? s_req[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const int4 ng = my_ng[0];
+                KahanF3 pacc;
+                pacc.init(make_float3(0.f, 0.f, 0.f));
+                int j = own ? tid : 0;
+                int jn = (j + 1 == N) ? 0 : j + 1;
+                float4 pj = s_pos[jn], rj = s_req[jn];
+                for (int d = 1; d <= nd; d++) {
+                    j = jn;
+                    const float4 pc = pj, rc = rj;
+                    jn = (j + 1 == N) ? 0 : j + 1;
+                    if (d < nd) {
+                        pj = s_pos[jn];
+                        rj = s_req[jn];
+                    }
+                    if (d > 1) {
+                        rr.x = wave_rot1(rr.x, N - 1, lane);
+                        rr.y = wave_rot1(rr.y, N - 1, lane);
+                        rr.z = wave_rot1(rr.z, N - 1, lane);
+                    }
+                    const bool active = own && !(even && d == nd && tid >= nd);
+                    if (active && !(j == ng.x || j == ng.y || j == ng.z || j == ng.w)) {
+                        float3 f;
+                        (void)ljq_pair(f3(pc) - f3(pi4), ri, rc, T.r2damp, f);
+                        pacc.add(f);
+                        rr.x -= f.x; rr.y -= f.y; rr.z -= f.z;
+                    }
+                }
+                int src = tid - nd;
+                if (src < 0) src += N;
+                rr.x = __shfl(rr.x, src, 64);
+                rr.y = __shfl(rr.y, src, 64);
+                rr.z = __shfl(rr.z, src, 64);
+                pf[0] = pacc.value();
+            } else if (A.pair_mode && T.use_pairs) {
                 float4 *fpw = s_fp + wid * A.lds_atoms;
                 const int nd = N >> 1;
                 const bool even = (N & 1) == 0;
@@ -420,22 +482,25 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                     acc.init(fb[k]);
                     if (A.pair_mode && T.use_pairs) {
                         acc.add(pf[k]);
-                        for (int w = 0; w < NW; w++) acc.add(f3(s_fp[w * A.lds_atoms + i]));
+                        if constexpr (ROT) {
+                            acc.add(rr);
+                        } else {
+                            for (int w = 0; w < NW; w++) acc.add(f3(s_fp[w * A.lds_atoms + i]));
+                        }
                     }
-                    // pieces four at a time: the index and piece loads of a
-                    // group are independent, so their LDS latencies overlap;
-                    // the adds keep the list order
-                    const unsigned short *idx = t_a2f + my_a2f[k].x;
-                    const int cnt = my_a2f[k].y;
-                    for (int q = 0; q < cnt; q += 4) {
-                        int pp[4];
+                    // the atom's pieces are contiguous (atom-major order); eight
+                    // at a time, so that their LDS latencies overlap; the
+                    // adds keep the list order
+                    const int pb = my_a2f[k].x, cnt = my_a2f[k].y;
+                    for (int q = 0; q < cnt; q += 8) {
+                        float3 pv[8];
 #pragma unroll
-                        for (int u = 0; u < 4; u++) pp[u] = idx[min(q + u, cnt - 1)];
-                        float3 pv[4];
+                        for (int u = 0; u < 8; u++) {
+                            const int pp = pb + min(q + u, cnt - 1);
+                            pv[u] = make_float3(fint_x[pp], fint_y[pp], fint_z[pp]);
+                        }
 #pragma unroll
-                        for (int u = 0; u < 4; u++) pv[u] = make_float3(fint_x[pp[u]], fint_y[pp[u]], fint_z[pp[u]]);
-#pragma unroll
-                        for (int u = 0; u < 4; u++)
+                        for (int u = 0; u < 8; u++)
                             if (q + u < cnt) acc.add(pv[u]);
                     }
                     const float4 pi4 = s_pos[i];

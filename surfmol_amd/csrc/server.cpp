@@ -109,7 +109,7 @@ struct Request {
 constexpr size_t kAngPerAtom = 3, kDihPerAtom = 6, kInvPerAtom = 3;
 constexpr size_t kPiecesPerAtom = 4 * kDihPerAtom + 4 * kInvPerAtom + 3 * kAngPerAtom;
 constexpr size_t kBytesPerAtom = 3 * 16 + 5 * 16 + 8 + 4 * 32 + 256 + 4;   // state, tables, FP64 tables, stencil
-constexpr size_t kBytesPerAng = 4 + 16 + 32, kBytesPerDih = 8 + 16 + 32, kBytesPerInv = 8 + 16 + 32;
+constexpr size_t kBytesPerAng = 4 + 8 + 16 + 32, kBytesPerDih = 8 + 8 + 16 + 32, kBytesPerInv = 8 + 8 + 16 + 32;
 constexpr size_t kUnitBytes = kBytesPerAtom + kAngPerAtom * kBytesPerAng + kDihPerAtom * kBytesPerDih +
                               kInvPerAtom * kBytesPerInv + kPiecesPerAtom * 2;
 
@@ -125,6 +125,7 @@ struct smr_server {
     FireConst fc;
     DevBuf pos, vel, force, stencil, skey, ng, bk, bl, req, plq, a2f, bk64, bl64, req64, plq64;
     DevBuf ang_ng, ang_par, ang64, dih_ng, dih_par, dih64, inv_ng, inv_par, inv64, a2f_idx;
+    DevBuf ang_dst, dih_dst, inv_dst;
     DevBuf topo, sys, state, work, list, dstage, dout;
     RangeAlloc ra_atoms, ra_ang, ra_dih, ra_inv, ra_pc;
     size_t arena_bytes = 0;
@@ -296,6 +297,9 @@ int upload(smr_server *S, const std::vector<Request *> &admit) {
         g.add(h.inv_par.data(), td.ninv * sizeof(float4), at<float4>(S->inv_par, r->i_off));
         g.add(h.inv64.data(), td.ninv * sizeof(double4), at<double4>(S->inv64, r->i_off));
         g.add(h.a2f_idx.data(), td.nf * sizeof(unsigned short), at<unsigned short>(S->a2f_idx, r->f_off));
+        g.add(h.ang_dst.data(), td.nangles * sizeof(uint2), at<uint2>(S->ang_dst, r->an_off));
+        g.add(h.dih_dst.data(), td.ndih * sizeof(uint2), at<uint2>(S->dih_dst, r->d_off));
+        g.add(h.inv_dst.data(), td.ninv * sizeof(uint2), at<uint2>(S->inv_dst, r->i_off));
         g.add(&td, sizeof(TopoDesc), at<TopoDesc>(S->topo, r->slot));
         g.add(&sd, sizeof(SysDesc), at<SysDesc>(S->sys, r->slot));
         g.add(&st, sizeof(SysState), at<SysState>(S->state, r->slot));
@@ -537,6 +541,7 @@ extern "C" int smr_server_create(smr_ctx *c, const smr_gridff *grid, const smr_s
     ALLOC(dih_ng, cd * sizeof(uint2)) ALLOC(dih_par, cd * sizeof(float4)) ALLOC(dih64, cd * sizeof(double4))
     ALLOC(inv_ng, ci * sizeof(uint2)) ALLOC(inv_par, ci * sizeof(float4)) ALLOC(inv64, ci * sizeof(double4))
     ALLOC(a2f_idx, cf * sizeof(unsigned short))
+    ALLOC(ang_dst, cn * sizeof(uint2)) ALLOC(dih_dst, cd * sizeof(uint2)) ALLOC(inv_dst, ci * sizeof(uint2))
     ALLOC(topo, nr * sizeof(TopoDesc)) ALLOC(sys, nr * sizeof(SysDesc)) ALLOC(state, nr * sizeof(SysState))
     ALLOC(work, nr * sizeof(int)) ALLOC(list, nr * sizeof(int))
 #undef ALLOC
@@ -573,6 +578,9 @@ extern "C" int smr_server_create(smr_ctx *c, const smr_gridff *grid, const smr_s
     T.inv_ng = (const uint2 *)S->inv_ng.ptr;
     T.inv_par = (const float4 *)S->inv_par.ptr;
     T.a2f_idx = (const unsigned short *)S->a2f_idx.ptr;
+    T.ang_dst = (const uint2 *)S->ang_dst.ptr;
+    T.dih_dst = (const uint2 *)S->dih_dst.ptr;
+    T.inv_dst = (const uint2 *)S->inv_dst.ptr;
     T.bk64 = (const double4 *)S->bk64.ptr;
     T.bl64 = (const double4 *)S->bl64.ptr;
     T.req64 = (const double4 *)S->req64.ptr;

@@ -68,6 +68,10 @@ struct DevTables {
     const uint2 *inv_ng;         // (s21 | s31 << 16, s41)
     const float4 *inv_par;       // K*(C0, C1, C2), 0
     const unsigned short *a2f_idx;  // piece indices (local to the topology's fint layout)
+    // where each force piece of a term goes in the atom-major piece order
+    // (the a2f lists inverted; 16-bit local positions, 4 per uint2, piece k
+    // in bits 16*(k&1) of component k>>1)
+    const uint2 *ang_dst, *dih_dst, *inv_dst;
     // GridFF
     const float4 *grid;    // [nx*ny*nz] z-fastest (P, L, Q, 0); NULL = no surface
     int gnx, gny, gnz;

@@ -43,12 +43,14 @@ struct HostPools {
     std::vector<uint2> inv_ng;
     std::vector<float4> inv_par;
     std::vector<unsigned short> a2f_idx;
+    std::vector<uint2> ang_dst, dih_dst, inv_dst;
     // FP64 parameters for the energy evaluation
     std::vector<double4> bk64, bl64, req64, plq64, ang64, dih64, inv64;
 };
 
 struct DevPools {
     DevBuf topo, sys, ng, bk, bl, req, plq, a2f, ang_ng, ang_par, dih_ng, dih_par, inv_ng, inv_par, a2f_idx;
+    DevBuf ang_dst, dih_dst, inv_dst;
     DevBuf bk64, bl64, req64, plq64, ang64, dih64, inv64;
 };
 

@@ -80,17 +80,17 @@ inline int relax_lds_atoms(int block, int max_atoms) { return max_atoms > 16 * (
 
 // bytes of dynamic LDS for a launch shape: positions, REQ rows, hneigh,
 // pair reaction forces (one buffer per wave in the unordered-pair sweep),
-// reduction scratch, packed interaction slots and the 16-bit piece-gather
-// list (not for 4 atoms per thread: those tables stay in global memory), and
-// the force pieces (optional)
+// reduction scratch, packed interaction slots and piece destinations (not
+// for 4 atoms per thread: those tables stay in global memory), and the force
+// pieces (optional)
 inline size_t relax_lds_bytes(int block, int apt, int lds_atoms, int nf_max, int na, int nd, int ni, bool fint_lds,
                               bool pair_mode) {
     const int nw = block / 64;
     size_t b = (size_t)lds_atoms * 16 * (6 + (pair_mode ? nw : 1)) + (size_t)nw * 8 * 8 + 16;
     const bool tables = apt < 4;
-    if (tables) b += (size_t)nd * 8 + (size_t)ni * 8 + (size_t)na * 4;
+    // slots (8, 8, 4 B) and piece destinations (8 B) per dihedral, inversion, angle
+    if (tables) b += (size_t)nd * 16 + (size_t)ni * 16 + (size_t)na * 12;
     if (fint_lds) b += (size_t)nf_max * 12;
-    if (tables) b += ((size_t)nf_max * 2 + 15) & ~(size_t)15;
     return b;
 }
 

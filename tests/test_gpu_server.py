@@ -74,7 +74,14 @@ def test_server_matches_batch_and_fp64(ctx, surf, oracle, params):
         osys = oracle_system(oracle, params, mols[s], X[s], surf["C"], surf["spec"])
         fo, eo = oracle.eval_forces(osys, r["positions"])
         assert abs(eo.sum() - r["energy"]) / mols[s].natoms < 1e-6
-        assert np.linalg.norm(fo, axis=1).max() < 2e-3
+        # convergence is tested on the forces before the last move (the
+        # reference's order), so the returned positions carry a slightly
+        # larger Fmax; the pin is that they are an FP64 minimum: FP64 FIRE
+        # started there converges without moving the energy by 1e-6 eV/atom
+        assert np.linalg.norm(fo, axis=1).max() < 1e-2
+        rf = oracle.relax_fire(osys, oracle.FireParams(**fire_params_internal()), 3000, 1e-3, apos=r["positions"])
+        assert rf["converged"]
+        assert abs(rf["E"] - r["energy"]) / mols[s].natoms < 1e-6, (s, rf["E"], r["energy"])
         assert r["latency"] > 0
 
 
