@@ -373,12 +373,27 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                 // by one lane per sweep (each atom's reaction terms are added
                 // in sweep order, as in the LDS variant).  Afterwards lane
                 // (i - nd) mod N holds atom i's sum.
+                // The loop body is branch-free: the pair force is computed in
+                // every lane and zeroed where the lane has no atom, the pair
+                // is bonded (bit d of `skip`: the 1-2 exclusions of the own
+                // atom's sweep) or, on the last sweep of an even N, the pair
+                // belongs to the lower half.
                 const bool own = tid < N;
                 const int nd = N >> 1;
                 const bool even = (N & 1) == 0;
                 const float4 pi4 = own ? s_pos[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
                 const float4 ri = own ? s_req[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const int4 ng = my_ng[0];
+                unsigned long long skip = own ? 0ull : ~0ull;
+                {
+                    const int ngs[4] = {my_ng[0].x, my_ng[0].y, my_ng[0].z, my_ng[0].w};
+#pragma unroll
+                    for (int sl = 0; sl < 4; sl++) {
+                        int dd = ngs[sl] - tid;
+                        if (dd < 0) dd += N;
+                        if (ngs[sl] >= 0 && dd <= nd) skip |= 1ull << dd;
+                    }
+                    if (even && tid >= nd) skip |= 1ull << nd;
+                }
                 KahanF3 pacc;
                 pacc.init(make_float3(0.f, 0.f, 0.f));
                 int j = own ? tid : 0;
@@ -397,13 +412,11 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                         rr.y = wave_rot1(rr.y, N - 1, lane);
                         rr.z = wave_rot1(rr.z, N - 1, lane);
                     }
-                    const bool active = own && !(even && d == nd && tid >= nd);
-                    if (active && !(j == ng.x || j == ng.y || j == ng.z || j == ng.w)) {
-                        float3 f;
-                        (void)ljq_pair(f3(pc) - f3(pi4), ri, rc, T.r2damp, f);
-                        pacc.add(f);
-                        rr.x -= f.x; rr.y -= f.y; rr.z -= f.z;
-                    }
+                    float3 f;
+                    (void)ljq_pair(f3(pc) - f3(pi4), ri, rc, T.r2damp, f);
+                    if ((skip >> d) & 1ull) f = make_float3(0.f, 0.f, 0.f);
+                    pacc.add(f);
+                    rr.x -= f.x; rr.y -= f.y; rr.z -= f.z;
                 }
                 int src = tid - nd;
                 if (src < 0) src += N;
