@@ -49,6 +49,21 @@ def falg_per_system_step(nb, na, nd, ni, npairs, n):
     return 28 * nb + 80 * na + 120 * nd + 120 * ni + 41 * npairs + 740 * n + 40 * n
 
 
+def measured_traffic(args):
+    """HBM bytes per k_relax launch from the committed rocprofv3 counter runs
+    (FETCH_SIZE and WRITE_SIZE, copy-calibrated), for the workload they were
+    measured on; (None, None) otherwise.  Counters need their own profiler
+    run, so this figure cannot be taken inside the timed loop."""
+    if args.workload != "uniform" or args.nsys != 4096 or args.max_steps != 2000:
+        return None, None
+    path = os.path.join(ROOT, "profiles", "k_relax_uniform4096_ptcda.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        rec = json.load(f)
+    return float(rec["traffic_bytes_per_launch"]), rec["source"]
+
+
 def grid_for(ctx, S, W, L, zlen=20.0, step=0.1, alpha=1.5, rdamp=1.0, npbc=2):
     """Rock-salt GridFF of an L x L x zlen cell at `step` (device FP64 build + fit)."""
     nxy, nz = int(round(L / step)), int(round(zlen / step))
@@ -191,6 +206,7 @@ def main():
         flop_launch += 41.0 * n_inter * float(steps_arr[0])
     kavg_s = (sum(kms) / len(kms)) / 1e3
     achieved_tf = flop_launch / kavg_s / 1e12
+    traffic, traffic_src = measured_traffic(args)
     natoms_sys = np.array(natoms_sys)
 
     cfg = {
@@ -229,7 +245,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": "k_relax",
             "flop_per_relaxation": flop_launch,
             "kernel_ms_avg": round(1e3 * kavg_s, 3),
