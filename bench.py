@@ -193,6 +193,15 @@ def main():
         times.append(time.perf_counter() - t0)
         kms.append(batch.last_launch()["kernel_ms"])
     T = sum(times)
+    # the same relaxation with the boundary's host buffers: FP64 positions in
+    # (smr_batch_set_positions, over PCIe), relaxed, positions out
+    batch.reset()
+    t0 = time.perf_counter()
+    batch.set_positions(X)
+    r_io = batch.relax_batched(fire, args.max_steps, fconv)
+    _ = batch.positions()
+    t_io = time.perf_counter() - t0
+    nconv_io = sum(1 for r in r_io if r["status"] == S.SMR_SYS_CONVERGED)
     nconv = sum(1 for r in results if r["status"] == S.SMR_SYS_CONVERGED)
     steps_arr = np.array([r["steps"] for r in results])
     if layer:
@@ -265,6 +274,10 @@ def main():
         },
     }
 
+    out["host_buffers_inclusive"] = {
+        "value": round((1 if layer and nconv_io == nsys else (0 if layer else nconv_io)) / t_io, 4 if layer else 2),
+        "unit": out["unit"], "note": "one relaxation including the FP64 position upload and download over PCIe "
+        "(smr_batch_set_positions / smr_batch_get_positions); not `value`"}
     if layer:
         cfg["intermolecular_pairs_within_cutoff"] = n_inter
         cfg["fmax_final"] = round(float(results[0]["fmax"]), 6)
