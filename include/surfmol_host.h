@@ -48,6 +48,36 @@ int smh_molecule_bonds(const smh_molecule *m, int *bonds);
 const char *smh_molecule_types(const smh_molecule *m);
 void smh_molecule_free(smh_molecule *m);
 
+/* ---- the reference's on-disk topology formats ----
+ * Both readers take the file's bonds, angles, dihedrals and inversions as
+ * given (Uff::from_topology, molff/src/uff.rs:238-240, keeps the topology's
+ * lists and order), check that every term is made of bonds, then build the
+ * neighbour tables, parameters, hneigh slots and REQH as smh_molecule_build
+ * does.  charges (may be NULL) overrides REQH.Q per atom: neither format
+ * carries charges.  Errors (no abort): unreadable file, bad magic/version,
+ * counts that disagree with the payload, indices out of range, a term that
+ * is not made of bonds, an unknown UFF type. */
+/* UFFTOPO binary written by buildff --bin (crates/apps/buildff/src/main.rs:
+ * 10-30 format, 125-182 writer): "UFFTOPO", u8 version 1, i32 natoms nbonds
+ * nangles ndihedrals ninversions ntypes flags, then f64 apos[3N], i32 type
+ * index[N], i32 bonds[2nb], angles[3na], dihedrals[4nd], inversions[4ni] and
+ * the type table (8-byte names padded with spaces); little endian.  UFF types
+ * come from the file's table (elements from AtomTypes.dat). */
+int smh_molecule_read_ufftopo(const char *param_dir, const char *path, const double *charges, smh_molecule **out);
+/* TopologyData JSON (moltopo/src/export.rs:12-26; buildff --json, read by
+ * import_json :57-76 / surfmol::import::load_topology_from_json): natoms,
+ * elements, positions, bonds, angles, dihedrals, inversions; the *_params
+ * arrays are ignored (the reference writes them empty).  UFF types are
+ * assigned from the bonds (assign_uff.rs:8-118). */
+int smh_molecule_read_json(const char *param_dir, const char *path, const double *charges, smh_molecule **out);
+/* Writers of the same two formats (the reference's export_json and
+ * buildff --bin layouts), e.g. to hand a topology built here to the
+ * reference's tools. */
+int smh_molecule_write_ufftopo(const smh_molecule *m, const char *path);
+int smh_molecule_write_json(const smh_molecule *m, const char *path);
+/* Positions the molecule was built from or read with, [3*natoms]. */
+int smh_molecule_positions(const smh_molecule *m, double *apos);
+
 #ifdef __cplusplus
 }
 #endif

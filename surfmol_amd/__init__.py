@@ -23,6 +23,7 @@ Context needs one.
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -74,6 +75,40 @@ class Molecule:
         m = cls(h.value)
         m._keep = (arr, apos, q, bp)
         return m
+
+    @classmethod
+    def _read(cls, fn, path, charges, param_dir):
+        q = None if charges is None else np.ascontiguousarray(charges, dtype=np.float64)
+        h = ctypes.c_void_p()
+        check(fn(param_dir.encode(), os.fspath(path).encode(), None if q is None else _dptr(q), ctypes.byref(h)),
+              host=True)
+        m = cls(h.value)
+        if q is not None and len(q) != m.natoms:
+            raise ValueError(f"{len(q)} charges for {m.natoms} atoms")
+        m._keep = (q,)
+        return m
+
+    @classmethod
+    def read_ufftopo(cls, path, charges=None, param_dir=DATA_DIR):
+        """A topology from buildff's UFFTOPO binary (crates/apps/buildff/src/
+        main.rs:10-30): its term lists and UFF types as given."""
+        return cls._read(lib().smh_molecule_read_ufftopo, path, charges, param_dir)
+
+    @classmethod
+    def read_json(cls, path, charges=None, param_dir=DATA_DIR):
+        """A topology from a TopologyData JSON (moltopo/src/export.rs:12-26)."""
+        return cls._read(lib().smh_molecule_read_json, path, charges, param_dir)
+
+    def write_ufftopo(self, path):
+        check(lib().smh_molecule_write_ufftopo(self._h, os.fspath(path).encode()), host=True)
+
+    def write_json(self, path):
+        check(lib().smh_molecule_write_json(self._h, os.fspath(path).encode()), host=True)
+
+    def positions(self):
+        x = np.zeros((self.natoms, 3))
+        check(lib().smh_molecule_positions(self._h, _dptr(x)), host=True)
+        return x
 
     def __del__(self):
         try:

@@ -108,6 +108,11 @@ _SIGNATURES = [
     ("smh_molecule_bonds", c_int, [c_void_p, P(c_int)]),
     ("smh_molecule_types", c_char_p, [c_void_p]),
     ("smh_molecule_free", None, [c_void_p]),
+    ("smh_molecule_read_ufftopo", c_int, [c_char_p, c_char_p, P(c_double), P(c_void_p)]),
+    ("smh_molecule_read_json", c_int, [c_char_p, c_char_p, P(c_double), P(c_void_p)]),
+    ("smh_molecule_write_ufftopo", c_int, [c_void_p, c_char_p]),
+    ("smh_molecule_write_json", c_int, [c_void_p, c_char_p]),
+    ("smh_molecule_positions", c_int, [c_void_p, P(c_double)]),
 ]
 
 EXPORTED_SYMBOLS = [s[0] for s in _SIGNATURES]

@@ -12,14 +12,22 @@
 //   hneigh     molff/src/uff.rs:250-315
 //   REQH       moltopo/src/params.rs:595-604
 // The tables are returned through the C ABI in surfmol_host.h and feed
-// smr_batch_create() (surfmol_relax.h) unchanged.
+// smr_batch_create() (surfmol_relax.h) unchanged.  Topologies can also be
+// read from (and written to) the reference's two on-disk formats: buildff's
+// UFFTOPO binary (crates/apps/buildff/src/main.rs:10-30,125-182) and the
+// TopologyData JSON (moltopo/src/export.rs:12-76); their angle, dihedral and
+// inversion lists are taken as given, as Uff::from_topology does.
 #include "surfmol_host.h"
 
 #include <algorithm>
 #include <array>
+#include <cctype>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <iterator>
 #include <fstream>
 #include <map>
 #include <mutex>
@@ -207,9 +215,29 @@ struct smh_molecule {
     std::vector<double> reqs;                  // [4N]
     std::vector<std::string> types;
     std::string types_joined;
+    std::vector<std::string> elems;            // element symbols
+    std::vector<double> apos;                  // [3N] positions as built or read
 };
 
 extern "C" const char *smh_last_error(void) { return g_host_err.c_str(); }
+
+namespace {
+
+// Term lists and UFF types a caller supplies instead of the enumerated /
+// assigned ones (a topology read from a file: Uff::from_topology,
+// molff/src/uff.rs:238-240, takes the topology's lists as given).
+struct GivenTerms {
+    const std::vector<std::array<int, 3>> *angles = nullptr;
+    const std::vector<std::array<int, 4>> *dihedrals = nullptr;
+    const std::vector<std::array<int, 4>> *inversions = nullptr;
+    const std::vector<std::string> *types = nullptr;
+};
+
+int build_core(const ParamTables &P, const std::vector<std::string> &el, const double *apos, const double *charges,
+               const std::vector<std::array<int, 2>> &bonds, const GivenTerms &given, const std::string &who,
+               smh_molecule **out);
+
+}  // namespace
 
 extern "C" int smh_molecule_build(const char *param_dir, int natoms, const char *const *elems, const double *apos,
                                   const double *charges, int nbonds_in, const int *bonds_in, double rcut_bond,
@@ -244,6 +272,15 @@ extern "C" int smh_molecule_build(const char *param_dir, int natoms, const char 
                 if (dx * dx + dy * dy + dz * dz < r2cut) bonds.push_back({i, j});
             }
     }
+    return build_core(P, el, apos, charges, bonds, GivenTerms(), "smh_molecule_build", out);
+}
+
+namespace {
+
+int build_core(const ParamTables &P, const std::vector<std::string> &el, const double *apos, const double *charges,
+               const std::vector<std::array<int, 2>> &bonds, const GivenTerms &given, const std::string &who,
+               smh_molecule **out) {
+    const int n = (int)el.size();
     std::vector<std::vector<int>> adj(n);
     for (auto &b : bonds) {
         adj[b[0]].push_back(b[1]);
@@ -251,35 +288,70 @@ extern "C" int smh_molecule_build(const char *param_dir, int natoms, const char 
     }
     for (int i = 0; i < n; i++)
         if (adj[i].size() > 4)
-            return host_fail("smh_molecule_build: atom " + std::to_string(i) + " (" + el[i] + ") has " +
+            return host_fail(who + ": atom " + std::to_string(i) + " (" + el[i] + ") has " +
                              std::to_string(adj[i].size()) + " bonds; neighbour tables hold at most 4");
+    auto bonded = [&](int a, int b) { return std::find(adj[a].begin(), adj[a].end(), b) != adj[a].end(); };
 
-    // ---- angles (topology.rs:113-129) ----
+    // ---- angles (topology.rs:113-129), or as given: i-j-k with j bonded to both ----
     std::vector<std::array<int, 3>> angles;
-    for (int j = 0; j < n; j++)
-        for (size_t a = 0; a < adj[j].size(); a++)
-            for (size_t b = a + 1; b < adj[j].size(); b++) angles.push_back({adj[j][a], j, adj[j][b]});
-    // ---- dihedrals (topology.rs:131-159), as a sorted set ----
-    std::set<std::array<int, 4>> dset;
-    for (int j = 0; j < n; j++)
-        for (int k : adj[j])
-            for (int i : adj[j]) {
-                if (i == k) continue;
-                for (int l : adj[k]) {
-                    if (l == j) continue;
-                    if (l > i) dset.insert({i, j, k, l});
-                }
-            }
-    std::vector<std::array<int, 4>> dih(dset.begin(), dset.end());
-    // ---- inversions (topology.rs:161-182) ----
-    std::vector<std::array<int, 4>> inv;
-    for (int i = 0; i < n; i++)
-        if (adj[i].size() == 3) {
-            const auto &s = adj[i];
-            inv.push_back({i, s[0], s[1], s[2]});
-            inv.push_back({i, s[2], s[0], s[1]});
-            inv.push_back({i, s[1], s[2], s[0]});
+    if (given.angles) {
+        for (size_t q = 0; q < given.angles->size(); q++) {
+            const auto &a = (*given.angles)[q];
+            for (int v : a)
+                if (v < 0 || v >= n) return host_fail(who + ": angle " + std::to_string(q) + " has an atom out of range");
+            if (!bonded(a[0], a[1]) || !bonded(a[2], a[1]))
+                return host_fail(who + ": angle " + std::to_string(q) + " is not two bonds at atom " + std::to_string(a[1]));
         }
+        angles = *given.angles;
+    } else {
+        for (int j = 0; j < n; j++)
+            for (size_t a = 0; a < adj[j].size(); a++)
+                for (size_t b = a + 1; b < adj[j].size(); b++) angles.push_back({adj[j][a], j, adj[j][b]});
+    }
+    // ---- dihedrals (topology.rs:131-159), as a sorted set, or as given: a bonded chain i-j-k-l ----
+    std::vector<std::array<int, 4>> dih;
+    if (given.dihedrals) {
+        for (size_t q = 0; q < given.dihedrals->size(); q++) {
+            const auto &d = (*given.dihedrals)[q];
+            for (int v : d)
+                if (v < 0 || v >= n) return host_fail(who + ": dihedral " + std::to_string(q) + " has an atom out of range");
+            if (!bonded(d[0], d[1]) || !bonded(d[1], d[2]) || !bonded(d[2], d[3]))
+                return host_fail(who + ": dihedral " + std::to_string(q) + " is not a chain of three bonds");
+        }
+        dih = *given.dihedrals;
+    } else {
+        std::set<std::array<int, 4>> dset;
+        for (int j = 0; j < n; j++)
+            for (int k : adj[j])
+                for (int i : adj[j]) {
+                    if (i == k) continue;
+                    for (int l : adj[k]) {
+                        if (l == j) continue;
+                        if (l > i) dset.insert({i, j, k, l});
+                    }
+                }
+        dih.assign(dset.begin(), dset.end());
+    }
+    // ---- inversions (topology.rs:161-182), or as given: centre i bonded to j, k, l ----
+    std::vector<std::array<int, 4>> inv;
+    if (given.inversions) {
+        for (size_t q = 0; q < given.inversions->size(); q++) {
+            const auto &v4 = (*given.inversions)[q];
+            for (int v : v4)
+                if (v < 0 || v >= n) return host_fail(who + ": inversion " + std::to_string(q) + " has an atom out of range");
+            if (!bonded(v4[0], v4[1]) || !bonded(v4[0], v4[2]) || !bonded(v4[0], v4[3]))
+                return host_fail(who + ": inversion " + std::to_string(q) + " is not a centre with three bonds");
+        }
+        inv = *given.inversions;
+    } else {
+        for (int i = 0; i < n; i++)
+            if (adj[i].size() == 3) {
+                const auto &s = adj[i];
+                inv.push_back({i, s[0], s[1], s[2]});
+                inv.push_back({i, s[2], s[0], s[1]});
+                inv.push_back({i, s[1], s[2], s[0]});
+            }
+    }
     // ---- neighbour tables (molecular.rs:34-78) ----
     std::vector<std::array<int, 4>> ng(n, {-1, -1, -1, -1}), ngb(n, {-1, -1, -1, -1});
     for (size_t ib = 0; ib < bonds.size(); ib++) {
@@ -289,9 +361,18 @@ extern "C" int smh_molecule_build(const char *param_dir, int natoms, const char 
     }
     auto nneigh = [&](int i) { int c = 0; for (int s = 0; s < 4; s++) { if (ng[i][s] < 0) break; c++; } return c; };
 
-    // ---- UFF typing (assign_uff.rs:8-118) ----
+    // ---- UFF typing (assign_uff.rs:8-118), or as given ----
     std::vector<std::string> types(el);
     std::vector<char> set_atom(n, 0);
+    if (given.types) {
+        types = *given.types;
+        for (int ia = 0; ia < n; ia++) {
+            if (!P.atype(types[ia]))
+                return host_fail(who + ": atom " + std::to_string(ia) + " has UFF type '" + types[ia] +
+                                 "', not in AtomTypes.dat");
+            set_atom[ia] = 1;
+        }
+    }
     for (int ia = 0; ia < n; ia++) {
         if (set_atom[ia]) continue;
         const std::string &name = el[ia];
@@ -323,6 +404,8 @@ extern "C" int smh_molecule_build(const char *param_dir, int natoms, const char 
     auto *m = new smh_molecule();
     m->natoms = n;
     m->types = types;
+    m->elems = el;
+    m->apos.assign(apos, apos + 3 * (size_t)n);
     for (int i = 0; i < n; i++) {
         m->types_joined += types[i];
         if (i + 1 < n) m->types_joined += " ";
@@ -492,6 +575,8 @@ extern "C" int smh_molecule_build(const char *param_dir, int natoms, const char 
     return SMH_OK;
 }
 
+}  // namespace
+
 extern "C" int smh_molecule_counts(const smh_molecule *m, int *counts5) {
     if (!m || !counts5) return host_fail("smh_molecule_counts: NULL argument");
     counts5[0] = m->natoms;
@@ -534,3 +619,401 @@ extern "C" int smh_molecule_bonds(const smh_molecule *m, int *bonds) {
 extern "C" const char *smh_molecule_types(const smh_molecule *m) { return m ? m->types_joined.c_str() : ""; }
 
 extern "C" void smh_molecule_free(smh_molecule *m) { delete m; }
+
+// ===========================================================================
+// On-disk topology formats of the reference
+// ===========================================================================
+namespace {
+
+// ---- UFFTOPO binary (crates/apps/buildff/src/main.rs:10-30, writer :125-182):
+//      "UFFTOPO" u8 version=1, i32 natoms nbonds nangles ndihedrals
+//      ninversions ntypes flags (little endian), then apos f64[3N], type index
+//      i32[N], bonds i32[2nb], angles i32[3na], dihedrals i32[4nd], inversions
+//      i32[4ni], type table ntypes x 8 bytes (names padded with spaces) ----
+constexpr char kUffTopoMagic[7] = {'U', 'F', 'F', 'T', 'O', 'P', 'O'};
+
+struct ByteReader {
+    const std::vector<unsigned char> &b;
+    size_t p = 0;
+    bool ok = true;
+    explicit ByteReader(const std::vector<unsigned char> &buf) : b(buf) {}
+    bool take(void *dst, size_t n) {
+        if (!ok || p + n > b.size()) { ok = false; return false; }
+        std::memcpy(dst, b.data() + p, n);
+        p += n;
+        return true;
+    }
+    int32_t i32() {
+        unsigned char x[4] = {0, 0, 0, 0};
+        take(x, 4);
+        return (int32_t)((uint32_t)x[0] | ((uint32_t)x[1] << 8) | ((uint32_t)x[2] << 16) | ((uint32_t)x[3] << 24));
+    }
+    double f64() {
+        unsigned char x[8] = {0};
+        take(x, 8);
+        uint64_t u = 0;
+        for (int k = 7; k >= 0; k--) u = (u << 8) | x[k];
+        double d;
+        std::memcpy(&d, &u, 8);
+        return d;
+    }
+};
+
+void put_i32(std::vector<unsigned char> &o, int32_t v) {
+    const uint32_t u = (uint32_t)v;
+    for (int k = 0; k < 4; k++) o.push_back((unsigned char)((u >> (8 * k)) & 0xffu));
+}
+void put_f64(std::vector<unsigned char> &o, double d) {
+    uint64_t u;
+    std::memcpy(&u, &d, 8);
+    for (int k = 0; k < 8; k++) o.push_back((unsigned char)((u >> (8 * k)) & 0xffu));
+}
+
+bool read_file(const char *path, std::vector<unsigned char> &buf, std::string &err) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) { err = std::string("cannot open ") + path; return false; }
+    buf.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+    return true;
+}
+
+// ---- a small JSON reader, enough for TopologyData (moltopo/src/export.rs:12-26):
+//      objects, arrays, strings, numbers, true/false/null ----
+struct JVal {
+    enum Kind { NUL, BOOL, NUM, STR, ARR, OBJ } kind = NUL;
+    double num = 0;
+    std::string str;
+    std::vector<JVal> arr;
+    std::vector<std::pair<std::string, JVal>> obj;
+    const JVal *get(const std::string &k) const {
+        for (auto &kv : obj)
+            if (kv.first == k) return &kv.second;
+        return nullptr;
+    }
+};
+
+struct JParser {
+    const std::string &t;
+    size_t p = 0;
+    std::string err;
+    int depth = 0;
+    explicit JParser(const std::string &text) : t(text) {}
+    void ws() { while (p < t.size() && std::isspace((unsigned char)t[p])) p++; }
+    bool fail(const std::string &m) {
+        if (err.empty()) err = m + " at byte " + std::to_string(p);
+        return false;
+    }
+    bool str(std::string &out) {
+        if (p >= t.size() || t[p] != '"') return fail("expected a string");
+        p++;
+        out.clear();
+        while (p < t.size() && t[p] != '"') {
+            char c = t[p++];
+            if (c == '\\') {
+                if (p >= t.size()) return fail("bad escape");
+                char e = t[p++];
+                switch (e) {
+                    case 'n': out += '\n'; break;
+                    case 't': out += '\t'; break;
+                    case 'r': out += '\r'; break;
+                    case 'b': out += '\b'; break;
+                    case 'f': out += '\f'; break;
+                    case 'u': {
+                        if (p + 4 > t.size()) return fail("bad \\u escape");
+                        unsigned cp = (unsigned)std::stoul(t.substr(p, 4), nullptr, 16);
+                        p += 4;
+                        out += cp < 0x80 ? (char)cp : '?';   // names are ASCII
+                        break;
+                    }
+                    default: out += e;
+                }
+            } else {
+                out += c;
+            }
+        }
+        if (p >= t.size()) return fail("unterminated string");
+        p++;
+        return true;
+    }
+    bool value(JVal &v) {
+        if (++depth > 64) return fail("nesting too deep");
+        ws();
+        if (p >= t.size()) return fail("unexpected end");
+        const char c = t[p];
+        bool ok = true;
+        if (c == '{') {
+            v.kind = JVal::OBJ;
+            p++;
+            ws();
+            if (p < t.size() && t[p] == '}') { p++; depth--; return true; }
+            while (ok) {
+                ws();
+                std::string k;
+                if (!str(k)) return false;
+                ws();
+                if (p >= t.size() || t[p] != ':') return fail("expected ':'");
+                p++;
+                v.obj.emplace_back(k, JVal());
+                if (!value(v.obj.back().second)) return false;
+                ws();
+                if (p < t.size() && t[p] == ',') { p++; continue; }
+                if (p < t.size() && t[p] == '}') { p++; break; }
+                return fail("expected ',' or '}'");
+            }
+        } else if (c == '[') {
+            v.kind = JVal::ARR;
+            p++;
+            ws();
+            if (p < t.size() && t[p] == ']') { p++; depth--; return true; }
+            while (ok) {
+                v.arr.emplace_back();
+                if (!value(v.arr.back())) return false;
+                ws();
+                if (p < t.size() && t[p] == ',') { p++; continue; }
+                if (p < t.size() && t[p] == ']') { p++; break; }
+                return fail("expected ',' or ']'");
+            }
+        } else if (c == '"') {
+            v.kind = JVal::STR;
+            if (!str(v.str)) return false;
+        } else if (t.compare(p, 4, "true") == 0) {
+            v.kind = JVal::BOOL; v.num = 1; p += 4;
+        } else if (t.compare(p, 5, "false") == 0) {
+            v.kind = JVal::BOOL; v.num = 0; p += 5;
+        } else if (t.compare(p, 4, "null") == 0) {
+            v.kind = JVal::NUL; p += 4;
+        } else {
+            const char *b = t.c_str() + p;
+            char *e = nullptr;
+            v.num = std::strtod(b, &e);
+            if (e == b) return fail("bad value");
+            v.kind = JVal::NUM;
+            p += (size_t)(e - b);
+        }
+        depth--;
+        return true;
+    }
+};
+
+// rows of an array of k-integer arrays
+bool int_rows(const JVal *a, int k, std::vector<int> &out, std::string &err, const char *name) {
+    out.clear();
+    if (!a) return true;   // absent: empty
+    if (a->kind != JVal::ARR) { err = std::string(name) + " is not an array"; return false; }
+    for (size_t q = 0; q < a->arr.size(); q++) {
+        const JVal &r = a->arr[q];
+        if (r.kind != JVal::ARR || (int)r.arr.size() != k) {
+            err = std::string(name) + "[" + std::to_string(q) + "] is not an array of " + std::to_string(k);
+            return false;
+        }
+        for (const JVal &x : r.arr) {
+            if (x.kind != JVal::NUM || x.num != std::floor(x.num)) {
+                err = std::string(name) + "[" + std::to_string(q) + "] holds a non-integer";
+                return false;
+            }
+            out.push_back((int)x.num);
+        }
+    }
+    return true;
+}
+
+// shared tail of both readers: bonds + given terms -> molecule
+int build_read(const char *param_dir, const std::vector<std::string> &el, const std::vector<double> &apos,
+               const double *charges, const std::vector<int> &b2, const std::vector<int> &a3, const std::vector<int> &d4,
+               const std::vector<int> &i4, const std::vector<std::string> *types, const std::string &who,
+               smh_molecule **out) {
+    std::string err;
+    const ParamTables *Pp = get_params(param_dir ? param_dir : ".", err);
+    if (!Pp) return host_fail(who + ": " + err);
+    const int n = (int)el.size();
+    std::vector<std::array<int, 2>> bonds;
+    for (size_t q = 0; q < b2.size() / 2; q++) {
+        const int a = b2[2 * q], c = b2[2 * q + 1];
+        if (a < 0 || c < 0 || a >= n || c >= n || a == c)
+            return host_fail(who + ": bond " + std::to_string(q) + " has invalid atoms (" + std::to_string(a) + "," +
+                             std::to_string(c) + ")");
+        bonds.push_back({a, c});
+    }
+    std::vector<std::array<int, 3>> ang(a3.size() / 3);
+    for (size_t q = 0; q < ang.size(); q++) ang[q] = {a3[3 * q], a3[3 * q + 1], a3[3 * q + 2]};
+    std::vector<std::array<int, 4>> dih(d4.size() / 4), inv(i4.size() / 4);
+    for (size_t q = 0; q < dih.size(); q++) dih[q] = {d4[4 * q], d4[4 * q + 1], d4[4 * q + 2], d4[4 * q + 3]};
+    for (size_t q = 0; q < inv.size(); q++) inv[q] = {i4[4 * q], i4[4 * q + 1], i4[4 * q + 2], i4[4 * q + 3]};
+    GivenTerms g;
+    g.angles = &ang;
+    g.dihedrals = &dih;
+    g.inversions = &inv;
+    g.types = types;
+    return build_core(*Pp, el, apos.data(), charges, bonds, g, who, out);
+}
+
+}  // namespace
+
+extern "C" int smh_molecule_read_ufftopo(const char *param_dir, const char *path, const double *charges,
+                                         smh_molecule **out) {
+    const std::string who = "smh_molecule_read_ufftopo";
+    if (!out || !path) return host_fail(who + ": NULL argument");
+    *out = nullptr;
+    std::vector<unsigned char> buf;
+    std::string err;
+    if (!read_file(path, buf, err)) return host_fail(who + ": " + err);
+    ByteReader r(buf);
+    char magic[7];
+    unsigned char version = 0;
+    if (!r.take(magic, 7) || std::memcmp(magic, kUffTopoMagic, 7) != 0)
+        return host_fail(who + ": " + path + " does not start with \"UFFTOPO\"");
+    r.take(&version, 1);
+    if (version != 1) return host_fail(who + ": version " + std::to_string(version) + " (only 1 is defined)");
+    const int n = r.i32(), nb = r.i32(), na = r.i32(), nd = r.i32(), ni = r.i32(), nt = r.i32();
+    (void)r.i32();   // flags (reserved)
+    if (!r.ok || n <= 0 || nb < 0 || na < 0 || nd < 0 || ni < 0 || nt <= 0)
+        return host_fail(who + ": bad header counts");
+    const size_t need = (size_t)n * 24 + (size_t)n * 4 + (size_t)nb * 8 + (size_t)na * 12 + (size_t)nd * 16 +
+                        (size_t)ni * 16 + (size_t)nt * 8;
+    if (buf.size() - r.p != need)
+        return host_fail(who + ": " + std::to_string(buf.size() - r.p) + " payload bytes, the header implies " +
+                         std::to_string(need));
+    std::vector<double> apos(3 * (size_t)n);
+    for (auto &x : apos) x = r.f64();
+    std::vector<int> ti(n), b2(2 * (size_t)nb), a3(3 * (size_t)na), d4(4 * (size_t)nd), i4(4 * (size_t)ni);
+    for (auto &x : ti) x = r.i32();
+    for (auto &x : b2) x = r.i32();
+    for (auto &x : a3) x = r.i32();
+    for (auto &x : d4) x = r.i32();
+    for (auto &x : i4) x = r.i32();
+    std::vector<std::string> table(nt);
+    for (auto &t : table) {
+        char c[8];
+        r.take(c, 8);
+        std::string name(c, 8);
+        while (!name.empty() && (name.back() == ' ' || name.back() == '\0')) name.pop_back();
+        t = name;
+    }
+    if (!r.ok) return host_fail(who + ": truncated file");
+    std::string perr;
+    const ParamTables *Pp = get_params(param_dir ? param_dir : ".", perr);
+    if (!Pp) return host_fail(who + ": " + perr);
+    std::vector<std::string> types(n), el(n);
+    for (int i = 0; i < n; i++) {
+        if (ti[i] < 0 || ti[i] >= nt) return host_fail(who + ": atom " + std::to_string(i) + " type index out of range");
+        types[i] = table[ti[i]];
+        const AtomTypeRow *at = Pp->atype(types[i]);
+        if (!at) return host_fail(who + ": UFF type '" + types[i] + "' is not in AtomTypes.dat");
+        el[i] = at->element;
+    }
+    return build_read(param_dir, el, apos, charges, b2, a3, d4, i4, &types, who, out);
+}
+
+extern "C" int smh_molecule_read_json(const char *param_dir, const char *path, const double *charges,
+                                      smh_molecule **out) {
+    const std::string who = "smh_molecule_read_json";
+    if (!out || !path) return host_fail(who + ": NULL argument");
+    *out = nullptr;
+    std::vector<unsigned char> buf;
+    std::string err;
+    if (!read_file(path, buf, err)) return host_fail(who + ": " + err);
+    const std::string text(buf.begin(), buf.end());
+    JParser jp(text);
+    JVal root;
+    if (!jp.value(root)) return host_fail(who + ": " + path + ": " + jp.err);
+    if (root.kind != JVal::OBJ) return host_fail(who + ": the top level is not an object");
+    const JVal *na = root.get("natoms"), *je = root.get("elements"), *jp_ = root.get("positions");
+    if (!na || na->kind != JVal::NUM || !je || je->kind != JVal::ARR || !jp_ || jp_->kind != JVal::ARR)
+        return host_fail(who + ": natoms, elements and positions are required");
+    const int n = (int)na->num;
+    if (n <= 0 || (int)je->arr.size() != n || (int)jp_->arr.size() != n)
+        return host_fail(who + ": natoms = " + std::to_string(n) + " but " + std::to_string(je->arr.size()) +
+                         " elements and " + std::to_string(jp_->arr.size()) + " positions");
+    std::vector<std::string> el(n);
+    std::vector<double> apos(3 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        if (je->arr[i].kind != JVal::STR) return host_fail(who + ": elements[" + std::to_string(i) + "] is not a string");
+        el[i] = je->arr[i].str;
+        const JVal &q = jp_->arr[i];
+        if (q.kind != JVal::ARR || q.arr.size() != 3) return host_fail(who + ": positions[" + std::to_string(i) + "] is not [x,y,z]");
+        for (int k = 0; k < 3; k++) {
+            if (q.arr[k].kind != JVal::NUM) return host_fail(who + ": positions[" + std::to_string(i) + "] holds a non-number");
+            apos[3 * (size_t)i + k] = q.arr[k].num;
+        }
+    }
+    std::vector<int> b2, a3, d4, i4;
+    if (!int_rows(root.get("bonds"), 2, b2, err, "bonds") || !int_rows(root.get("angles"), 3, a3, err, "angles") ||
+        !int_rows(root.get("dihedrals"), 4, d4, err, "dihedrals") ||
+        !int_rows(root.get("inversions"), 4, i4, err, "inversions"))
+        return host_fail(who + ": " + err);
+    // the *_params arrays carry nothing the parameters are computed from
+    // (export.rs:37-42 writes them empty); UFF types follow from the bonds
+    return build_read(param_dir, el, apos, charges, b2, a3, d4, i4, nullptr, who, out);
+}
+
+extern "C" int smh_molecule_positions(const smh_molecule *m, double *apos) {
+    if (!m || !apos) return host_fail("smh_molecule_positions: NULL argument");
+    std::copy(m->apos.begin(), m->apos.end(), apos);
+    return SMH_OK;
+}
+
+extern "C" int smh_molecule_write_ufftopo(const smh_molecule *m, const char *path) {
+    if (!m || !path) return host_fail("smh_molecule_write_ufftopo: NULL argument");
+    // type table: sorted, deduplicated names, as buildff writes it
+    std::vector<std::string> table(m->types);
+    std::sort(table.begin(), table.end());
+    table.erase(std::unique(table.begin(), table.end()), table.end());
+    for (auto &t : table)
+        if (t.size() > 8) return host_fail("smh_molecule_write_ufftopo: type name '" + t + "' longer than 8 bytes");
+    std::vector<unsigned char> o(kUffTopoMagic, kUffTopoMagic + 7);
+    o.push_back(1);
+    const int n = m->natoms;
+    put_i32(o, n);
+    put_i32(o, (int)m->bonds.size() / 2);
+    put_i32(o, (int)m->ang_atoms.size() / 3);
+    put_i32(o, (int)m->dih_atoms.size() / 4);
+    put_i32(o, (int)m->inv_atoms.size() / 4);
+    put_i32(o, (int)table.size());
+    put_i32(o, 0);
+    for (double x : m->apos) put_f64(o, x);
+    for (int i = 0; i < n; i++)
+        put_i32(o, (int)(std::lower_bound(table.begin(), table.end(), m->types[i]) - table.begin()));
+    for (int v : m->bonds) put_i32(o, v);
+    for (int v : m->ang_atoms) put_i32(o, v);
+    for (int v : m->dih_atoms) put_i32(o, v);
+    for (int v : m->inv_atoms) put_i32(o, v);
+    for (auto &t : table) {
+        for (size_t k = 0; k < 8; k++) o.push_back(k < t.size() ? (unsigned char)t[k] : (unsigned char)' ');
+    }
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return host_fail(std::string("smh_molecule_write_ufftopo: cannot open ") + path);
+    f.write(reinterpret_cast<const char *>(o.data()), (std::streamsize)o.size());
+    if (!f) return host_fail(std::string("smh_molecule_write_ufftopo: write failed: ") + path);
+    return SMH_OK;
+}
+
+extern "C" int smh_molecule_write_json(const smh_molecule *m, const char *path) {
+    if (!m || !path) return host_fail("smh_molecule_write_json: NULL argument");
+    std::ostringstream s;
+    s.precision(17);
+    auto rows = [&](const char *name, const std::vector<int> &v, int k) {
+        s << "  \"" << name << "\": [";
+        for (size_t q = 0; q < v.size() / k; q++) {
+            s << (q ? ", [" : "[");
+            for (int c = 0; c < k; c++) s << (c ? ", " : "") << v[q * k + c];
+            s << "]";
+        }
+        s << "],\n";
+    };
+    s << "{\n  \"natoms\": " << m->natoms << ",\n  \"elements\": [";
+    for (int i = 0; i < m->natoms; i++) s << (i ? ", " : "") << "\"" << m->elems[i] << "\"";
+    s << "],\n  \"positions\": [";
+    for (int i = 0; i < m->natoms; i++)
+        s << (i ? ", [" : "[") << m->apos[3 * i] << ", " << m->apos[3 * i + 1] << ", " << m->apos[3 * i + 2] << "]";
+    s << "],\n";
+    rows("bonds", m->bonds, 2);
+    rows("angles", m->ang_atoms, 3);
+    rows("dihedrals", m->dih_atoms, 4);
+    rows("inversions", m->inv_atoms, 4);
+    s << "  \"bond_params\": [],\n  \"angle_params\": [],\n  \"dihedral_params\": [],\n"
+         "  \"inversion_params\": [],\n  \"atom_params\": []\n}\n";
+    std::ofstream f(path);
+    if (!f) return host_fail(std::string("smh_molecule_write_json: cannot open ") + path);
+    f << s.str();
+    if (!f) return host_fail(std::string("smh_molecule_write_json: write failed: ") + path);
+    return SMH_OK;
+}
