@@ -101,6 +101,23 @@ int smr_gridff_download(const smr_gridff *g, float *coefs);
 int smr_gridff_download_reference(const smr_gridff *g, double *values);
 int smr_gridff_destroy(smr_gridff *g);
 
+/* On-disk coefficient cache: the reference's lazy file cache of fitted grids
+ * (GridFF.h tryLoad_new reads Bspline_PLQd.npy and refits only when it is
+ * missing; notes/designs/2026-08-29_gridff_faf_porting_notes.md:393-394).
+ * Files are .npy, float32, shape (nx, ny, nz, 4) in C order -- the node
+ * layout above -- so numpy.load reads them.
+ * smr_gridff_save writes the grid's FP32 coefficients (to path.tmp, then
+ * renamed).  smr_gridff_load reads such a file into a new grid; the file's
+ * shape must match desc.  smr_gridff_build_cached keys a file in cache_dir
+ * by a hash of every build input (desc, npbc, substrate positions and REQ
+ * rows): it loads the file when present and valid (*hit = 1), else builds
+ * as smr_gridff_build, writes the file and sets *hit = 0 (hit may be NULL).
+ * A grid with keep_reference is always built. */
+int smr_gridff_save(const smr_gridff *g, const char *path);
+int smr_gridff_load(smr_ctx *ctx, const smr_grid_desc *desc, const char *path, smr_gridff **out);
+int smr_gridff_build_cached(smr_ctx *ctx, const smr_grid_desc *desc, int nsub, const double *sub_pos,
+                            const double *sub_reqs, int npbc, const char *cache_dir, int *hit, smr_gridff **out);
+
 /* ----------------------------------------------------------- systems */
 /* One molecule's force-field tables: exactly the public SoA fields of the
  * reference's molff::uff::Uff (crates/libs/molff/src/uff.rs:44-85), the

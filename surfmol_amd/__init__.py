@@ -237,6 +237,31 @@ class GridFF:
         return cls(ctx, h, spec)
 
     @classmethod
+    def build_cached(cls, ctx: Context, spec: GridSpec, sub_pos, sub_reqs, cache_dir, npbc=2):
+        """Load the fitted grid of these inputs from cache_dir, or build it and
+        write it there.  Returns (grid, hit)."""
+        sp = np.ascontiguousarray(sub_pos, dtype=np.float64).reshape(-1, 3)
+        sr = np.ascontiguousarray(sub_reqs, dtype=np.float64).reshape(-1, 4)
+        h = ctypes.c_void_p()
+        hit = ctypes.c_int(0)
+        d = spec.desc()
+        check(lib().smr_gridff_build_cached(ctx._h, ctypes.byref(d), len(sp), _dptr(sp), _dptr(sr), int(npbc),
+                                            os.fspath(cache_dir).encode(), ctypes.byref(hit), ctypes.byref(h)))
+        return cls(ctx, h, spec), bool(hit.value)
+
+    @classmethod
+    def load(cls, ctx: Context, spec: GridSpec, path):
+        """A grid from a coefficient file written by save() (.npy, float32,
+        shape (nx, ny, nz, 4))."""
+        h = ctypes.c_void_p()
+        d = spec.desc()
+        check(lib().smr_gridff_load(ctx._h, ctypes.byref(d), os.fspath(path).encode(), ctypes.byref(h)))
+        return cls(ctx, h, spec)
+
+    def save(self, path):
+        check(lib().smr_gridff_save(self._h, os.fspath(path).encode()))
+
+    @classmethod
     def upload(cls, ctx: Context, spec: GridSpec, coefs):
         c = np.ascontiguousarray(coefs, dtype=np.float32).reshape(-1)
         if c.size != spec.nx * spec.ny * spec.nz * 4:

@@ -76,6 +76,10 @@ _SIGNATURES = [
     ("smr_gridff_download", c_int, [c_void_p, P(c_float)]),
     ("smr_gridff_download_reference", c_int, [c_void_p, P(c_double)]),
     ("smr_gridff_destroy", c_int, [c_void_p]),
+    ("smr_gridff_save", c_int, [c_void_p, c_char_p]),
+    ("smr_gridff_load", c_int, [c_void_p, P(SmrGridDesc), c_char_p, P(c_void_p)]),
+    ("smr_gridff_build_cached", c_int, [c_void_p, P(SmrGridDesc), c_int, P(c_double), P(c_double), c_int, c_char_p,
+                                        P(c_int), P(c_void_p)]),
     ("smr_batch_create", c_int, [c_void_p, c_void_p, c_int, P(SmrTopology), c_int, P(c_int), P(c_double),
                                  P(SmrFFOptions), P(c_void_p)]),
     ("smr_batch_destroy", c_int, [c_void_p]),

@@ -8,8 +8,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -214,6 +216,128 @@ extern "C" int smr_gridff_download(const smr_gridff *g, float *coefs) {
     if (!g || !coefs) return fail(SMR_ERR_INVALID, "smr_gridff_download: NULL argument");
     HIPCHK(hipSetDevice(g->ctx->device));
     HIPCHK(hipMemcpy(coefs, g->coef.ptr, g->coef.bytes, hipMemcpyDeviceToHost));
+    return SMR_OK;
+}
+
+// ---- on-disk coefficient cache: the reference's lazy file cache of fitted
+//      grids (GridFF.h tryLoad_new loads Bspline_PLQd.npy and refits only
+//      when it is missing, notes/designs/2026-08-29_gridff_faf_porting_notes.md:
+//      393-394).  Files are .npy, float32, shape (nx, ny, nz, 4): the device
+//      node layout iz + nz*(iy + ny*ix) in C order. ----
+namespace {
+
+std::string npy_header(const smr_grid_desc *d) {
+    std::string h = "{'descr': '<f4', 'fortran_order': False, 'shape': (" + std::to_string(d->nx) + ", " +
+                    std::to_string(d->ny) + ", " + std::to_string(d->nz) + ", 4), }";
+    // magic (6) + version (2) + length (2) + header + '\n' is a multiple of 64
+    const size_t total = 10 + h.size() + 1;
+    h.append((64 - total % 64) % 64, ' ');
+    h += '\n';
+    return h;
+}
+
+// FNV-1a over the bytes of every build input
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void *p, size_t n) {
+        const unsigned char *b = (const unsigned char *)p;
+        for (size_t i = 0; i < n; i++) { h ^= b[i]; h *= 1099511628211ull; }
+    }
+    template <class T> void val(const T &v) { add(&v, sizeof(T)); }
+};
+
+}  // namespace
+
+extern "C" int smr_gridff_save(const smr_gridff *g, const char *path) {
+    if (!g || !path) return fail(SMR_ERR_INVALID, "smr_gridff_save: NULL argument");
+    HIPCHK(hipSetDevice(g->ctx->device));
+    std::vector<float> h(g->coef.bytes / sizeof(float));
+    HIPCHK(hipMemcpy(h.data(), g->coef.ptr, g->coef.bytes, hipMemcpyDeviceToHost));
+    const std::string hdr = npy_header(&g->desc);
+    // written to a temporary name, then renamed: a reader never sees half a file
+    const std::string tmp = std::string(path) + ".tmp";
+    {
+        std::ofstream f(tmp, std::ios::binary);
+        if (!f) return fail(SMR_ERR_INVALID, std::string("smr_gridff_save: cannot open ") + tmp);
+        const unsigned char pre[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+        f.write((const char *)pre, 8);
+        const uint16_t hl = (uint16_t)hdr.size();
+        const unsigned char len[2] = {(unsigned char)(hl & 0xff), (unsigned char)(hl >> 8)};
+        f.write((const char *)len, 2);
+        f.write(hdr.data(), (std::streamsize)hdr.size());
+        f.write((const char *)h.data(), (std::streamsize)(h.size() * sizeof(float)));
+        if (!f) return fail(SMR_ERR_INVALID, std::string("smr_gridff_save: write failed: ") + tmp);
+    }
+    if (std::rename(tmp.c_str(), path) != 0) return fail(SMR_ERR_INVALID, std::string("smr_gridff_save: rename to ") + path);
+    return SMR_OK;
+}
+
+extern "C" int smr_gridff_load(smr_ctx *c, const smr_grid_desc *d, const char *path, smr_gridff **out) {
+    if (!c || !d || !path || !out) return fail(SMR_ERR_INVALID, "smr_gridff_load: NULL argument");
+    *out = nullptr;
+    int rc = check_desc(d, "smr_gridff_load");
+    if (rc) return rc;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return fail(SMR_ERR_INVALID, std::string("smr_gridff_load: cannot open ") + path);
+    unsigned char pre[10];
+    f.read((char *)pre, 10);
+    if (!f || std::memcmp(pre, "\x93NUMPY", 6) != 0 || pre[6] != 1)
+        return fail(SMR_ERR_INVALID, std::string("smr_gridff_load: ") + path + " is not a version-1 .npy file");
+    const size_t hl = (size_t)pre[8] | ((size_t)pre[9] << 8);
+    std::string hdr(hl, ' ');
+    f.read(&hdr[0], (std::streamsize)hl);
+    if (!f || hdr != npy_header(d))
+        return fail(SMR_ERR_INVALID, std::string("smr_gridff_load: ") + path + " holds another dtype or shape than (" +
+                                         std::to_string(d->nx) + ", " + std::to_string(d->ny) + ", " +
+                                         std::to_string(d->nz) + ", 4) float32");
+    const size_t n = (size_t)d->nx * d->ny * d->nz * 4;
+    std::vector<float> h(n);
+    f.read((char *)h.data(), (std::streamsize)(n * sizeof(float)));
+    if (!f || f.peek() != std::char_traits<char>::eof())
+        return fail(SMR_ERR_INVALID, std::string("smr_gridff_load: ") + path + " has the wrong payload size");
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(h[i])) return fail(SMR_ERR_INVALID, std::string("smr_gridff_load: non-finite value in ") + path);
+    return smr_gridff_upload(c, d, h.data(), out);
+}
+
+extern "C" int smr_gridff_build_cached(smr_ctx *c, const smr_grid_desc *d, int nsub, const double *sub_pos,
+                                       const double *sub_reqs, int npbc, const char *cache_dir, int *hit,
+                                       smr_gridff **out) {
+    if (!c || !d || !out || !cache_dir) return fail(SMR_ERR_INVALID, "smr_gridff_build_cached: NULL argument");
+    *out = nullptr;
+    if (hit) *hit = 0;
+    if (nsub <= 0 || !sub_pos || !sub_reqs) return fail(SMR_ERR_INVALID, "smr_gridff_build_cached: need nsub > 0 substrate atoms");
+    Fnv k;
+    const char tag[] = "smr-gridff-v1";
+    k.add(tag, sizeof(tag));
+    k.val(d->nx); k.val(d->ny); k.val(d->nz);
+    k.add(d->p0, sizeof(d->p0)); k.add(d->step, sizeof(d->step));
+    k.val(d->alpha_morse); k.val(d->rdamp); k.val(npbc); k.val(nsub);
+    k.add(sub_pos, sizeof(double) * 3 * (size_t)nsub);
+    k.add(sub_reqs, sizeof(double) * 4 * (size_t)nsub);
+    char name[40];
+    std::snprintf(name, sizeof(name), "gridff_%016llx.npy", (unsigned long long)k.h);
+    const std::string path = std::string(cache_dir) + "/" + name;
+    // the FP64 reference values are not cached: a grid that keeps them is built
+    if (!d->keep_reference) {
+        std::ifstream probe(path, std::ios::binary);
+        if (probe.good()) {
+            probe.close();
+            if (smr_gridff_load(c, d, path.c_str(), out) == SMR_OK) {
+                if (hit) *hit = 1;
+                return SMR_OK;
+            }
+            // an unusable entry is rebuilt and overwritten
+        }
+    }
+    int rc = smr_gridff_build(c, d, nsub, sub_pos, sub_reqs, npbc, out);
+    if (rc) return rc;
+    rc = smr_gridff_save(*out, path.c_str());
+    if (rc) {
+        (void)smr_gridff_destroy(*out);
+        *out = nullptr;
+        return rc;
+    }
     return SMR_OK;
 }
 

@@ -25,7 +25,7 @@ def surf(ctx, oracle, params):
     E, C, (el, pos, q, reqs) = oracle_grid(oracle, params, spec)
     gs = S.GridSpec(spec["nx"], spec["ny"], spec["nz"], spec["p0"], spec["step"], ALPHA, RDAMP)
     g = S.GridFF.build(ctx, gs, pos, reqs, npbc=2, keep_reference=True)
-    return dict(spec=spec, E=E, C=C, grid=g)
+    return dict(spec=spec, E=E, C=C, grid=g, sub_pos=pos, sub_reqs=reqs)
 
 
 def test_gridff_build_matches_oracle(surf):
@@ -160,3 +160,38 @@ def test_launch_variants(ctx, surf):
         for a, c in zip(r0, r2):
             assert c["status"] == S.SMR_SYS_CONVERGED
             assert abs(a["energy"] - c["energy"]) / mol.natoms < 2e-5, (key, a, c)
+
+
+def test_gridff_cache(ctx, surf, tmp_path):
+    """The on-disk coefficient cache: a miss builds and writes a .npy that
+    numpy reads as the grid's coefficients; a hit returns the same FP32
+    values bit for bit; other substrate inputs miss; an unusable file is
+    rebuilt; a file of another shape is refused."""
+    import surfmol_amd as S
+    spec = surf["spec"]
+    gs = S.GridSpec(spec["nx"], spec["ny"], spec["nz"], spec["p0"], spec["step"], ALPHA, RDAMP)
+    pos, reqs = surf["sub_pos"], surf["sub_reqs"]
+    g1, hit1 = S.GridFF.build_cached(ctx, gs, pos, reqs, tmp_path, npbc=2)
+    files = sorted(tmp_path.glob("gridff_*.npy"))
+    assert not hit1 and len(files) == 1
+    c1 = g1.coefficients()
+    np.testing.assert_array_equal(np.load(files[0]).reshape(c1.shape), c1)
+    g2, hit2 = S.GridFF.build_cached(ctx, gs, pos, reqs, tmp_path, npbc=2)
+    assert hit2
+    np.testing.assert_array_equal(g2.coefficients(), c1)
+    # the cached grid equals a fresh build
+    np.testing.assert_array_equal(surf["grid"].coefficients(), c1)
+    pos2 = pos.copy()
+    pos2[0, 2] += 0.01
+    g3, hit3 = S.GridFF.build_cached(ctx, gs, pos2, reqs, tmp_path, npbc=2)
+    assert not hit3 and len(list(tmp_path.glob("gridff_*.npy"))) == 2
+    data = files[0].read_bytes()
+    files[0].write_bytes(data[:len(data) // 2])
+    g4, hit4 = S.GridFF.build_cached(ctx, gs, pos, reqs, tmp_path, npbc=2)
+    assert not hit4
+    np.testing.assert_array_equal(g4.coefficients(), c1)
+    bad = S.GridSpec(spec["nx"] + 1, spec["ny"], spec["nz"], spec["p0"], spec["step"], ALPHA, RDAMP)
+    with pytest.raises(S.SmrError):
+        S.GridFF.load(ctx, bad, files[0])
+    for g in (g1, g2, g3, g4):
+        g.close()
