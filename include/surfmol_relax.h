@@ -162,6 +162,18 @@ typedef struct {
                                  x = (r - (inter_cutoff - w)) / w, over the last w
                                  below the cutoff, so energy and forces stay
                                  continuous when pairs cross it                    */
+    double pbc_lx, pbc_ly;    /* coupled layer: lateral periodic box [A] (0: open
+                                 along that axis).  Every atom pair interacts in
+                                 every periodic image within the cutoff, a
+                                 molecule with its own images included; only an
+                                 atom pair of one molecule in the same image is
+                                 the intramolecular pair (the 3x3 image loop of
+                                 getNonBond's PBC branch, opencl/UFF.cl:1154-1182;
+                                 NonBondedFF::eval_pbc, molff/src/nonbonded.rs:
+                                 376-418, culled at the cutoff).  A box length
+                                 must be >= 3 cutoffs, a whole number of GridFF
+                                 node steps (1 A without a surface) and of GridFF
+                                 periods when there is a surface.                 */
 } smr_ff_options;
 
 /* Create a batch of nsys systems.  Systems reference one of ntopo

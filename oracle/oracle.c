@@ -69,6 +69,13 @@ typedef struct {
     int layer_cells;          /* 1: find intermolecular pairs through a cell grid (same terms) */
     double rswitch_inter;     /* > 0: intermolecular pair energies are switched off smoothly over
                                  [rc - rswitch_inter, rc] by S(x) = 1 - 3x^2 + 2x^3; 0: hard cut */
+    /* coupled layer with lateral periodic images (the 3x3 image loop of
+     * getNonBond's PBC branch, opencl/UFF.cl:1154-1182, and NonBondedFF::
+     * eval_pbc, nonbonded.rs:376-418, with a cutoff): box lengths along x and
+     * y (0: open).  Every pair of atoms, in every image within the cutoff,
+     * interacts as an intermolecular pair -- except an atom with the same
+     * image of its own molecule, which is the intramolecular pair. */
+    double pbc_lx, pbc_ly;
 } orc_system;
 
 typedef struct {
@@ -444,7 +451,53 @@ double orc_nb_eval(const orc_system *s, const double *apos, double *fapos) {
     int na = s->natoms;
     double r2damp = s->rdamp * s->rdamp;
     double etot = 0.0;
-    if (s->nb_mode == ORC_NB_GPU && s->mol_id && s->layer_cells) return orc_nb_eval_layer(s, apos, fapos);
+    const int pbc = s->mol_id && (s->pbc_lx > 0.0 || s->pbc_ly > 0.0);
+    if (s->nb_mode == ORC_NB_GPU && s->mol_id && s->layer_cells && !pbc) return orc_nb_eval_layer(s, apos, fapos);
+    if (s->nb_mode == ORC_NB_GPU && pbc) {
+        /* brute force over atom pairs and the images around the minimum
+         * image (a box of at least two cutoffs puts at most one image of a
+         * pair within the cutoff) */
+        const double L[2] = {s->pbc_lx, s->pbc_ly};
+        for (int ia = 0; ia < na; ia++) {
+            v3 pi = v3_ld(apos + 3 * ia);
+            const double *ri = s->reqs + 4 * ia;
+            const int *ng = s->neighs + 4 * ia;
+            v3 fe = {0, 0, 0};
+            double e = 0.0;
+            for (int ja = 0; ja < na; ja++) {
+                const double *rj = s->reqs + 4 * ja;
+                const v3 dp0 = v3_sub(v3_ld(apos + 3 * ja), pi);
+                const int same = s->mol_id[ja] == s->mol_id[ia];
+                int m[2] = {0, 0};
+                const double d0[2] = {dp0.x, dp0.y};
+                for (int k = 0; k < 2; k++)
+                    if (L[k] > 0.0) m[k] = -(int)floor(d0[k] / L[k] + 0.5);
+                for (int iy = -1; iy <= 1; iy++) {
+                    if (L[1] <= 0.0 && iy != 0) continue;
+                    for (int ix = -1; ix <= 1; ix++) {
+                        if (L[0] <= 0.0 && ix != 0) continue;
+                        const int nx = m[0] + ix, ny = m[1] + iy;
+                        v3 dp = dp0;
+                        dp.x += nx * L[0];
+                        dp.y += ny * L[1];
+                        v3 f;
+                        if (same && nx == 0 && ny == 0) {
+                            /* the molecule itself: getNonBond's intramolecular rule */
+                            if (ja == ia || ja == ng[0] || ja == ng[1] || ja == ng[2] || ja == ng[3]) continue;
+                            e += orc_ljqh_gpu(dp, ri[0] + rj[0], ri[1] * rj[1], ri[2] * rj[2], r2damp, &f);
+                        } else {
+                            if (v3_dot(dp, dp) >= s->rcut_inter2) continue;
+                            e += orc_inter_pair(s, dp, ri, rj, r2damp, &f);
+                        }
+                        fe = v3_add(fe, f);
+                    }
+                }
+            }
+            etot += e;
+            v3_st(fapos + 3 * ia, v3_add(v3_ld(fapos + 3 * ia), fe));
+        }
+        return etot;
+    }
     if (s->nb_mode == ORC_NB_GPU) {
         for (int ia = 0; ia < na; ia++) {
             v3 pi = v3_ld(apos + 3 * ia);

@@ -133,7 +133,7 @@ class OrcSystem(ctypes.Structure):
         ("gp0", ctypes.c_double * 3), ("ginv_step", ctypes.c_double * 3),
         ("alpha_morse", ctypes.c_double),
         ("mol_id", ctypes.POINTER(ctypes.c_int)), ("rcut_inter2", ctypes.c_double), ("layer_cells", ctypes.c_int),
-        ("rswitch_inter", ctypes.c_double),
+        ("rswitch_inter", ctypes.c_double), ("pbc_lx", ctypes.c_double), ("pbc_ly", ctypes.c_double),
     ]
 
 
@@ -706,6 +706,7 @@ class OracleSystem:
     rcut_inter: float = 0.0               # coupled layer: cutoff of pairs between molecules
     layer_cells: bool = False             # coupled layer: cell-grid pair search (same terms, O(N))
     rswitch_inter: float = 0.0            # coupled layer: width of the smooth switch below the cutoff
+    pbc: tuple = (0.0, 0.0)               # coupled layer: lateral periodic box (Lx, Ly); 0 = open
     _keep: list = field(default_factory=list)
 
     @property
@@ -763,6 +764,7 @@ class OracleSystem:
             s.rcut_inter2 = self.rcut_inter * self.rcut_inter
             s.layer_cells = 1 if self.layer_cells else 0
             s.rswitch_inter = self.rswitch_inter
+            s.pbc_lx, s.pbc_ly = float(self.pbc[0]), float(self.pbc[1])
         return s
 
 

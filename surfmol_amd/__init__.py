@@ -321,7 +321,7 @@ class Batch:
     """A set of systems on the device (shared topologies, per-system state)."""
 
     def __init__(self, ctx: Context, molecules, sys_topo, apos, grid: GridFF | None = None, rdamp=1.0,
-                 nonbonded=True, surface=True, inter_cutoff=0.0, inter_switch=0.0):
+                 nonbonded=True, surface=True, inter_cutoff=0.0, inter_switch=0.0, pbc=(0.0, 0.0)):
         self.ctx = ctx
         self.grid = grid
         self.molecules = list(molecules)
@@ -332,7 +332,7 @@ class Batch:
         if x.shape[0] != nat:
             raise ValueError(f"apos has {x.shape[0]} atoms, systems need {nat}")
         opts = _lib.SmrFFOptions(float(rdamp), 1 if nonbonded else 0, 1 if (surface and grid is not None) else 0,
-                                 float(inter_cutoff), float(inter_switch))
+                                 float(inter_cutoff), float(inter_switch), float(pbc[0]), float(pbc[1]))
         h = ctypes.c_void_p()
         check(lib().smr_batch_create(ctx._h, grid._h if grid is not None else None, len(self.molecules), topos,
                                      len(st), _iptr(st), _dptr(x), ctypes.byref(opts), ctypes.byref(h)))

@@ -45,7 +45,7 @@ class SmrGridDesc(ctypes.Structure):
 
 class SmrFFOptions(ctypes.Structure):
     _fields_ = [("rdamp", c_double), ("nonbonded", c_int), ("surface", c_int), ("inter_cutoff", c_double),
-                ("inter_switch", c_double)]
+                ("inter_switch", c_double), ("pbc_lx", c_double), ("pbc_ly", c_double)]
 
 
 class SmrFireParams(ctypes.Structure):

@@ -657,15 +657,44 @@ int layer_setup(smr_batch *b, const double *apos) {
     // 27-cell search), so a small margin suffices
     const double margin = 1.0;
     double h = rc * (1.0 + 1e-4) + 1e-3;
+    // periodic axes: the box [0, L) of the origin-grid frame in whole cells
+    // of at least the cutoff, at least 3 of them (one image per pair)
+    const double Lb[2] = {b->opts.pbc_lx, b->opts.pbc_ly};
+    const double ostep[2] = {b->tab.ostep64_x, b->tab.ostep64_y};
+    int nL[2] = {0, 0}, np_[2] = {0, 0};
+    for (int k = 0; k < 2; k++) {
+        if (Lb[k] <= 0.0) continue;
+        const char *ax = k ? "pbc_ly" : "pbc_lx";
+        np_[k] = (int)std::floor(Lb[k] / h);
+        if (np_[k] < 3)
+            return fail(SMR_ERR_INVALID, std::string("smr_batch_create: ") + ax + " = " + std::to_string(Lb[k]) +
+                                             " A holds fewer than 3 cells of the cutoff " + std::to_string(rc) + " A");
+        const double q = Lb[k] / ostep[k];
+        nL[k] = (int)std::lround(q);
+        if (std::fabs(q - nL[k]) > 1e-6 * std::max(1.0, q))
+            return fail(SMR_ERR_INVALID, std::string("smr_batch_create: ") + ax + " = " + std::to_string(Lb[k]) +
+                                             " A is not a whole number of origin-grid steps (" +
+                                             std::to_string(ostep[k]) + " A)");
+        if (b->grid) {
+            const double period = (k ? b->grid->desc.ny * b->grid->desc.step[1] : b->grid->desc.nx * b->grid->desc.step[0]);
+            const double m = Lb[k] / period;
+            if (std::fabs(m - std::lround(m)) > 1e-6 * std::max(1.0, m))
+                return fail(SMR_ERR_INVALID, std::string("smr_batch_create: ") + ax + " = " + std::to_string(Lb[k]) +
+                                                 " A is not a whole number of GridFF periods (" +
+                                                 std::to_string(period) + " A)");
+        }
+    }
     int n[3];
     for (;;) {
         long long tot = 1;
         for (int k = 0; k < 3; k++) {
-            n[k] = std::max(1, (int)std::ceil((hi[k] - lo[k] + 2.0 * margin) / h));
+            n[k] = (k < 2 && np_[k]) ? np_[k] : std::max(1, (int)std::ceil((hi[k] - lo[k] + 2.0 * margin) / h));
             tot *= n[k];
         }
         if (tot <= (1 << 20)) break;
         h *= 1.5;   // a very sparse layer: fewer, larger cells
+        for (int k = 0; k < 2; k++)
+            if (np_[k]) np_[k] = std::max(3, (int)std::floor(Lb[k] / h));
     }
     CellGrid &C = b->cells;
     memset(&C, 0, sizeof(C));
@@ -673,10 +702,20 @@ int layer_setup(smr_batch *b, const double *apos) {
     C.nox = n[0] > 1 ? 3 : 1;
     C.noy = n[1] > 1 ? 3 : 1;
     C.noz = n[2] > 1 ? 3 : 1;
-    C.lox = (float)(lo[0] - margin);
-    C.loy = (float)(lo[1] - margin);
+    C.lox = np_[0] ? 0.f : (float)(lo[0] - margin);
+    C.loy = np_[1] ? 0.f : (float)(lo[1] - margin);
     C.loz = (float)(lo[2] - margin);
     C.inv_h = (float)(1.0 / h);
+    C.inv_hx = np_[0] ? (float)(n[0] / Lb[0]) : C.inv_h;
+    C.inv_hy = np_[1] ? (float)(n[1] / Lb[1]) : C.inv_h;
+    C.pbc_x = np_[0] ? 1 : 0;
+    C.pbc_y = np_[1] ? 1 : 0;
+    C.Lx = (float)Lb[0];
+    C.Ly = (float)Lb[1];
+    C.inv_Lx = np_[0] ? (float)(1.0 / Lb[0]) : 0.f;
+    C.inv_Ly = np_[1] ? (float)(1.0 / Lb[1]) : 0.f;
+    C.nLx = nL[0];
+    C.nLy = nL[1];
     C.rc2 = (float)(rc * rc);
     C.rc2_64 = rc * rc;
     const double w = b->opts.inter_switch;
@@ -697,6 +736,10 @@ int layer_setup(smr_batch *b, const double *apos) {
     if (e == hipSuccess) e = b->cell_spos.alloc(sizeof(float4) * na);
     if (e == hipSuccess) e = b->cell_sreq.alloc(sizeof(float4) * na);
     if (e == hipSuccess) e = b->atom_cell.alloc(sizeof(int) * na);
+    if (e == hipSuccess) e = b->atom_wrap.alloc(sizeof(int2) * na);
+    if (e == hipSuccess) e = hipMemsetAsync(b->atom_wrap.ptr, 0, sizeof(int2) * na, st);
+    if (e == hipSuccess) e = b->cell_swrap.alloc(sizeof(int2) * na);
+    if (e == hipSuccess) e = hipMemsetAsync(b->cell_swrap.ptr, 0, sizeof(int2) * na, st);
     if (e == hipSuccess) e = b->cell_part.alloc(sizeof(float4) * 27 * (size_t)na);
     if (e == hipSuccess) e = b->slot_of.alloc(sizeof(int) * na);
     if (e == hipSuccess) e = b->partials.alloc(sizeof(double4) * ns);
@@ -709,6 +752,7 @@ int layer_setup(smr_batch *b, const double *apos) {
     C.start = (const int *)b->cell_start.ptr;
     C.part = (const float4 *)b->cell_part.ptr;
     C.slot_of = (const int *)b->slot_of.ptr;
+    C.swrap = (const int2 *)b->cell_swrap.ptr;
     C.natoms = na;
     b->layer = true;
     return SMR_OK;
@@ -732,6 +776,8 @@ LayerArgs layer_args(smr_batch *b) {
     L.spos = (float4 *)b->cell_spos.ptr;
     L.sreq = (float4 *)b->cell_sreq.ptr;
     L.atom_cell = (int *)b->atom_cell.ptr;
+    L.atom_wrap = (int2 *)b->atom_wrap.ptr;
+    L.swrap = (int2 *)b->cell_swrap.ptr;
     L.part = (float4 *)b->cell_part.ptr;
     L.slot_of = (int *)b->slot_of.ptr;
     L.cells = b->cells;
@@ -762,6 +808,10 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
         return fail(SMR_ERR_INVALID, "smr_batch_create: a coupled layer (inter_cutoff > 0) needs nonbonded = 1");
     if (!(opts->inter_switch >= 0.0) || (opts->inter_switch > 0.0 && opts->inter_switch >= opts->inter_cutoff))
         return fail(SMR_ERR_INVALID, "smr_batch_create: inter_switch must be in [0, inter_cutoff)");
+    if (!(opts->pbc_lx >= 0.0) || !(opts->pbc_ly >= 0.0) || opts->pbc_lx > 1e6 || opts->pbc_ly > 1e6)
+        return fail(SMR_ERR_INVALID, "smr_batch_create: pbc_lx, pbc_ly must be in [0, 1e6] A");
+    if ((opts->pbc_lx > 0.0 || opts->pbc_ly > 0.0) && !(opts->inter_cutoff > 0.0))
+        return fail(SMR_ERR_INVALID, "smr_batch_create: a periodic box needs a coupled layer (inter_cutoff > 0)");
     HIPCHK(hipSetDevice(c->device));
     auto *b = new smr_batch();
     b->ctx = c;

@@ -28,8 +28,8 @@ namespace smr {
 namespace {
 
 __device__ __forceinline__ int cell_of(const CellGrid &C, float gx, float gy, float gz) {
-    int cx = (int)floorf((gx - C.lox) * C.inv_h);
-    int cy = (int)floorf((gy - C.loy) * C.inv_h);
+    int cx = (int)floorf((gx - C.lox) * C.inv_hx);
+    int cy = (int)floorf((gy - C.loy) * C.inv_hy);
     int cz = (int)floorf((gz - C.loz) * C.inv_h);
     cx = min(max(cx, 0), C.ncx - 1);
     cy = min(max(cy, 0), C.ncy - 1);
@@ -60,8 +60,21 @@ __global__ void k_layer_move_count(LayerArgs L, int move) {
         L.pos[a] = p;
     }
     const SysDesc sd = L.tab.sys[L.atom_sys[a]];
-    const int c = cell_of(L.cells, p.x + (float)sd.gx * L.tab.ostep_x, p.y + (float)sd.gy * L.tab.ostep_y,
-                          p.z + (float)sd.gz * L.tab.ostep_z);
+    const CellGrid &C = L.cells;
+    float gx = p.x + (float)sd.gx * L.tab.ostep_x, gy = p.y + (float)sd.gy * L.tab.ostep_y;
+    // periodic axes: the atom's image in the box [0, L) and how many box
+    // lengths it was moved by (an exact integer used by the pair kernels)
+    int2 w = make_int2(0, 0);
+    if (C.pbc_x) {
+        w.x = (int)floorf(gx * C.inv_Lx);
+        gx -= (float)w.x * C.Lx;
+    }
+    if (C.pbc_y) {
+        w.y = (int)floorf(gy * C.inv_Ly);
+        gy -= (float)w.y * C.Ly;
+    }
+    L.atom_wrap[a] = w;
+    const int c = cell_of(C, gx, gy, p.z + (float)sd.gz * L.tab.ostep_z);
     L.atom_cell[a] = c;
     atomicAdd(&L.count[c], 1);
 }
@@ -134,6 +147,7 @@ __global__ __launch_bounds__(256) void k_cells_sort(LayerArgs L) {
             const float4 p = L.pos[a];
             L.spos[st + t] = make_float4(p.x, p.y, p.z, __int_as_float(L.atom_sys[a]));
             L.sreq[st + t] = L.req_atom[a];
+            L.swrap[st + t] = L.atom_wrap[a];
         }
     } else if (t == 0) {
         // a crowded cell: insertion sort and gather by one thread
@@ -152,6 +166,7 @@ __global__ __launch_bounds__(256) void k_cells_sort(LayerArgs L) {
             const float4 p = L.pos[a];
             L.spos[q] = make_float4(p.x, p.y, p.z, __int_as_float(L.atom_sys[a]));
             L.sreq[q] = L.req_atom[a];
+            L.swrap[q] = L.atom_wrap[a];
         }
     }
 }
@@ -161,6 +176,11 @@ __global__ __launch_bounds__(256) void k_cells_sort(LayerArgs L) {
 // single cell thick along some axis): partners of other molecules within
 // the cutoff, in slot order, Kahan-summed.  Writes part[o][slot] for every
 // atom of the cell (zero when the neighbour cell lies outside the grid).
+// Along a periodic axis the neighbour cell wraps around the box and carries
+// an image shift; the displacement of a pair is then
+//   (pj - pi) + ostep * ((gj - gi) + (image - wj + wi) * nL)
+// with g the systems' origin offsets and w the atoms' box wraps, all exact
+// integers, and a molecule meets its own atoms in every other image.
 __global__ __launch_bounds__(64) void k_layer_inter(LayerArgs L) {
     const CellGrid &C = L.cells;
     const DevTables &T = L.tab;
@@ -170,9 +190,18 @@ __global__ __launch_bounds__(64) void k_layer_inter(LayerArgs L) {
     if (n == 0) return;
     const int cx = c % C.ncx, cy = (c / C.ncx) % C.ncy, cz = c / (C.ncx * C.ncy);
     // neighbour offsets only along axes with more than one cell
-    const int x = cx + (C.nox == 3 ? o % 3 - 1 : 0);
-    const int y = cy + (C.noy == 3 ? (o / C.nox) % 3 - 1 : 0);
+    int x = cx + (C.nox == 3 ? o % 3 - 1 : 0);
+    int y = cy + (C.noy == 3 ? (o / C.nox) % 3 - 1 : 0);
     const int z = cz + (C.noz == 3 ? o / (C.nox * C.noy) - 1 : 0);
+    int imx = 0, imy = 0;
+    if (C.pbc_x) {
+        imx = x < 0 ? -1 : (x >= C.ncx ? 1 : 0);
+        x -= imx * C.ncx;
+    }
+    if (C.pbc_y) {
+        imy = y < 0 ? -1 : (y >= C.ncy ? 1 : 0);
+        y -= imy * C.ncy;
+    }
     const bool inside = x >= 0 && x < C.ncx && y >= 0 && y < C.ncy && z >= 0 && z < C.ncz;
     const int cc = inside ? (z * C.ncy + y) * C.ncx + x : 0;
     const int q0 = inside ? C.start[cc] : 0, q1 = inside ? C.start[cc + 1] : 0;
@@ -182,21 +211,27 @@ __global__ __launch_bounds__(64) void k_layer_inter(LayerArgs L) {
         const float4 ri = C.sreq[s0 + t];
         const int si = __float_as_int(pi.w);
         const SysDesc di = T.sys[si];
+        const int2 wi = C.swrap[s0 + t];
         KahanF3 acc;
         acc.init(make_float3(0.f, 0.f, 0.f));
-        int last = -1;
-        float dgx = 0.f, dgy = 0.f, dgz = 0.f;
+        int last = -1, ogx = 0, ogy = 0, ogz = 0;
         for (int q = q0; q < q1; q++) {
             const float4 pj = C.spos[q];
             const int sj = __float_as_int(pj.w);
-            if (sj == si) continue;
+            const int2 wj = C.swrap[q];
+            // box lengths between the images (0 on an open axis)
+            const int bx = imx - wj.x + wi.x, by = imy - wj.y + wi.y;
+            if (sj == si && bx == 0 && by == 0) continue;   // the molecule itself: intramolecular
             if (sj != last) {
                 const SysDesc dj = T.sys[sj];
-                dgx = (float)(dj.gx - di.gx) * T.ostep_x;
-                dgy = (float)(dj.gy - di.gy) * T.ostep_y;
-                dgz = (float)(dj.gz - di.gz) * T.ostep_z;
+                ogx = dj.gx - di.gx;
+                ogy = dj.gy - di.gy;
+                ogz = dj.gz - di.gz;
                 last = sj;
             }
+            const float dgx = (float)(ogx + bx * C.nLx) * T.ostep_x;
+            const float dgy = (float)(ogy + by * C.nLy) * T.ostep_y;
+            const float dgz = (float)ogz * T.ostep_z;
             const float3 dp = make_float3((pj.x - pi.x) + dgx, (pj.y - pi.y) + dgy, (pj.z - pi.z) + dgz);
             const float r2 = dot3(dp, dp);
             if (r2 >= C.rc2) continue;
@@ -231,18 +266,28 @@ __global__ __launch_bounds__(256) void k_energy64_inter(LayerArgs L, SysState *s
         const float4 pi = L.pos[a];
         const double4 ri = T.req64[td.atom_off + i];
         const int c = L.atom_cell[a];
+        const int2 wi = L.atom_wrap[a];
         const int cx = c % C.ncx, cy = (c / C.ncx) % C.ncy, cz = c / (C.ncx * C.ncy);
+        // neighbour cells: clipped on open axes, wrapped with an image on periodic ones
+        const int x0 = C.pbc_x ? cx - 1 : max(cx - 1, 0), x1 = C.pbc_x ? cx + 1 : min(cx + 1, C.ncx - 1);
+        const int y0 = C.pbc_y ? cy - 1 : max(cy - 1, 0), y1 = C.pbc_y ? cy + 1 : min(cy + 1, C.ncy - 1);
         for (int z = max(cz - 1, 0); z <= min(cz + 1, C.ncz - 1); z++)
-            for (int y = max(cy - 1, 0); y <= min(cy + 1, C.ncy - 1); y++)
-                for (int x = max(cx - 1, 0); x <= min(cx + 1, C.ncx - 1); x++) {
+            for (int yy = y0; yy <= y1; yy++)
+                for (int xx = x0; xx <= x1; xx++) {
+                    const int imx = xx < 0 ? -1 : (xx >= C.ncx ? 1 : 0), imy = yy < 0 ? -1 : (yy >= C.ncy ? 1 : 0);
+                    const int x = xx - imx * C.ncx, y = yy - imy * C.ncy;
                     const int cc = (z * C.ncy + y) * C.ncx + x;
                     for (int q = C.start[cc]; q < C.start[cc + 1]; q++) {
                         const float4 pj = C.spos[q];
                         const int sj = __float_as_int(pj.w);
-                        if (sj == s) continue;
+                        const int2 wj = C.swrap[q];
+                        const int bx = imx - wj.x + wi.x, by = imy - wj.y + wi.y;
+                        if (sj == s && bx == 0 && by == 0) continue;
                         const SysDesc dj = T.sys[sj];
-                        const double dx = ((double)pj.x - (double)pi.x) + (double)(dj.gx - sd.gx) * T.ostep64_x;
-                        const double dy = ((double)pj.y - (double)pi.y) + (double)(dj.gy - sd.gy) * T.ostep64_y;
+                        const double dx = ((double)pj.x - (double)pi.x) +
+                                          (double)((dj.gx - sd.gx) + bx * C.nLx) * T.ostep64_x;
+                        const double dy = ((double)pj.y - (double)pi.y) +
+                                          (double)((dj.gy - sd.gy) + by * C.nLy) * T.ostep64_y;
                         const double dz = ((double)pj.z - (double)pi.z) + (double)(dj.gz - sd.gz) * T.ostep64_z;
                         const double r2 = dx * dx + dy * dy + dz * dz;
                         if (r2 >= rc2) continue;

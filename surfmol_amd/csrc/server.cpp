@@ -496,7 +496,7 @@ extern "C" int smr_server_create(smr_ctx *c, const smr_gridff *grid, const smr_s
     int rc = check_fire(&cfg->fire, "smr_server_create");
     if (rc) return rc;
     if (cfg->ff.surface && !grid) return fail(SMR_ERR_INVALID, "smr_server_create: surface requested but grid == NULL");
-    if (cfg->ff.inter_cutoff != 0.0)
+    if (cfg->ff.inter_cutoff != 0.0 || cfg->ff.pbc_lx != 0.0 || cfg->ff.pbc_ly != 0.0)
         return fail(SMR_ERR_UNSUPPORTED, "smr_server_create: requests are independent molecules (inter_cutoff must be 0)");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");

@@ -22,7 +22,15 @@ struct CellGrid {
     int natoms;
     int ncx, ncy, ncz;
     float lox, loy, loz;     // lower corner in the origin-grid frame
-    float inv_h;             // 1 / cell size
+    float inv_h;             // 1 / cell size (z; x and y unless periodic)
+    float inv_hx, inv_hy;    // 1 / cell size along x, y
+    // lateral periodic box (pbc_x/pbc_y: 0 open, 1 periodic): lengths, their
+    // inverses and their length in origin-grid steps (an exact integer), and
+    // the wrap count of each atom by sorted slot
+    int pbc_x, pbc_y;
+    float Lx, Ly, inv_Lx, inv_Ly;
+    int nLx, nLy;
+    const int2 *swrap;
     float rc2;               // cutoff squared (force test, FP32)
     double rc2_64;           // cutoff squared (energy test, FP64)
     float ron, ron2, inv_w;  // switch: start, start squared, 1 / width (ron2 = rc2: hard cutoff)
@@ -129,6 +137,8 @@ struct LayerArgs {
     int *count, *fill, *start, *sidx;   // [ncells], [ncells], [ncells+1], [natoms]
     float4 *spos, *sreq;
     int *atom_cell;          // [natoms]
+    int2 *atom_wrap;         // [natoms] box wraps of each atom (periodic layer)
+    int2 *swrap;             // [natoms] the same by sorted slot
     float4 *part;            // [K][natoms]
     int *slot_of;            // [natoms]
     CellGrid cells;

@@ -108,3 +108,73 @@ def test_layer_relax_is_reproducible(ctx, surf, oracle, params):
         assert np.array_equal(b.positions(), x1)
         assert [r["energy"] for r in r1] == [r["energy"] for r in r2]
         assert r1[0]["steps"] == r2[0]["steps"]
+
+
+# ---------------------------------------------------------------------------
+# lateral periodic box (getNonBond's PBC branch, opencl/UFF.cl:1154-1182,
+# culled at the cutoff): device against the FP64 oracle's periodic layer
+# ---------------------------------------------------------------------------
+def _pbc_layer(ctx, surf, oracle, params, nx, ny, spacing, rc, shift=(0.0, 0.0), w=0.0, seed=11):
+    import surfmol_amd as S
+    from surfmol_amd import workloads as W
+    mol, X = W.layer_lattice(nx, ny, spacing=spacing, seed=seed)
+    X = [x + np.array([shift[0], shift[1], 0.0]) for x in X]
+    box = (nx * spacing, ny * spacing)
+    hm = S.Molecule.build(mol.elems, X[0], mol.charges, bonds=mol.bonds)
+    b = S.Batch(ctx, [hm], np.zeros(len(X), dtype=np.int32), np.concatenate(X), grid=surf["grid"], rdamp=RDAMP,
+                inter_cutoff=rc, inter_switch=w, pbc=box)
+    spec = surf["spec"]
+    osys = oracle.layer_from_molecules([mol] * len(X), X, params, rc, rdamp=RDAMP, rswitch_inter=w, pbc=box,
+                                       grid=surf["C"], grid_ns=(spec["nx"], spec["ny"], spec["nz"]),
+                                       grid_p0=spec["p0"], grid_step=spec["step"], alpha_morse=spec["alpha"])
+    return mol, X, box, b, osys
+
+
+@pytest.mark.parametrize("rc,w,shift", [(9.0, 0.0, (0.0, 0.0)), (12.0, 2.0, (-5.3, 7.1)), (12.0, 0.0, (-9.0, -2.0))])
+def test_pbc_layer_eval_forces(ctx, surf, oracle, params, rc, w, shift):
+    """4 x 4 molecules at 12 A in a 48 x 48 A box (four GridFF periods); the
+    shifted lattices put molecules across the box edges."""
+    mol, X, box, b, osys = _pbc_layer(ctx, surf, oracle, params, 4, 4, 12.0, rc, shift=shift, w=w)
+    F, E = b.eval_forces()
+    Xd = b.positions()
+    fo, eo = oracle.eval_forces(osys, Xd)
+    err = rel_force_err(F, fo)
+    print("pbc rc", rc, "shift", shift, "force err", err)
+    assert err < 1e-5
+    n = len(Xd)
+    assert (np.abs(E.sum(axis=0) - eo) / n < 1e-6).all(), (E.sum(axis=0), eo)
+    # the images matter: the open layer's forces differ
+    import surfmol_amd as S
+    bo = S.Batch(ctx, [b.molecules[0]], np.zeros(len(X), dtype=np.int32), np.concatenate(X), grid=surf["grid"],
+                 rdamp=RDAMP, inter_cutoff=rc, inter_switch=w)
+    Fo, _ = bo.eval_forces()
+    assert np.abs(Fo - F).max() > 1e-3
+
+
+def test_pbc_layer_relax_is_fp64_minimum(ctx, surf, oracle, params):
+    import surfmol_amd as S
+    mol, X, box, b, osys = _pbc_layer(ctx, surf, oracle, params, 3, 3, 16.0, 12.0, shift=(-4.0, 3.0), w=2.0)
+    fp = fire_params_internal()
+    res = b.relax_batched(S.FireParams(**fp), 4000, 1e-3)
+    assert all(r["status"] == S.SMR_SYS_CONVERGED for r in res), res[0]
+    Xf = b.positions()
+    fo, eo = oracle.eval_forces(osys, Xf)
+    n = len(Xf)
+    Etot = sum(r["energy"] for r in res)
+    assert abs(eo.sum() - Etot) / n < 1e-6, (eo.sum(), Etot)
+    rf = oracle.relax_fire(osys, oracle.FireParams(**fp), 4000, 1e-3, apos=Xf)
+    assert rf["converged"] and abs(rf["E"] - Etot) / n < 1e-6, (rf["E"], Etot)
+    print("pbc layer steps", res[0]["steps"], "E", Etot)
+
+
+def test_pbc_box_checks(ctx, surf):
+    import surfmol_amd as S
+    from surfmol_amd import workloads as W
+    mol, X = W.layer_lattice(2, 2, spacing=12.0, seed=3)
+    hm = S.Molecule.build(mol.elems, X[0], mol.charges, bonds=mol.bonds)
+    args = (ctx, [hm], np.zeros(4, dtype=np.int32), np.concatenate(X))
+    for box, rc in (((24.0, 24.0), 12.0),      # fewer than 3 cells of the cutoff
+                    ((50.0, 48.0), 12.0),      # not a whole number of GridFF periods (12 A)
+                    ((48.0, 48.0), 0.0)):      # a box without a coupled layer
+        with pytest.raises(S.SmrError):
+            S.Batch(*args, grid=surf["grid"], rdamp=RDAMP, inter_cutoff=rc, pbc=box)

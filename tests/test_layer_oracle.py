@@ -131,3 +131,62 @@ def test_switched_layer_energy_is_smooth(oracle, params):
     # switch shape: S(rc - w) = 1, S(rc) = 0, S' = 0 at both ends
     for x, S in ((0.0, 1.0), (1.0, 0.0)):
         assert abs(1 - 3 * x * x + 2 * x ** 3 - S) < 1e-15 and abs(-6 * x + 6 * x * x) < 1e-15
+
+
+# ---------------------------------------------------------------------------
+# lateral periodic images (getNonBond's PBC branch, opencl/UFF.cl:1154-1182;
+# NonBondedFF::eval_pbc, nonbonded.rs:376-418), with the layer's cutoff
+# ---------------------------------------------------------------------------
+def _pbc_layer(oracle, params, nx, ny, spacing, rc, w=0.0, seed=5):
+    mol, X = W.layer_lattice(nx, ny, spacing=spacing, seed=seed)
+    L = (nx * spacing, ny * spacing)
+    sysP = oracle.layer_from_molecules([mol] * len(X), X, params, rc, rdamp=RDAMP, rswitch_inter=w, pbc=L)
+    return mol, X, L, sysP
+
+
+def test_pbc_equals_the_central_copy_of_a_replicated_layer(oracle, params):
+    """A periodic box of 2 x 2 molecules acts on its atoms exactly as the
+    central copy of an open 3 x 3 tiling of that box (the box is at least two
+    cutoffs wide, so every partner within the cutoff lies in the tiling)."""
+    for w in (0.0, 2.0):
+        mol, X, L, sysP = _pbc_layer(oracle, params, 2, 2, 13.0, 9.0, w=w)
+        FP, EP = oracle.eval_forces(sysP)
+        Xr, centre = [], None
+        for iy in (-1, 0, 1):
+            for ix in (-1, 0, 1):
+                if ix == 0 and iy == 0:
+                    centre = len(Xr)
+                Xr += [x + np.array([ix * L[0], iy * L[1], 0.0]) for x in X]
+        sysR = oracle.layer_from_molecules([mol] * len(Xr), Xr, params, 9.0, rdamp=RDAMP, rswitch_inter=w)
+        FR, _ = oracle.eval_forces(sysR)
+        n = mol.natoms
+        FRc = FR[centre * n:(centre + len(X)) * n]
+        assert np.abs(FP - FRc).max() < 1e-10 * max(1.0, np.abs(FP).max()), w
+        # the images matter: the open 2 x 2 layer differs
+        sysO = oracle.layer_from_molecules([mol] * len(X), X, params, 9.0, rdamp=RDAMP, rswitch_inter=w)
+        FO, _ = oracle.eval_forces(sysO)
+        assert np.abs(FO - FP).max() > 1e-3
+
+
+def test_pbc_moving_a_molecule_by_a_box_vector_changes_nothing(oracle, params):
+    mol, X, L, sysP = _pbc_layer(oracle, params, 2, 2, 13.0, 9.0, w=2.0)
+    F0, E0 = oracle.eval_forces(sysP)
+    X2 = [x.copy() for x in X]
+    X2[1] += np.array([L[0], -2 * L[1], 0.0])
+    F1, E1 = oracle.eval_forces(sysP, np.concatenate(X2))
+    assert np.abs(F0 - F1).max() < 1e-9 and np.abs(E0 - E1).max() < 1e-9
+
+
+def test_pbc_force_is_minus_gradient(oracle, params):
+    mol, X, L, sysP = _pbc_layer(oracle, params, 2, 2, 13.0, 9.0, w=3.0)
+    x0 = np.concatenate(X)
+    F, _ = oracle.nb_eval(sysP, x0)
+    h = 1e-5
+    rng = np.random.Generator(np.random.PCG64(4))
+    for a in rng.choice(len(x0), 10, replace=False):
+        for k in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[a, k] += h
+            xm[a, k] -= h
+            d = -(oracle.nb_eval(sysP, xp)[1] - oracle.nb_eval(sysP, xm)[1]) / (2 * h)
+            assert abs(d / 2 - F[a, k]) < 1e-5 * max(1.0, abs(F[a, k])), (a, k, d / 2, F[a, k])
