@@ -240,10 +240,12 @@ int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fire, int max
                     smr_relax_result *result);
 
 /* Launch variant of the fused kernel for this batch: threads per workgroup
- * (64/128/256; 0 = automatic by system size; atoms per thread follow), the
- * per-lane register cache of each atom's surface stencil (1 on, 0 off, -1
- * automatic) and the pair loop (1 unordered-pair sweep, 0 every ordered
- * pair, -1 automatic).  All variants compute the same forces; results differ
+ * (64/128/256/512; 0 = automatic by system size; atoms per thread follow),
+ * the stored per-atom surface stencils (1 on, 0 off, -1 automatic: with one
+ * atom per thread the stencil lives in registers during a launch and the
+ * stored copy carries it across launches; with more atoms per thread the
+ * stored copy is the cache) and the pair loop (1 unordered-pair sweep, 0
+ * every ordered pair, -1 automatic).  All variants compute the same forces; results differ
  * only by FP32 summation order.  No counterpart in the reference (tuning
  * knob of this implementation). */
 int smr_batch_set_launch(smr_batch *b, int block, int stencil_cache, int pair_sweep);

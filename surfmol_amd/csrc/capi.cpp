@@ -1366,7 +1366,8 @@ extern "C" int smr_batch_autotune(smr_batch *b, const smr_fire_params *fire, int
     A.stencil = (float4 *)b->stencil.ptr;
     A.steps_per_launch = trial_steps;
     A.mode = KMODE_FIRE;
-    static const int shapes[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {128, 4}, {256, 1}, {256, 2}, {256, 4}};
+    static const int shapes[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {128, 4}, {256, 1}, {256, 2}, {256, 4},
+                                    {512, 1}, {512, 2}};
     LaunchSet ls;
     ls.use_graph = false;
     auto trial = [&](const ClassPlan &p, float &ms) -> int {
@@ -1443,8 +1444,8 @@ extern "C" int smr_batch_reset(smr_batch *b) {
 
 extern "C" int smr_batch_set_launch(smr_batch *b, int block, int stencil_cache, int pair_sweep) {
     if (!b) return fail(SMR_ERR_INVALID, "smr_batch_set_launch: NULL argument");
-    if (block != 0 && block != 64 && block != 128 && block != 256)
-        return fail(SMR_ERR_INVALID, "smr_batch_set_launch: block " + std::to_string(block) + " not in {0,64,128,256}");
+    if (block != 0 && block != 64 && block != 128 && block != 256 && block != 512)
+        return fail(SMR_ERR_INVALID, "smr_batch_set_launch: block " + std::to_string(block) + " not in {0,64,128,256,512}");
     if (stencil_cache < -1 || stencil_cache > 1)
         return fail(SMR_ERR_INVALID, "smr_batch_set_launch: stencil_cache must be -1, 0 or 1");
     if (pair_sweep < -1 || pair_sweep > 1)

@@ -71,8 +71,10 @@ static LaunchShape best_variant(const smr_ctx *c, const ClassPlan &cp, int block
 
 LaunchShape class_shape(const smr_ctx *c, const ClassPlan &cp, bool have_grid, int tune_block, int tune_cache,
                         int tune_pairs) {
-    static const int def_block[NCLASS] = {64, 128, 256, 256, 256};
-    static const int def_apt[NCLASS] = {1, 1, 1, 2, 4};
+    // one atom per thread up to 512 atoms: a large system gets 8 waves (two
+    // per SIMD of its CU) rather than 4 waves with 2 atoms per thread
+    static const int def_block[NCLASS] = {64, 128, 256, 512, 256};
+    static const int def_apt[NCLASS] = {1, 1, 1, 1, 4};
     // stored surface stencils: on unless switched off
     const bool cache = (tune_cache != 0) && have_grid;
     if (tune_block > 0) {
@@ -87,7 +89,17 @@ LaunchShape class_shape(const smr_ctx *c, const ClassPlan &cp, bool have_grid, i
         if (L.block) return L;
     }
     const int b = def_block[cp.cls], a = def_apt[cp.cls];
-    return best_variant(c, cp, b, a, cache, have_grid, tune_pairs);
+    LaunchShape L = best_variant(c, cp, b, a, cache, have_grid, tune_pairs);
+    if (L.block) return L;
+    // the default does not fit this class's LDS needs: any other shape that
+    // holds its atoms, fewest atoms per thread first
+    static const int fb[][2] = {{512, 1}, {256, 1}, {512, 2}, {256, 2}, {128, 2}, {256, 4}, {128, 4}, {64, 4}};
+    for (const auto &f : fb) {
+        if (f[0] * f[1] < cp.max_atoms) continue;
+        L = best_variant(c, cp, f[0], f[1], cache, have_grid, tune_pairs);
+        if (L.block) return L;
+    }
+    return L;
 }
 
 void build_plans(std::vector<SysSize> sys, std::vector<ClassPlan> &plans, std::vector<int> &work) {

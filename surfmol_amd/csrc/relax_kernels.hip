@@ -964,7 +964,8 @@ static const void *kernel_ptr() {
 }
 
 // instantiated shapes: (threads per workgroup, atoms per thread)
-#define SMR_SHAPES(X) X(64, 1) X(64, 2) X(64, 4) X(128, 1) X(128, 2) X(128, 4) X(256, 1) X(256, 2) X(256, 4)
+#define SMR_SHAPES(X) \
+    X(64, 1) X(64, 2) X(64, 4) X(128, 1) X(128, 2) X(128, 4) X(256, 1) X(256, 2) X(256, 4) X(512, 1) X(512, 2)
 
 hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t stream) {
 #define SMR_CASE(B, P)                                                                                   \

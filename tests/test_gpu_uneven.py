@@ -53,11 +53,11 @@ def _batch(ctx, surf, mols, X, **kw):
                    **kw)
 
 
-@pytest.mark.parametrize("block", [0, 64, 128])
+@pytest.mark.parametrize("block", [0, 64, 128, 512])
 @pytest.mark.parametrize("pair_sweep", [1, 0])
 def test_uneven_eval_forces(ctx, surf, oracle, params, pair_sweep, block):
-    """Every launch shape (64/128/256 threads x 1/2/4 atoms per thread, pair
-    sweep or ordered pairs) gives the FP64 forces and energies."""
+    """Every launch shape (64/128/256/512 threads x 1/2/4 atoms per thread,
+    pair sweep or ordered pairs) gives the FP64 forces and energies."""
     mols = _molecules(EVAL_SIZES, 0x5EED1001)
     X = _poses(mols, surf["spec"]["L"], 0x5EED1002)
     b = _batch(ctx, surf, mols, X)
@@ -69,7 +69,10 @@ def test_uneven_eval_forces(ctx, surf, oracle, params, pair_sweep, block):
     for p in plan:
         if pair_sweep == 0:
             assert not p["pair_sweep"]
-        if block and p["max_atoms"] <= 4 * block:
+        # a forced width is used wherever a shape of that width fits; 512
+        # threads with 2 atoms each stage tables that exceed the LDS of the
+        # largest class, which then falls back to 256 x 4
+        if block and p["max_atoms"] <= (block if block == 512 else 4 * block):
             assert p["block"] == block
     F, E = b.eval_forces()
     Xd = b.positions()
