@@ -281,6 +281,15 @@ int smr_batch_autotune(smr_batch *b, const smr_fire_params *fire, int trial_step
  * No counterpart in the reference. */
 int smr_batch_set_graph(smr_batch *b, int use_graph);
 
+/* Order in which smr_relax_batched starts the systems of a size class:
+ * 1 (default) larger systems first and, among systems of one size, highest
+ * FP64 energy per atom of the starting positions first
+ * (one energy evaluation before the launch; systems far from a minimum need
+ * the most FIRE steps, and starting them first shortens the launch's tail);
+ * 0 batch order.  Results do not depend on it.  No counterpart in the
+ * reference. */
+int smr_batch_set_order(smr_batch *b, int energy_order);
+
 /* Damped MD of MolWorld::run_md (crates/libs/surfmol/src/mol_world.rs:141-164,
  * moltopo/src/molecular.rs:118-137) for every system: clamp |F| to flim,
  * v = (1-damping) v + F dt, x += v dt, v = 0 when sum v.F < 0, stop when

@@ -412,6 +412,11 @@ class Batch:
         check(lib().smr_batch_autotune(self._h, ctypes.byref(fp), int(trial_steps)))
         return self.plan()
 
+    def set_order(self, energy_order: bool):
+        """True (default): start the systems of highest initial energy per
+        atom first in each size class; False: batch order."""
+        check(lib().smr_batch_set_order(self._h, 1 if energy_order else 0))
+
     def set_graph(self, use_graph: bool):
         check(lib().smr_batch_set_graph(self._h, 1 if use_graph else 0))
 

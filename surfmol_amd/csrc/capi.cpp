@@ -884,6 +884,7 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
     std::vector<int> wl;
     build_plans(sizes, b->plans, wl);
     refresh_shapes(b);
+    b->work_host = wl;
     e = hipMemcpyAsync(b->work.ptr, wl.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) { delete b; ctx_release(c); return hip_fail(e, "smr_batch_create: upload"); }
@@ -987,6 +988,45 @@ int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch,
 }
 
 // FP64 energies of every system at its current positions (into state[].E)
+// Longest-expected-first order of the work list: inside each size class the
+// larger systems come first (as built), and among systems of one size those
+// whose starting positions have the highest FP64 energy per atom (the
+// furthest from a minimum, which need the most FIRE steps), so that long
+// relaxations do not start in the tail of the persistent launch.  The order
+// affects only the schedule: every system's trajectory is its own.
+int order_by_energy(smr_batch *b) {
+    if (!b->energy_order || b->layer || b->nsys < 2) return SMR_OK;
+    bool any = false;
+    for (const auto &p : b->plans) any = any || p.count > p.shape.grid;
+    if (!any) return SMR_OK;   // every system starts at once anyway
+    hipStream_t st = b->ctx->stream;
+    hipError_t e = launch_energy64(b->tab, (const float4 *)b->pos.ptr, (SysState *)b->state.ptr,
+                                   (const int *)b->work.ptr, b->nsys, st);
+    std::vector<SysState> h(b->nsys);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), b->state.ptr, sizeof(SysState) * h.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "order_by_energy");
+    std::vector<double> key(b->nsys);
+    for (int s = 0; s < b->nsys; s++) {
+        double E = 0.0;
+        for (int q = 0; q < 6; q++) E += h[s].E[q];
+        const int n = b->atom_off[s + 1] - b->atom_off[s];
+        key[s] = std::isfinite(E) ? E / n : 1e300;
+    }
+    // larger systems first (the class's own order), then higher energy per atom
+    std::vector<int> w = b->work_host;
+    auto natoms = [&](int s) { return b->atom_off[s + 1] - b->atom_off[s]; };
+    for (const auto &p : b->plans)
+        std::stable_sort(w.begin() + p.first, w.begin() + p.first + p.count, [&](int x, int y) {
+            if (natoms(x) != natoms(y)) return natoms(x) > natoms(y);
+            return key[x] > key[y];
+        });
+    e = hipMemcpyAsync(b->work.ptr, w.data(), sizeof(int) * w.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "order_by_energy: work list");
+    return SMR_OK;
+}
+
 int eval_energies64(smr_batch *b) {
     hipStream_t st = b->ctx->stream;
     hipError_t e = launch_energy64(b->tab, (const float4 *)b->pos.ptr, (SysState *)b->state.ptr,
@@ -1172,6 +1212,8 @@ extern "C" int smr_relax_batched(smr_batch *b, const smr_fire_params *fire, int 
     HIPCHK(hipSetDevice(b->ctx->device));
     if (b->layer) return layer_relax(b, fire, max_steps, fconv, results);
     HIPCHK(hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, b->ctx->stream));
+    rc = order_by_energy(b);
+    if (rc) return rc;
     FireConst fc = make_fire_const(fire, max_steps, fconv);
     rc = run_fused(b, KMODE_FIRE, fc, max_steps, true, fire->dt0, fire->alpha0, &b->last_kernel_ms);
     if (rc) return rc;
@@ -1422,6 +1464,17 @@ extern "C" int smr_batch_autotune(smr_batch *b, const smr_fire_params *fire, int
     if (rc2 != SMR_OK) return rc2;
     if (e != hipSuccess) return hip_fail(e, "smr_batch_autotune: restore");
     refresh_shapes(b);
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_set_order(smr_batch *b, int energy_order) {
+    if (!b) return fail(SMR_ERR_INVALID, "smr_batch_set_order: NULL argument");
+    if (energy_order != 0 && energy_order != 1) return fail(SMR_ERR_INVALID, "smr_batch_set_order: mode must be 0 or 1");
+    b->energy_order = energy_order;
+    if (!energy_order) {
+        HIPCHK(hipSetDevice(b->ctx->device));
+        HIPCHK(hipMemcpy(b->work.ptr, b->work_host.data(), sizeof(int) * b->work_host.size(), hipMemcpyHostToDevice));
+    }
     return SMR_OK;
 }
 

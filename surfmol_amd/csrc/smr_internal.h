@@ -178,5 +178,7 @@ struct smr_batch {
     int tune_block = 0;              // 0: automatic
     int tune_cache = -1;             // -1: automatic
     int tune_pairs = -1;             // -1: automatic
+    int energy_order = 1;            // 1: start systems of higher initial energy per atom first
+    std::vector<int> work_host;      // the work list as uploaded (class order, atom count inside a class)
     float last_kernel_ms = 0.f;
 };
