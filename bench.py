@@ -109,7 +109,7 @@ def workload_uneven(args, S, W):
     desc = ("uneven batch: %d random fused-6-ring C/N/O/H molecules, atom count log-uniform [20,400], rotation "
             "U(0,2pi), xy U(0,48) A, height U(3,5) A, on a 48x48x20 A rock-salt GridFF at 0.1 A (480x480x200 "
             "nodes); UFF + intra-molecular LJ/Coulomb + GridFF B-spline; FIRE dt0 0.1 fs, dt_max 1 fs, "
-            "<= 2000 steps" % nsys)
+            "<= 2000 steps; launch shapes chosen by the autotuner per size class" % nsys)
     return mols, np.arange(nsys, dtype=np.int32), X, 48.0, desc
 
 
@@ -127,7 +127,8 @@ def main():
     ap.add_argument("--layer-spacing", type=float, default=13.0,
                     help="layer: square lattice spacing [A]; at 11 A the molecule has no flat arrangement with "
                          "contacts above 2.05 A and the layer does not relax (FP64 oracle and device agree)")
-    ap.add_argument("--autotune", action="store_true", help="time every launch shape per size class first")
+    ap.add_argument("--autotune", action="store_true", help="time every launch shape per size class first "
+                    "(always on for the uneven workload)")
     ap.add_argument("--plain-sample", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -177,7 +178,7 @@ def main():
     fire = S.FireParams(dt0=W.fs_to_internal(0.1), dt_max=W.fs_to_internal(1.0))
     fconv = 1e-3
     tuned = None
-    if args.autotune:
+    if args.autotune or args.workload == "uneven":
         t0 = time.time()
         tuned = batch.autotune(fire, 50)
         t_tune = time.time() - t0
@@ -372,6 +373,16 @@ def serve(args, S, W, alpha, rdamp):
         X = [W.place_on_surface(m, ang[s], xy[s], h[s]) for s, m in enumerate(mols)]
         hm = [S.Molecule.build(m.elems, X[s], m.charges, bonds=m.bonds) for s, m in enumerate(mols)]
         arrivals = W.poisson_arrivals(rate, args.requests, seed + 3)
+        # the plain method on the same requests: one request at a time, one
+        # launch sequence + host FIRE decision per step; its service times on
+        # a sample feed a FIFO single-server queue over the same arrivals
+        plain_t = []
+        for s in range(min(args.plain_sample, args.requests)):
+            pb = S.Batch(ctx, [hm[s]], [0], X[s], grid=grid, rdamp=rdamp)
+            tq = time.perf_counter()
+            pb.relax_plain(0, fire, args.max_steps, 1e-3)
+            plain_t.append(time.perf_counter() - tq)
+            pb.close()
         srv = S.Server(ctx, grid, fire, max_resident=4096, max_atoms=600, hbm_fraction=0.25,
                        chunk_steps=args.chunk_steps, max_steps=args.max_steps, fconv=1e-3, rdamp=rdamp)
         ids = []
@@ -389,6 +400,18 @@ def serve(args, S, W, alpha, rdamp):
         conv = sum(1 for r in res if r["status"] == S.SMR_SYS_CONVERGED)
         span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
         steps = np.array([r["steps"] for r in res])
+        free, plat = 0.0, []
+        for s in range(args.requests):
+            start = max(free, arrivals[s])
+            free = start + plain_t[s % len(plain_t)]
+            plat.append(free - arrivals[s])
+        plat = np.array(plat)
+        plain = {"p50_ms": round(1e3 * float(np.percentile(plat, 50)), 1),
+                 "p99_ms": round(1e3 * float(np.percentile(plat, 99)), 1),
+                 "served_per_s": round(args.requests / (free - arrivals[0]), 2),
+                 "service_ms_mean": round(1e3 * float(np.mean(plain_t)), 2),
+                 "model": f"FIFO single-server queue over the request arrivals with the plain method's service "
+                          f"times measured on the first {len(plain_t)} requests"}
         per_rate.append({
             "rate_req_s": rate, "requests": args.requests, "converged": conv,
             "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
@@ -399,6 +422,8 @@ def serve(args, S, W, alpha, rdamp):
             "atoms_mean": round(float(np.mean(sizes)), 1),
             "chunks": int(st["chunks"]), "peak_resident": int(st["peak_resident"]),
             "device_busy_frac": round(st["device_s"] / span, 3),
+            "plain": plain,
+            "served_vs_plain": round((conv / span) / plain["served_per_s"], 1),
         })
         print(json.dumps({"progress": per_rate[-1]}), file=sys.stderr, flush=True)
     out = {
