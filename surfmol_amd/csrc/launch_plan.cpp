@@ -180,9 +180,10 @@ int prepare_plans(LaunchSet &ls, const std::vector<ClassPlan> &plans, const Kern
     for (size_t q = 0; q < plans.size(); q++) {
         const ClassPlan &p = plans[q];
         if (p.shape.block == 0)
-            return fail(SMR_ERR_UNSUPPORTED, "k_relax: no launch shape fits " + std::to_string(p.max_atoms) +
-                                                 " atoms with " + std::to_string(p.nf_max) +
-                                                 " force pieces in the device's LDS");
+            return fail(SMR_ERR_UNSUPPORTED, "k_relax: no launch shape allowed by the launch settings fits " +
+                                                 std::to_string(p.max_atoms) + " atoms with " + std::to_string(p.nf_max) +
+                                                 " force pieces in the device's LDS (smr_batch_set_launch with "
+                                                 "pair_sweep = 1 demands per-wave pair buffers)");
         off[q] = need;
         if (!p.shape.fint_lds) need += (size_t)3 * std::max(1, p.nf_max) * p.shape.grid;
     }
