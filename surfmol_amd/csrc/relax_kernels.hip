@@ -91,6 +91,11 @@ __device__ void layer_decide(const KernelArgs &A, double *s_red) {
     }
 }
 
+// Value of lane (lane+1) mod 64 (DPP wave_rol:1).
+__device__ __forceinline__ float wave_rol1(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x134, 0xf, 0xf, true));
+}
+
 // Value of lane (lane+1) mod (last+1) for lanes 0..last: the first last+1
 // lanes rotated by one towards lane 0 (DPP wave_shl:1, then lane `last`
 // takes lane 0's value).  Lanes above `last` receive garbage.
@@ -370,12 +375,20 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                 // The same sweep with the reaction sum of atom j = (i+d) mod N
                 // held by lane i: on sweep d+1 atom (i+d+1) mod N is the
                 // target of lane i+1 on sweep d, so the accumulators rotate
-                // by one lane per sweep (each atom's reaction terms are added
-                // in sweep order, as in the LDS variant).  Afterwards lane
-                // (i - nd) mod N holds atom i's sum.
+                // by one lane per sweep.
+                //  - N + N/2 <= 64: the accumulators rotate over all 64 lanes
+                //    (DPP wave_rol:1, no fix-up).  The accumulator that starts
+                //    in lane a collects the reactions of "virtual atom" a+1;
+                //    own lanes reach virtual atoms i+d <= N + N/2 - 1 < 64, so
+                //    nothing wraps, and atom k's reaction is the sum of its
+                //    virtual atoms k (pairs i < k) and k+N (pairs i > k, only
+                //    for k < N/2).  After the N/2 - 1 rotations accumulator a
+                //    sits in lane (a - N/2 + 1) mod 64.
+                //  - larger N: the rotation wraps at N (wave_rot1), and lane
+                //    (i - N/2) mod N ends with atom i's sum.
                 // The loop body is branch-free: the pair force is computed in
                 // every lane and zeroed where the lane has no atom, the pair
-                // is bonded (bit d of `skip`: the 1-2 exclusions of the own
+                // is bonded (bit d-1 of `skip`: the 1-2 exclusions of the own
                 // atom's sweep) or, on the last sweep of an even N, the pair
                 // belongs to the lower half.
                 const bool own = tid < N;
@@ -383,46 +396,67 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                 const bool even = (N & 1) == 0;
                 const float4 pi4 = own ? s_pos[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
                 const float4 ri = own ? s_req[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
-                unsigned long long skip = own ? 0ull : ~0ull;
+                // bit d-1: sweep d is skipped (nd <= 32)
+                unsigned skip = own ? 0u : ~0u;
                 {
                     const int ngs[4] = {my_ng[0].x, my_ng[0].y, my_ng[0].z, my_ng[0].w};
 #pragma unroll
                     for (int sl = 0; sl < 4; sl++) {
                         int dd = ngs[sl] - tid;
                         if (dd < 0) dd += N;
-                        if (ngs[sl] >= 0 && dd <= nd) skip |= 1ull << dd;
+                        if (ngs[sl] >= 0 && dd >= 1 && dd <= nd) skip |= 1u << (dd - 1);
                     }
-                    if (even && tid >= nd) skip |= 1ull << nd;
+                    if (even && tid >= nd && nd >= 1) skip |= 1u << (nd - 1);
                 }
                 KahanF3 pacc;
                 pacc.init(make_float3(0.f, 0.f, 0.f));
-                int j = own ? tid : 0;
-                int jn = (j + 1 == N) ? 0 : j + 1;
-                float4 pj = s_pos[jn], rj = s_req[jn];
-                for (int d = 1; d <= nd; d++) {
-                    j = jn;
-                    const float4 pc = pj, rc = rj;
-                    jn = (j + 1 == N) ? 0 : j + 1;
-                    if (d < nd) {
-                        pj = s_pos[jn];
-                        rj = s_req[jn];
+                // byte offset of the partner's row in s_pos / s_req, wrapping at N
+                const unsigned jend = 16u * (unsigned)N;
+                unsigned ja = own ? 16u * (unsigned)tid : 0u;
+                ja = (ja + 16u == jend) ? 0u : ja + 16u;
+                const char *pb = reinterpret_cast<const char *>(s_pos);
+                const char *rb = reinterpret_cast<const char *>(s_req);
+                float4 pj = *reinterpret_cast<const float4 *>(pb + ja), rj = *reinterpret_cast<const float4 *>(rb + ja);
+                // (the prefetch of the next row is unconditional: the offset
+                // wraps inside the N rows, so the last sweep's read is in bounds)
+                auto sweep = [&](auto rotate) {
+#pragma unroll 2
+                    for (int d = 1; d <= nd; d++) {
+                        const float4 pc = pj, rc = rj;
+                        ja = (ja + 16u == jend) ? 0u : ja + 16u;
+                        pj = *reinterpret_cast<const float4 *>(pb + ja);
+                        rj = *reinterpret_cast<const float4 *>(rb + ja);
+                        if (d > 1) rotate();
+                        const float3 dp = f3(pc) - f3(pi4);
+                        float fr = ljq_pair_fr(dp, ri, rc, T.r2damp);
+                        if ((skip >> (d - 1)) & 1u) fr = 0.f;
+                        const float3 f = dp * fr;
+                        pacc.add(f);
+                        rr.x -= f.x; rr.y -= f.y; rr.z -= f.z;
                     }
-                    if (d > 1) {
+                };
+                if (N + nd <= 64) {
+                    sweep([&]() {
+                        rr.x = wave_rol1(rr.x);
+                        rr.y = wave_rol1(rr.y);
+                        rr.z = wave_rol1(rr.z);
+                    });
+                    const int s1 = (tid - nd) & 63, s2 = (tid + N - nd) & 63;
+                    const float3 r1 = make_float3(__shfl(rr.x, s1, 64), __shfl(rr.y, s1, 64), __shfl(rr.z, s1, 64));
+                    const float3 r2 = make_float3(__shfl(rr.x, s2, 64), __shfl(rr.y, s2, 64), __shfl(rr.z, s2, 64));
+                    rr = tid < nd ? r1 + r2 : r1;
+                } else {
+                    sweep([&]() {
                         rr.x = wave_rot1(rr.x, N - 1, lane);
                         rr.y = wave_rot1(rr.y, N - 1, lane);
                         rr.z = wave_rot1(rr.z, N - 1, lane);
-                    }
-                    float3 f;
-                    (void)ljq_pair(f3(pc) - f3(pi4), ri, rc, T.r2damp, f);
-                    if ((skip >> d) & 1ull) f = make_float3(0.f, 0.f, 0.f);
-                    pacc.add(f);
-                    rr.x -= f.x; rr.y -= f.y; rr.z -= f.z;
+                    });
+                    int src = tid - nd;
+                    if (src < 0) src += N;
+                    rr.x = __shfl(rr.x, src, 64);
+                    rr.y = __shfl(rr.y, src, 64);
+                    rr.z = __shfl(rr.z, src, 64);
                 }
-                int src = tid - nd;
-                if (src < 0) src += N;
-                rr.x = __shfl(rr.x, src, 64);
-                rr.y = __shfl(rr.y, src, 64);
-                rr.z = __shfl(rr.z, src, 64);
                 pf[0] = pacc.value();
             } else if (A.pair_mode && T.use_pairs) {
                 float4 *fpw = s_fp + wid * A.lds_atoms;

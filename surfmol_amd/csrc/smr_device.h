@@ -254,6 +254,20 @@ __device__ __forceinline__ float ljq_pair(float3 dp, float4 ri, float4 rj, float
     return E;
 }
 
+// The same pair force as a radial factor: the force on i is dp * fr (no energy).
+__device__ __forceinline__ float ljq_pair_fr(float3 dp, float4 ri, float4 rj, float r2damp) {
+    const float R0 = ri.x + rj.x;
+    const float E0 = ri.y * rj.y;
+    const float kQ = ri.z * rj.z;
+    const float r2 = dot3(dp, dp);
+    const float ir_ = frsq(r2 + r2damp);
+    const float ir2 = frcp(r2);
+    const float u2 = R0 * R0 * ir2;
+    const float u6 = u2 * u2 * u2;
+    const float vdW = u6 * E0;
+    return -12.0f * (u6 - 1.0f) * vdW * ir2 - (kQ * ir_) * (ir_ * ir_);
+}
+
 // ---------------------------------------------------------------------------
 // GridFF tricubic B-spline sample: fe3d_pbc_comb (gridff_eval.cl:234-286)
 // ---------------------------------------------------------------------------
