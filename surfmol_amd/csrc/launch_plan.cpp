@@ -187,6 +187,7 @@ int prepare_plans(LaunchSet &ls, const std::vector<ClassPlan> &plans, const Kern
         off[q] = need;
         if (!p.shape.fint_lds) need += (size_t)3 * std::max(1, p.nf_max) * p.shape.grid;
     }
+    if (need) need += kPieceTail;   // the gather's reads past the last class's last piece
     if (need && ls.scratch.bytes < need * sizeof(float)) {
         ls.drop_graph();
         if ((e = ls.scratch.alloc(need * sizeof(float))) != hipSuccess) return hip_fail(e, "force-piece scratch");

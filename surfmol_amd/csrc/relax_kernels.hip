@@ -536,16 +536,17 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                         }
                     }
                     // the atom's pieces are contiguous (atom-major order); eight
-                    // at a time, so that their LDS latencies overlap; the
-                    // adds keep the list order
+                    // at a time from one base, so that their LDS latencies
+                    // overlap; the adds keep the list order.  A round reads up
+                    // to 7 entries past the atom's last piece: they lie in the
+                    // next atom's pieces, the next array or the 8-float tail
+                    // that every force-piece area carries (index < 3*nf_max + 8)
                     const int pb = my_a2f[k].x, cnt = my_a2f[k].y;
                     for (int q = 0; q < cnt; q += 8) {
+                        const float *px = fint_x + pb + q, *py = fint_y + pb + q, *pz = fint_z + pb + q;
                         float3 pv[8];
 #pragma unroll
-                        for (int u = 0; u < 8; u++) {
-                            const int pp = pb + min(q + u, cnt - 1);
-                            pv[u] = make_float3(fint_x[pp], fint_y[pp], fint_z[pp]);
-                        }
+                        for (int u = 0; u < 8; u++) pv[u] = make_float3(px[u], py[u], pz[u]);
 #pragma unroll
                         for (int u = 0; u < 8; u++)
                             if (q + u < cnt) acc.add(pv[u]);

@@ -336,6 +336,18 @@ struct GridCell {
     float tx, ty, tz;
 };
 
+// i mod n in [0, n): one add or subtract for |i| within a period of the
+// range (the usual case), an integer division only beyond that
+__device__ __forceinline__ int wrap_index(int i, int n) {
+    if (i >= n) i -= n;
+    else if (i < 0) i += n;
+    if ((unsigned)i >= (unsigned)n) {
+        i %= n;
+        if (i < 0) i += n;
+    }
+    return i;
+}
+
 __device__ __forceinline__ GridCell grid_cell(int nx, int ny, int nz, int ix0, int iy0, float3 ul) {
     GridCell g;
     float fx = floorf(ul.x), fy = floorf(ul.y);
@@ -350,8 +362,8 @@ __device__ __forceinline__ GridCell grid_cell(int nx, int ny, int nz, int ix0, i
     }
     int ix = ix0 + (int)fx - 1;
     int iy = iy0 + (int)fy - 1;
-    ix %= nx; if (ix < 0) ix += nx;
-    iy %= ny; if (iy < 0) iy += ny;
+    ix = wrap_index(ix, nx);
+    iy = wrap_index(iy, ny);
     g.ix = ix;
     g.iy = iy;
     g.iz = iz - 1;

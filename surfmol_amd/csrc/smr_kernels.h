@@ -91,6 +91,9 @@ inline int relax_lds_atoms(int block, int max_atoms) { return max_atoms > 16 * (
 // reduction scratch, packed interaction slots and piece destinations (not
 // for 4 atoms per thread: those tables stay in global memory), and the force
 // pieces (optional)
+// floats after a force-piece area that the gather may read (not use)
+constexpr int kPieceTail = 8;
+
 inline size_t relax_lds_bytes(int block, int apt, int lds_atoms, int nf_max, int na, int nd, int ni, bool fint_lds,
                               bool pair_mode) {
     const int nw = block / 64;
@@ -98,7 +101,9 @@ inline size_t relax_lds_bytes(int block, int apt, int lds_atoms, int nf_max, int
     const bool tables = apt < 4;
     // slots (8, 8, 4 B) and piece destinations (8 B) per dihedral, inversion, angle
     if (tables) b += (size_t)nd * 16 + (size_t)ni * 16 + (size_t)na * 12;
-    if (fint_lds) b += (size_t)nf_max * 12;
+    // force pieces (x, y, z arrays of nf_max) and the 8-float tail that the
+    // gather's rounds of eight may read past the last piece
+    if (fint_lds) b += (size_t)nf_max * 12 + kPieceTail * sizeof(float);
     return b;
 }
 
