@@ -537,7 +537,11 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                     }
                     // the atom's pieces are contiguous (atom-major order); eight
                     // at a time from one base, so that their LDS latencies
-                    // overlap; the adds keep the list order.  A round reads up
+                    // overlap.  Each round's pieces are added in list order in
+                    // FP32 and the round's sum goes into the compensated
+                    // per-atom sum (the rounding error of a round is a few ulp
+                    // of its pieces; across rounds it stays compensated).
+                    // A round reads up
                     // to 7 entries past the atom's last piece: they lie in the
                     // next atom's pieces, the next array or the 8-float tail
                     // that every force-piece area carries (index < 3*nf_max + 8)
@@ -547,9 +551,11 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                         float3 pv[8];
 #pragma unroll
                         for (int u = 0; u < 8; u++) pv[u] = make_float3(px[u], py[u], pz[u]);
+                        float3 rs8 = pv[0];
 #pragma unroll
-                        for (int u = 0; u < 8; u++)
-                            if (q + u < cnt) acc.add(pv[u]);
+                        for (int u = 1; u < 8; u++)
+                            if (q + u < cnt) rs8 = rs8 + pv[u];
+                        acc.add(rs8);
                     }
                     const float4 pi4 = s_pos[i];
                     const float3 pi = f3(pi4);
