@@ -419,20 +419,28 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                 float4 pj = *reinterpret_cast<const float4 *>(pb + ja), rj = *reinterpret_cast<const float4 *>(rb + ja);
                 // (the prefetch of the next row is unconditional: the offset
                 // wraps inside the N rows, so the last sweep's read is in bounds)
+                // The own-atom forces of four sweeps are added in FP32 and
+                // their sum goes into the compensated sum.
                 auto sweep = [&](auto rotate) {
-#pragma unroll 2
-                    for (int d = 1; d <= nd; d++) {
-                        const float4 pc = pj, rc = rj;
-                        ja = (ja + 16u == jend) ? 0u : ja + 16u;
-                        pj = *reinterpret_cast<const float4 *>(pb + ja);
-                        rj = *reinterpret_cast<const float4 *>(rb + ja);
-                        if (d > 1) rotate();
-                        const float3 dp = f3(pc) - f3(pi4);
-                        float fr = ljq_pair_fr(dp, ri, rc, T.r2damp);
-                        if ((skip >> (d - 1)) & 1u) fr = 0.f;
-                        const float3 f = dp * fr;
-                        pacc.add(f);
-                        rr.x -= f.x; rr.y -= f.y; rr.z -= f.z;
+                    for (int d0 = 1; d0 <= nd; d0 += 4) {
+                        float3 bs = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const int d = d0 + u;
+                            if (d > nd) break;
+                            const float4 pc = pj, rc = rj;
+                            ja = (ja + 16u == jend) ? 0u : ja + 16u;
+                            pj = *reinterpret_cast<const float4 *>(pb + ja);
+                            rj = *reinterpret_cast<const float4 *>(rb + ja);
+                            if (d > 1) rotate();
+                            const float3 dp = f3(pc) - f3(pi4);
+                            float fr = ljq_pair_fr(dp, ri, rc, T.r2damp);
+                            if ((skip >> (d - 1)) & 1u) fr = 0.f;
+                            const float3 f = dp * fr;
+                            bs = bs + f;
+                            rr.x -= f.x; rr.y -= f.y; rr.z -= f.z;
+                        }
+                        pacc.add(bs);
                     }
                 };
                 if (N + nd <= 64) {
