@@ -300,7 +300,7 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
             }
         };
         const int nsteps = A.steps_per_launch;
-        double fmax_cur = st.fmax;
+        double fm2_cur = st.fmax * st.fmax;   // max |F|^2 of the last step
         int step_in_launch = 0;
         for (;;) {
             // ---------------- phase A: bonds + hneigh ----------------
@@ -748,9 +748,11 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
             if (A.mode == KMODE_FIRE) {
                 // FIRE decision (relax_pentacene_uff.rs:162-181)
                 if (p_vf > 0.0 && p_ff > 1e-30 && p_vv > 1e-30) {
-                    const double a = 1.0 - st.alpha;
-                    const double b = st.alpha * sqrt(p_vv) / sqrt(p_ff);
-                    const float af = (float)a, bf = (float)b;
+                    // the mixing factors are applied in FP32: alpha |v| / |F|
+                    // from the FP64 sums with FP32 roots
+                    const float af = (float)(1.0 - st.alpha);
+                    const float vvf = (float)p_vv;
+                    const float bf = (float)st.alpha * (vvf * frsq(vvf)) * frsq((float)p_ff);
 #pragma unroll
                     for (int k = 0; k < APT; k++) my_v[k] = my_v[k] * af + F[k] * bf;
                     st.npos++;
@@ -791,12 +793,13 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
             }
             st.steps++;
             step_in_launch++;
-            fmax_cur = sqrt(p_fm);
+            fm2_cur = p_fm;
             bool done = false;
             if (nonfinite) {
                 st.status = SMR_SYS_NONFINITE;
                 done = true;
-            } else if (A.mode == KMODE_FIRE ? (fmax_cur < A.fire.fconv) : (p_ff < A.fire.fconv * A.fire.fconv)) {
+            } else if (A.mode == KMODE_FIRE ? (fm2_cur < A.fire.fconv * A.fire.fconv)
+                                            : (p_ff < A.fire.fconv * A.fire.fconv)) {
                 st.status = SMR_SYS_CONVERGED;
                 done = true;
             } else if (st.steps >= A.fire.max_steps) {
@@ -808,7 +811,7 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                 // the energies of the final positions are evaluated in FP64
                 // afterwards (k_energy64)
                 if (tid == 0) {
-                    st.fmax = fmax_cur;
+                    st.fmax = sqrt(fm2_cur);
                     A.state[s] = st;
                 }
 #pragma unroll
