@@ -303,6 +303,13 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
         double fm2_cur = st.fmax * st.fmax;   // max |F|^2 of the last step
         int step_in_launch = 0;
         for (;;) {
+            // parameters of each thread's first angle, dihedral and inversion
+            // (read through the caches), issued before the bonds so that their
+            // latency overlaps phase A; each term loop prefetches the next
+            const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 apar = tid < td.nangles ? T.ang_par[td.ang_off + tid] : zero4;
+            float4 dpar = tid < td.ndih ? T.dih_par[td.dih_off + tid] : zero4;
+            float4 ipar = tid < td.ninv ? T.inv_par[td.inv_off + tid] : zero4;
             // ---------------- phase A: bonds + hneigh ----------------
             if (!ROT && A.pair_mode)
                 for (int w = 0; w < NWP; w++)
@@ -338,24 +345,29 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                 for (int a = tid; a < td.nangles; a += BLOCK) {
                     const unsigned g = t_ang[a];
                     const uint2 dd = t_angd[a];
+                    const float4 par = apar;
+                    if (a + BLOCK < td.nangles) apar = T.ang_par[td.ang_off + a + BLOCK];
                     float3 fpi, fpj, fpk;
-                    (void)angle_term(s_h[g & 0xffffu], s_h[g >> 16], T.ang_par[td.ang_off + a], fpi, fpj, fpk);
+                    (void)angle_term(s_h[g & 0xffffu], s_h[g >> 16], par, fpi, fpj, fpk);
                     put(dd.x & 0xffffu, fpi); put(dd.x >> 16, fpj); put(dd.y & 0xffffu, fpk);
                 }
                 for (int d = tid; d < td.ndih; d += BLOCK) {
                     const uint2 g = t_dih[d];
                     const uint2 dd = t_dihd[d];
+                    const float4 par = dpar;
+                    if (d + BLOCK < td.ndih) dpar = T.dih_par[td.dih_off + d + BLOCK];
                     float3 f1, f2, f3v, f4;
-                    (void)dihedral_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y & 0xffffu], T.dih_par[td.dih_off + d],
-                                            (int)(g.y >> 16), f1, f2, f3v, f4);
+                    (void)dihedral_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y & 0xffffu], par, (int)(g.y >> 16), f1,
+                                        f2, f3v, f4);
                     put(dd.x & 0xffffu, f1); put(dd.x >> 16, f2); put(dd.y & 0xffffu, f3v); put(dd.y >> 16, f4);
                 }
                 for (int v = tid; v < td.ninv; v += BLOCK) {
                     const uint2 g = t_inv[v];
                     const uint2 dd = t_invd[v];
+                    const float4 par = ipar;
+                    if (v + BLOCK < td.ninv) ipar = T.inv_par[td.inv_off + v + BLOCK];
                     float3 f1, f2, f3v, f4;
-                    (void)inversion_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y], T.inv_par[td.inv_off + v], f1, f2,
-                                             f3v, f4);
+                    (void)inversion_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y], par, f1, f2, f3v, f4);
                     put(dd.x & 0xffffu, f1); put(dd.x >> 16, f2); put(dd.y & 0xffffu, f3v); put(dd.y >> 16, f4);
                 }
             }
