@@ -257,12 +257,14 @@ int smr_batch_set_launch(smr_batch *b, int block, int stencil_cache, int pair_sw
  * measured by smr_relax_batched).  No counterpart in the reference. */
 int smr_batch_last_launch(const smr_batch *b, int *shape4, float *kernel_ms);
 
-/* Size classes of the batch and their launch shapes.  Systems are grouped
- * by atom count (<= 64, 128, 256, 512, 1024); each class is one persistent
- * launch.  *nclass receives the number of non-empty classes and info
- * [8 * nclass] per class: (class id, systems, max atoms, threads per
- * workgroup, atoms per thread, workgroups, LDS bytes, flags: 1 force pieces
- * in LDS, 2 unordered-pair sweep, 4 stencil cache).  No counterpart in the
+/* Persistent launches of the batch and their shapes.  Systems are grouped
+ * by atom count into size classes (<= 64, 128, 256, 512, 1024), each with
+ * its own launch shape; adjacent classes whose shapes agree share one
+ * launch when it keeps the resident workgroups of each.  *nclass receives
+ * the number of launches (at most 5) and info [8 * nclass] per launch:
+ * (largest class id in it, systems, max atoms, threads per workgroup, atoms
+ * per thread, workgroups, LDS bytes, flags: 1 force pieces in LDS,
+ * 2 unordered-pair sweep, 4 stencil cache).  No counterpart in the
  * reference (its multi-system ABI has one fixed work-group shape,
  * opencl/relax_multi.cl:425-443). */
 int smr_batch_plan(const smr_batch *b, int *nclass, int *info);

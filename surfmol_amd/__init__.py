@@ -394,7 +394,9 @@ class Batch:
                 "stencil_cache": bool(shape[3] >> 30), "kernel_ms": ms.value}
 
     def plan(self):
-        """Size classes of the batch: one dict per class with its launch shape."""
+        """Persistent launches of the batch: one dict per launch with its shape
+        (adjacent size classes of one shape share a launch; "cls" is the
+        largest class in it)."""
         n = ctypes.c_int()
         info = (ctypes.c_int * (8 * 8))()
         check(lib().smr_batch_plan(self._h, ctypes.byref(n), info))

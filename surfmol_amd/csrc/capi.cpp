@@ -616,10 +616,48 @@ void system_origin(const double *x, int n, const smr_gridff *grid, bool surface,
     }
 }
 
-// launch shapes of a batch's classes (after a change of the tuning knobs)
+// workgroups of a shape that the device holds at once
+static long long shape_capacity(const smr_ctx *c, ClassPlan p, const LaunchShape &L, bool have_grid) {
+    p.count = 1 << 30;
+    const LaunchShape m = make_shape(c, p, L.block, L.apt, L.fint_lds, L.pair_mode, L.stencil_cache, have_grid);
+    return m.block ? m.grid : 0;
+}
+
+// launch shapes of a batch's classes (after a change of the tuning knobs),
+// then the launches: adjacent classes (contiguous slices of the work list)
+// whose shapes agree share one persistent launch when the merged launch
+// keeps the resident workgroups of each (so the larger class's long
+// relaxations start first and the smaller class fills in, instead of two
+// launches each ending in its own tail)
 void refresh_shapes(smr_batch *b) {
-    for (auto &p : b->plans)
-        p.shape = class_shape(b->ctx, p, b->grid != nullptr, b->tune_block, b->tune_cache, b->tune_pairs);
+    const bool g = b->grid != nullptr;
+    for (auto &p : b->class_plans)
+        p.shape = class_shape(b->ctx, p, g, b->tune_block, b->tune_cache, b->tune_pairs);
+    b->plans.clear();
+    for (const auto &p : b->class_plans) {
+        if (!b->plans.empty() && p.shape.block) {
+            ClassPlan &q = b->plans.back();
+            const LaunchShape &a = q.shape, &c = p.shape;
+            if (a.block == c.block && a.apt == c.apt && a.fint_lds == c.fint_lds && a.pair_mode == c.pair_mode &&
+                a.stencil_cache == c.stencil_cache && q.first + q.count == p.first) {
+                ClassPlan m = q;
+                m.count += p.count;
+                m.max_atoms = std::max(q.max_atoms, p.max_atoms);
+                m.nf_max = std::max(q.nf_max, p.nf_max);
+                m.na_max = std::max(q.na_max, p.na_max);
+                m.nd_max = std::max(q.nd_max, p.nd_max);
+                m.ni_max = std::max(q.ni_max, p.ni_max);
+                m.shape = make_shape(b->ctx, m, a.block, a.apt, a.fint_lds, a.pair_mode, a.stencil_cache, g);
+                if (m.shape.block &&
+                    shape_capacity(b->ctx, m, m.shape, g) >=
+                        std::max(shape_capacity(b->ctx, q, a, g), shape_capacity(b->ctx, p, c, g))) {
+                    q = m;
+                    continue;
+                }
+            }
+        }
+        b->plans.push_back(p);
+    }
     b->ls.drop_graph();
 }
 
@@ -882,7 +920,7 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
         sizes[s] = SysSize{s, td.natoms, td.nf, td.nangles, td.ndih, td.ninv};
     }
     std::vector<int> wl;
-    build_plans(sizes, b->plans, wl);
+    build_plans(sizes, b->class_plans, wl);
     refresh_shapes(b);
     b->work_host = wl;
     e = hipMemcpyAsync(b->work.ptr, wl.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, c->stream);
@@ -1420,7 +1458,7 @@ extern "C" int smr_batch_autotune(smr_batch *b, const smr_fire_params *fire, int
         return launch_plans(c->ss, ls, one, A, &ms);
     };
     int rc2 = SMR_OK;
-    for (auto &p : b->plans) {
+    for (auto &p : b->class_plans) {
         const int cap0 = kClassMaxAtoms[p.cls];
         std::vector<LaunchShape> cand;
         for (auto &sh : shapes) {

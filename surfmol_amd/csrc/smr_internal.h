@@ -173,7 +173,8 @@ struct smr_batch {
     smr::DevBuf partials, layer_state;
     smr::CellGrid cells;
     smr::DevBuf plain_hneigh, plain_fint, plain_e, plain_cvf;
-    std::vector<smr::ClassPlan> plans;
+    std::vector<smr::ClassPlan> class_plans;   // one per size class (shapes chosen per class)
+    std::vector<smr::ClassPlan> plans;         // launches: adjacent classes of one shape merged
     smr::LaunchSet ls;
     int tune_block = 0;              // 0: automatic
     int tune_cache = -1;             // -1: automatic

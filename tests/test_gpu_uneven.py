@@ -65,7 +65,9 @@ def test_uneven_eval_forces(ctx, surf, oracle, params, pair_sweep, block):
     plan = b.plan()
     print([(p["cls"], p["systems"], p["max_atoms"], p["block"], p["apt"], p["lds_bytes"], p["pair_sweep"],
             p["fint_lds"]) for p in plan])
-    assert sorted(p["cls"] for p in plan) == [0, 1, 2, 3, 4]
+    # every system in exactly one launch (adjacent classes of one shape may share one)
+    assert sum(p["systems"] for p in plan) == len(mols)
+    assert len({p["cls"] for p in plan}) == len(plan) and {p["cls"] for p in plan} <= {0, 1, 2, 3, 4}
     for p in plan:
         if pair_sweep == 0:
             assert not p["pair_sweep"]
@@ -167,7 +169,8 @@ def test_autotune_same_basins(ctx, surf):
     plan = b.autotune(fp, trial_steps=20)
     print(plan)
     ncls = len({min(k for k in range(5) if m.natoms <= (64 << k)) for m in mols})
-    assert len(plan) == ncls and all(p["block"] > 0 for p in plan)
+    assert 1 <= len(plan) <= ncls and all(p["block"] > 0 for p in plan)
+    assert sum(p["systems"] for p in plan) == len(mols)
     r1 = b.relax_batched(fp, 3000, 1e-3)
     for m, a, c in zip(mols, r0, r1):
         print(m.natoms, a["status"], a["steps"], a["energy"], c["status"], c["steps"], c["energy"])
