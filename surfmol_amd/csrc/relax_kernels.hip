@@ -492,29 +492,45 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                     const int4 ng = my_ng[k];
                     KahanF3 pacc;
                     pacc.init(make_float3(0.f, 0.f, 0.f));
-                    // positions and REQ rows do not change during the sweep:
-                    // the next partner's rows are loaded before this pair's
-                    // reaction store, hiding their LDS latency
+                    // Positions and REQ rows do not change during the sweep:
+                    // the next partner's rows are loaded one sweep ahead (the
+                    // index wraps inside the N rows, so the read is in
+                    // bounds).  Within a sweep the partners are distinct, so
+                    // the reaction slot is read at the top of the iteration
+                    // (after the previous sweep's store) and its latency
+                    // overlaps the pair.  Branch-free: the radial factor is
+                    // zeroed for lanes without an atom, bonded partners and
+                    // the upper half of an even N's last sweep.  The own
+                    // forces of four sweeps are added in FP32 before the
+                    // compensated add.
                     int j = own ? i : 0;
                     int jn = (j + 1 == N) ? 0 : j + 1;
                     float4 pj = s_pos[jn], rj = s_req[jn];
-                    for (int d = 1; d <= nd; d++) {
-                        j = jn;
-                        const float4 pc = pj, rc = rj;
-                        jn = (j + 1 == N) ? 0 : j + 1;
-                        if (d < nd) {
+                    for (int d0 = 1; d0 <= nd; d0 += 4) {
+                        float3 bs = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const int d = d0 + u;
+                            if (d > nd) break;
+                            j = jn;
+                            const float4 pc = pj, rc = rj;
+                            jn = (j + 1 == N) ? 0 : j + 1;
                             pj = s_pos[jn];
                             rj = s_req[jn];
-                        }
-                        const bool active = own && !(even && d == nd && i >= nd);
-                        if (active && !(j == ng.x || j == ng.y || j == ng.z || j == ng.w)) {
-                            float3 f;
-                            (void)ljq_pair(f3(pc) - f3(pi4), ri, rc, T.r2damp, f);
-                            pacc.add(f);
                             float4 fj = fpw[j];
-                            fj.x -= f.x; fj.y -= f.y; fj.z -= f.z;
-                            fpw[j] = fj;
+                            const float3 dp = f3(pc) - f3(pi4);
+                            float fr = ljq_pair_fr(dp, ri, rc, T.r2damp);
+                            const bool skip = !own || (even && d == nd && i >= nd) || j == ng.x || j == ng.y ||
+                                              j == ng.z || j == ng.w;
+                            if (skip) fr = 0.f;
+                            const float3 f = dp * fr;
+                            bs = bs + f;
+                            if (own) {
+                                fj.x -= f.x; fj.y -= f.y; fj.z -= f.z;
+                                fpw[j] = fj;
+                            }
                         }
+                        pacc.add(bs);
                     }
                     pf[k] = pacc.value();
                 }
