@@ -129,6 +129,8 @@ def main():
                          "contacts above 2.05 A and the layer does not relax (FP64 oracle and device agree)")
     ap.add_argument("--autotune", action="store_true", help="time every launch shape per size class first "
                     "(always on for the uneven workload)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="issue the launches directly instead of replaying captured HIP graphs (same results)")
     ap.add_argument("--plain-sample", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -177,6 +179,8 @@ def main():
     batch = S.Batch(ctx, hms, sys_topo, X, grid=grid, rdamp=rdamp, inter_cutoff=rc, inter_switch=2.0 if layer else 0.0)
     fire = S.FireParams(dt0=W.fs_to_internal(0.1), dt_max=W.fs_to_internal(1.0))
     fconv = 1e-3
+    if args.no_graph:
+        batch.set_graph(False)
     tuned = None
     if args.autotune or args.workload == "uneven":
         t0 = time.time()

@@ -278,9 +278,10 @@ int smr_batch_plan(const smr_batch *b, int *nclass, int *info);
 int smr_batch_autotune(smr_batch *b, const smr_fire_params *fire, int trial_steps);
 
 /* 1 (default): capture the launch sequence of a relaxation (counter reset +
- * one persistent launch per size class on forked streams) into a HIP graph
+ * one persistent launch per size class on forked streams; for a coupled
+ * layer, 16 steps of force launches and cell-list kernels) into a HIP graph
  * once and replay it while nothing changes; 0: issue the launches directly.
- * No counterpart in the reference. */
+ * Results are the same either way.  No counterpart in the reference. */
 int smr_batch_set_graph(smr_batch *b, int use_graph);
 
 /* Order in which smr_relax_batched starts the systems of a size class:

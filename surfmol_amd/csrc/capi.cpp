@@ -1160,23 +1160,33 @@ int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double
     rc = prepare_plans(b->ls, b->plans, A, args);
     if (rc) return rc;
     const LayerArgs LA = layer_args(b);
+    // S steps of launches, captured once into a graph (or issued directly
+    // when the batch's graphs are switched off)
+    auto issue_steps = [&]() -> hipError_t {
+        hipError_t ei = hipSuccess;
+        for (int k = 0; k < S && ei == hipSuccess; k++) {
+            ei = issue_plans(c->ss, b->ls, b->plans, args);
+            if (ei == hipSuccess) ei = launch_layer_cells(LA, true, st);
+        }
+        return ei;
+    };
+    const bool use_graph = b->ls.use_graph;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipError_t ei = hipSuccess;
-    for (int k = 0; k < S && ei == hipSuccess; k++) {
-        ei = issue_plans(c->ss, b->ls, b->plans, args);
-        if (ei == hipSuccess) ei = launch_layer_cells(LA, true, st);
-    }
-    hipError_t e = hipStreamEndCapture(st, &graph);
-    if (ei != hipSuccess || e != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return hip_fail(ei != hipSuccess ? ei : e, "capture of the layer steps");
-    }
-    e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        return hip_fail(e, "hipGraphInstantiate (layer steps)");
+    hipError_t e = hipSuccess;
+    if (use_graph) {
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const hipError_t ei = issue_steps();
+        e = hipStreamEndCapture(st, &graph);
+        if (ei != hipSuccess || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return hip_fail(ei != hipSuccess ? ei : e, "capture of the layer steps");
+        }
+        e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return hip_fail(e, "hipGraphInstantiate (layer steps)");
+        }
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     e = hipEventCreate(&e0);
@@ -1185,7 +1195,7 @@ int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double
     LayerState hs = L0;
     const int max_rep = max_steps / S + 2;
     for (int r = 0; r < max_rep && e == hipSuccess; r++) {
-        e = hipGraphLaunch(gexec, st);
+        e = use_graph ? hipGraphLaunch(gexec, st) : issue_steps();
         if (e == hipSuccess) e = hipMemcpyAsync(&hs, b->layer_state.ptr, sizeof(hs), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (hs.status != SMR_SYS_RUNNING) break;
@@ -1195,8 +1205,8 @@ int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double
     if (e == hipSuccess) e = hipEventElapsedTime(&b->last_kernel_ms, e0, e1);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    (void)hipGraphExecDestroy(gexec);
-    (void)hipGraphDestroy(graph);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return hip_fail(e, "layer relaxation");
     rc = eval_energies64(b);
     if (rc) return rc;

@@ -96,16 +96,19 @@ def test_layer_relax_is_fp64_minimum(ctx, surf, oracle, params):
 
 
 def test_layer_relax_is_reproducible(ctx, surf, oracle, params):
+    """Repeated relaxations, replayed from the captured graph or issued
+    directly, are bit-identical."""
     import surfmol_amd as S
     mol, X, b, osys = _layer(ctx, surf, oracle, params, 4, 4, 12.0, w=2.0)
     fp = S.FireParams(**fire_params_internal())
     r1 = b.relax_batched(fp, 600, 1e-3)
     x1 = b.positions()
-    for _ in range(2):
+    for graph in (True, False, True):
+        b.set_graph(graph)
         b.reset()
         b.eval_forces()
         r2 = b.relax_batched(fp, 600, 1e-3)
-        assert np.array_equal(b.positions(), x1)
+        assert np.array_equal(b.positions(), x1), graph
         assert [r["energy"] for r in r1] == [r["energy"] for r in r2]
         assert r1[0]["steps"] == r2[0]["steps"]
 
