@@ -780,6 +780,7 @@ int layer_setup(smr_batch *b, const double *apos) {
     if (e == hipSuccess) e = hipMemsetAsync(b->cell_swrap.ptr, 0, sizeof(int2) * na, st);
     if (e == hipSuccess) e = b->cell_part.alloc(sizeof(float4) * 27 * (size_t)na);
     if (e == hipSuccess) e = b->slot_of.alloc(sizeof(int) * na);
+    if (e == hipSuccess) e = hipMemsetAsync(b->slot_of.ptr, 0, sizeof(int) * na, st);
     if (e == hipSuccess) e = b->partials.alloc(sizeof(double4) * ns);
     if (e == hipSuccess) e = b->layer_state.alloc(sizeof(LayerState));
     if (e == hipSuccess) e = hipMemsetAsync(b->layer_state.ptr, 0, sizeof(LayerState), st);
@@ -820,6 +821,8 @@ LayerArgs layer_args(smr_batch *b) {
     L.slot_of = (int *)b->slot_of.ptr;
     L.cells = b->cells;
     L.layer = (LayerState *)b->layer_state.ptr;
+    L.work_counters = (int *)b->ls.counters.ptr;
+    L.nwork_counters = NCLASS;
     return L;
 }
 
@@ -1165,7 +1168,7 @@ int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double
     auto issue_steps = [&]() -> hipError_t {
         hipError_t ei = hipSuccess;
         for (int k = 0; k < S && ei == hipSuccess; k++) {
-            ei = issue_plans(c->ss, b->ls, b->plans, args);
+            ei = issue_plans(c->ss, b->ls, b->plans, args, false);
             if (ei == hipSuccess) ei = launch_layer_cells(LA, true, st);
         }
         return ei;
@@ -1188,8 +1191,10 @@ int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double
             return hip_fail(e, "hipGraphInstantiate (layer steps)");
         }
     }
+    // the first step's work counters (each step's cell scan resets them for the next)
+    if (e == hipSuccess) e = launch_zero_i32((int *)b->ls.counters.ptr, NCLASS, st);
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) e = hipEventRecord(e0, st);
     LayerState hs = L0;

@@ -219,10 +219,12 @@ int prepare_plans(LaunchSet &ls, const std::vector<ClassPlan> &plans, const Kern
 }
 
 hipError_t issue_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans,
-                       const std::vector<KernelArgs> &args) {
+                       const std::vector<KernelArgs> &args, bool zero_counters) {
     const int naux = std::min<int>(3, (int)plans.size() - 1);
-    hipError_t r = launch_zero_i32((int *)ls.counters.ptr, NCLASS, ss.main);
-    if (r != hipSuccess) return r;
+    hipError_t r = hipSuccess;
+    // the work counters start at zero (a coupled layer's cell scan zeroes
+    // them for the next step instead)
+    if (zero_counters && (r = launch_zero_i32((int *)ls.counters.ptr, NCLASS, ss.main)) != hipSuccess) return r;
     if (naux > 0) {
         if ((r = hipEventRecord(ss.fork, ss.main)) != hipSuccess) return r;
         for (int a = 0; a < naux; a++)

@@ -9,11 +9,15 @@
 //   k_relax (KMODE_LAYER)  forces of every molecule, v += F dt, per-molecule
 //                          FIRE sums; the last molecule to finish reduces
 //                          them and takes the FIRE decision
-//   k_layer_move_count     apply the decision (mix v, x += v dt), bin atoms
+//   k_layer_move_count     apply the decision (mix v, x += v dt), bin atoms,
+//                          write the new position into the atom's sorted slot
+//                          and flag the step if any atom changed cell
 //   k_cells_scan           cell starts (exclusive scan), clears the decision
 //   k_cells_scatter        atoms into their cells
 //   k_cells_sort           order each cell by atom index, gather the sorted
 //                          positions / REQ rows
+// (scan, scatter and sort do nothing more on a step where no atom changed
+// cell: cells at least the cutoff wide keep their atoms for many steps)
 //   k_layer_inter          intermolecular forces: one workgroup per (cell,
 //                          neighbour cell), one lane per atom of the cell; one
 //                          partial force per atom and neighbour cell that the
@@ -75,19 +79,36 @@ __global__ void k_layer_move_count(LayerArgs L, int move) {
     }
     L.atom_wrap[a] = w;
     const int c = cell_of(C, gx, gy, p.z + (float)sd.gz * L.tab.ostep_z);
+    // without a move (set-up, evaluation) the list is always rebuilt
+    if (!move || c != L.atom_cell[a]) L.layer->cells_dirty = 1;
     L.atom_cell[a] = c;
     atomicAdd(&L.count[c], 1);
+    if (move) {
+        // the sorted slot of the last rebuild (rewritten by the sort when the
+        // list is rebuilt this step)
+        const int q = L.slot_of[a];
+        L.spos[q] = make_float4(p.x, p.y, p.z, __int_as_float(L.atom_sys[a]));
+        L.swrap[q] = w;
+    }
 }
 
 // Exclusive scan of the cell counts (one workgroup of 1024 threads): each
 // thread scans a contiguous chunk, chunk totals are scanned in LDS.  Zeroes
-// the counts for the next step and clears the applied decision.
+// the counts for the next step, clears the applied decision and resets the
+// next force launch's work counters.
 __global__ __launch_bounds__(1024) void k_cells_scan(LayerArgs L) {
     __shared__ int part[1024];
     const int n = L.cells.ncx * L.cells.ncy * L.cells.ncz;
     const int t = threadIdx.x;
     const int chunk = (n + 1023) / 1024;
     const int c0 = min(n, t * chunk), c1 = min(n, c0 + chunk);
+    if (!L.layer->cells_dirty) {
+        // no atom changed cell: starts and order stand, the counts restart
+        for (int c = c0; c < c1; c++) L.count[c] = 0;
+        if (t == 0) L.layer->pending_move = 0;
+        if (L.work_counters && t < L.nwork_counters) L.work_counters[t] = 0;
+        return;
+    }
     int sum = 0;
     for (int c = c0; c < c1; c++) sum += L.count[c];
     part[t] = sum;
@@ -108,11 +129,12 @@ __global__ __launch_bounds__(1024) void k_cells_scan(LayerArgs L) {
     }
     if (t == 1023) L.start[n] = part[1023];
     if (t == 0) L.layer->pending_move = 0;
+    if (L.work_counters && t < L.nwork_counters) L.work_counters[t] = 0;
 }
 
 __global__ void k_cells_scatter(LayerArgs L) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= L.natoms) return;
+    if (a >= L.natoms || !L.layer->cells_dirty) return;
     const int slot = atomicAdd(&L.fill[L.atom_cell[a]], 1);
     L.sidx[slot] = a;
 }
@@ -123,6 +145,7 @@ __global__ void k_cells_scatter(LayerArgs L) {
 __global__ __launch_bounds__(256) void k_cells_sort(LayerArgs L) {
     __shared__ int key[256];
     const int c = blockIdx.x, t = threadIdx.x;
+    if (!L.layer->cells_dirty) return;
     const int st = L.start[c], n = L.start[c + 1] - st;
     if (n == 0) return;
     if (n <= 256) {
@@ -186,6 +209,8 @@ __global__ __launch_bounds__(64) void k_layer_inter(LayerArgs L) {
     const DevTables &T = L.tab;
     const int K = C.nox * C.noy * C.noz;
     const int c = blockIdx.x / K, o = blockIdx.x - K * c;
+    // the move, scan, scatter and sort of this step have read the flag
+    if (blockIdx.x == 0 && threadIdx.x == 0) L.layer->cells_dirty = 0;
     const int s0 = C.start[c], n = C.start[c + 1] - s0;
     if (n == 0) return;
     const int cx = c % C.ncx, cy = (c / C.ncx) % C.ncy, cz = c / (C.ncx * C.ncy);

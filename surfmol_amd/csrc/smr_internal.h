@@ -119,7 +119,8 @@ int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan
 int prepare_plans(LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
                   std::vector<KernelArgs> &args);
 hipError_t issue_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans,
-                       const std::vector<KernelArgs> &args);
+                       const std::vector<KernelArgs> &args,
+                       bool zero_counters = true);
 
 int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, HostPools &P, TopoDesc &td);
 hipError_t upload_pools(HostPools &P, DevPools &D, hipStream_t st);

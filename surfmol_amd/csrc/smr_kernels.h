@@ -47,6 +47,7 @@ struct LayerState {
     int reset;
     int pending_move;        // the decision of this step still has to be applied
     unsigned ticket;         // systems finished in the running force launch
+    int cells_dirty;         // an atom changed cell in this step's move: the cell list is rebuilt
 };
 
 struct KernelArgs {
@@ -148,6 +149,8 @@ struct LayerArgs {
     int *slot_of;            // [natoms]
     CellGrid cells;
     LayerState *layer;
+    int *work_counters;      // [nwork_counters] reset by the cell scan for the next force launch (or NULL)
+    int nwork_counters;
 };
 hipError_t launch_layer_cells(const LayerArgs &L, bool move, hipStream_t st);
 hipError_t launch_energy64_inter(const LayerArgs &L, SysState *state, const int *list, int n, hipStream_t st);
