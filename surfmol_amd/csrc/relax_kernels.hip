@@ -193,12 +193,25 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
 
     // a coupled layer that has converged (or stopped) does no more work
     if (A.mode == KMODE_LAYER && A.layer->status != SMR_SYS_RUNNING) return;
+    // a launch of one step (coupled-layer steps, force evaluations) with a
+    // workgroup for every system: each takes its own, without a contended
+    // counter; otherwise systems are fetched through the atomic counter in
+    // the order the workgroups start (concurrent launches share the chip,
+    // so not every workgroup starts at once)
+    const bool one_each = A.steps_per_launch == 1 && A.nwork <= (int)gridDim.x;
+    int pass = 0;
     while (true) {
         // ---- fetch the next system from the work list ----
-        if (tid == 0) s_next[0] = atomicAdd(A.work_counter, 1);
-        block_sync<NW>();
-        const int wi = s_next[0];
-        block_sync<NW>();
+        int wi;
+        if (one_each) {
+            if (pass++) break;
+            wi = blockIdx.x;
+        } else {
+            if (tid == 0) s_next[0] = atomicAdd(A.work_counter, 1);
+            block_sync<NW>();
+            wi = s_next[0];
+            block_sync<NW>();
+        }
         if (wi >= A.nwork) break;
         const int s = A.work_list[wi];
         const SysDesc sd = T.sys[s];
