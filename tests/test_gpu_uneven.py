@@ -135,6 +135,31 @@ def test_uneven_same_alone_and_in_mix(ctx, surf):
         assert alone[0]["energy"] == mix[s]["energy"], (s, alone[0], mix[s])
 
 
+def test_merged_launch_same_as_alone(ctx, surf):
+    """With 256 threads forced, the <= 128- and <= 256-atom classes share one
+    persistent launch; each system relaxes bit for bit as in a batch of its
+    own (its launch shape is the same either way)."""
+    import surfmol_amd as S
+    mols = _molecules([90, 120, 150, 200, 250, 100], 0x5EED100B)
+    X = _poses(mols, surf["spec"]["L"], 0x5EED100C)
+    fp = S.FireParams(**fire_params_internal())
+    b = _batch(ctx, surf, mols, X)
+    b.set_launch(256, -1, -1)
+    plan = b.plan()
+    print(plan)
+    assert len(plan) == 1 and plan[0]["systems"] == len(mols) and plan[0]["block"] == 256, plan
+    mix = b.relax_batched(fp, 1200, 1e-3)
+    xm = b.positions()
+    for s in [0, 3, 4]:
+        a = _batch(ctx, surf, mols[s:s + 1], X[s:s + 1])
+        a.set_launch(256, -1, -1)
+        ra = a.relax_batched(fp, 1200, 1e-3)[0]
+        assert ra["steps"] == mix[s]["steps"] and ra["energy"] == mix[s]["energy"], (s, ra, mix[s])
+        assert np.array_equal(a.positions(), xm[b.offsets[s]:b.offsets[s + 1]])
+        a.close()
+    b.close()
+
+
 def test_graph_replays_match_eager(ctx, surf):
     """The captured launch sequence, replayed several times with eager work
     and synchronisations between replays, gives the eager result each time."""
