@@ -1,6 +1,7 @@
-"""Edge sizes on the GPU: the smallest systems (one atom: surface only; two
-atoms: one bond, no pair), the largest supported system (exactly 1024
-atoms), and requests beyond what is built, which are errors.
+"""Edge sizes and statuses on the GPU: the smallest systems (one atom:
+surface only; two atoms: one bond, no pair), the largest supported system
+(exactly 1024 atoms), requests beyond what is built, which are errors, and
+the step-cap and non-finite statuses (a non-finite system stops alone).
 
 The reference's relax_multi takes systems of any size (relax_multi.rs reads
 natoms per system from the topology); this library supports 1..1024 atoms
@@ -198,3 +199,55 @@ def test_size_limits_are_errors(ctx, surf):
     with pytest.raises(S.SmrError) as e:   # a topology index outside the list
         S.Batch(ctx, [small], [-1], np.array([[0.0, 0.0, 3.0]]), grid=surf["grid"], rdamp=RDAMP)
     assert e.value.code == L.SMR_ERR_INVALID
+
+
+def test_nonfinite_input_is_an_error(ctx, surf):
+    import surfmol_amd as S
+    from surfmol_amd import _lib as L
+    mols, X = _tiny()
+    hm = S.Molecule.build(mols[2].elems, X[2], mols[2].charges, bonds=mols[2].bonds)
+    bad = X[2].copy()
+    bad[1, 0] = np.nan
+    with pytest.raises(S.SmrError) as e:
+        S.Batch(ctx, [hm], [0], bad, grid=surf["grid"], rdamp=RDAMP)
+    assert e.value.code == L.SMR_ERR_INVALID
+
+
+def test_step_cap_status(ctx, surf, oracle, params):
+    """A relaxation cut at max_steps reports SMR_SYS_MAXSTEPS after exactly
+    that many steps, with the FP64 energy of the positions it returns."""
+    import surfmol_amd as S
+    mols, X = _tiny()
+    b = _batch(ctx, surf, mols, X)
+    res = b.relax_batched(S.FireParams(**fire_params_internal()), 10, 1e-3)
+    Xf = b.positions()
+    for s, (m, r) in enumerate(zip(mols, res)):
+        assert r["status"] == S.SMR_SYS_MAXSTEPS and r["steps"] == 10, (s, r)
+        sl = slice(b.offsets[s], b.offsets[s + 1])
+        fo, eo = oracle.eval_forces(oracle_system(oracle, params, m, X[s], surf["C"], surf["spec"]), Xf[sl])
+        assert abs(eo.sum() - r["energy"]) / m.natoms < E_ATOL_PER_ATOM, (s, eo.sum(), r["energy"])
+    b.close()
+
+
+def test_nonfinite_system_stops_alone(ctx, surf):
+    """Two unbonded atoms on the same spot give a non-finite pair force: that
+    system stops with SMR_SYS_NONFINITE after one step, and the other system
+    of the batch relaxes bit for bit as in a batch of its own."""
+    import surfmol_amd as S
+    mols, X = _tiny()
+    Xt = np.array([[2.0, 3.0, 3.4], [2.0, 3.0, 3.4]])
+    # no bonds: nbonds == 0 with rcut_bond = 0 (no pair is closer than 0)
+    twin = S.Molecule.build(["C", "C"], Xt, [0.0, 0.0], rcut_bond=0.0)
+    assert twin.counts == (2, 0, 0, 0, 0)
+    poly = S.Molecule.build(mols[3].elems, X[3], mols[3].charges, bonds=mols[3].bonds)
+    fp = S.FireParams(**fire_params_internal())
+    b = S.Batch(ctx, [twin, poly], [0, 1], np.concatenate([Xt, X[3]]), grid=surf["grid"], rdamp=RDAMP)
+    res = b.relax_batched(fp, 2000, 1e-3)
+    assert res[0]["status"] == S.SMR_SYS_NONFINITE and res[0]["steps"] == 1, res[0]
+    a = _batch(ctx, surf, [mols[3]], [X[3]])
+    ra = a.relax_batched(fp, 2000, 1e-3)[0]
+    assert res[1]["status"] == ra["status"] == S.SMR_SYS_CONVERGED
+    assert res[1]["steps"] == ra["steps"] and res[1]["energy"] == ra["energy"], (res[1], ra)
+    assert np.array_equal(b.positions()[b.offsets[1]:], a.positions())
+    a.close()
+    b.close()
