@@ -121,7 +121,7 @@ def main():
     ap.add_argument("--workload", choices=["uniform", "uneven", "layer", "serve"], default="uniform")
     ap.add_argument("--rates", type=str, default="200,1000,5000", help="serve: arrival rates [req/s]")
     ap.add_argument("--requests", type=int, default=1000, help="serve: requests per rate")
-    ap.add_argument("--chunk-steps", type=int, default=64, help="serve: FIRE steps per persistent launch")
+    ap.add_argument("--chunk-steps", type=int, default=64, help="serve: FIRE steps per persistent launch (<= 64 atoms)")
     ap.add_argument("--nsys", type=int, default=None)
     ap.add_argument("--max-steps", type=int, default=None, help="FIRE step cap (2000; 20000 for the layer)")
     ap.add_argument("--layer-spacing", type=float, default=13.0,

@@ -305,7 +305,9 @@ typedef struct {
     int max_atoms;            /* largest request accepted (<= 1024)             */
     double hbm_fraction;      /* device memory for resident requests, as a
                                  fraction of the HBM free at creation (0,1]     */
-    int chunk_steps;          /* FIRE steps per persistent launch               */
+    int chunk_steps;          /* FIRE steps per persistent launch of requests of
+                                 <= 64 atoms; size class c (64, 128, 256, 512,
+                                 1024 atoms) takes chunk_steps*2/(2+c)          */
     int max_steps;            /* step cap per request                           */
     double fconv;             /* convergence threshold [eV/A]                   */
     smr_fire_params fire;
@@ -316,8 +318,8 @@ typedef struct {
  * max_atoms) are queued by smr_server_submit; a worker thread admits them
  * into free slots while the device arenas (sized from hbm_fraction of free
  * HBM) hold their topology and state, relaxes every resident request by
- * chunked persistent launches of chunk_steps FIRE steps (one launch per
- * size class), and retires each request as soon as it converges (or hits
+ * chunked persistent launches (one launch per size class; the larger
+ * classes take fewer steps per chunk), and retires each request as soon as it converges (or hits
  * max_steps), with FP64 energies of its final positions.  Requests that
  * arrive while others run are admitted at the next chunk.  A request's
  * result does not depend on what else is resident (its size class fixes

@@ -205,6 +205,7 @@ int prepare_plans(LaunchSet &ls, const std::vector<ClassPlan> &plans, const Kern
         memcpy(&A, &base, sizeof(KernelArgs));
         A.work_list = base.work_list + p.first;
         A.nwork = p.count;
+        if (p.steps > 0) A.steps_per_launch = p.steps;
         A.work_counter = (int *)ls.counters.ptr + p.cls;
         A.fint_scratch = (float *)ls.scratch.ptr + off[q];
         A.nf_max = std::max(1, p.nf_max);

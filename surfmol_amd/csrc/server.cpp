@@ -12,7 +12,8 @@
 // caller's thread and queues it.  A worker thread loops:
 //   admit queued requests into free slots / arena ranges (FIFO);
 //   upload their tables, positions and FIRE state with one staged copy;
-//   run one chunk: chunk_steps FIRE steps of every resident request, one
+//   run one chunk: a few FIRE steps of every resident request (chunk_steps
+//     for the smallest size class, fewer for the larger ones), one
 //     persistent launch per size class (launch_plans);
 //   read the FIRE states back; for requests that converged or hit the step
 //     cap, evaluate FP64 energies, copy their positions out, free their
@@ -320,7 +321,16 @@ int run_chunk(smr_server *S) {
     std::vector<ClassPlan> plans;
     std::vector<int> work;
     build_plans(sizes, plans, work);
-    for (auto &p : plans) p.shape = class_shape(S->ctx, p, S->grid != nullptr, 0, -1, -1);
+    // a chunk ends when its slowest launch ends, and a step of a larger
+    // class costs more (its pair sweep grows with the atom count): class c
+    // takes chunk_steps * 2 / (2 + c) steps (64, 42, 32, 25, 21 of 64), so
+    // the small requests, most of the traffic, wait less for the large ones'
+    // steps. Halving per class instead starved the large requests (p99 up
+    // by a third); chunk boundaries change no result
+    for (auto &p : plans) {
+        p.shape = class_shape(S->ctx, p, S->grid != nullptr, 0, -1, -1);
+        p.steps = std::max(1, S->cfg.chunk_steps * 2 / (2 + p.cls));
+    }
     hipError_t e = hipStreamSynchronize(S->ss.main);   // hwork may still feed the previous copy
     if (e != hipSuccess) return hip_fail(e, "server chunk");
     memcpy(S->hwork, work.data(), work.size() * sizeof(int));

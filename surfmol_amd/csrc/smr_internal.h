@@ -66,6 +66,7 @@ struct ClassPlan {
     int max_atoms = 0, nf_max = 0, na_max = 0, nd_max = 0, ni_max = 0;
     LaunchShape shape;
     size_t scratch_off = 0;                   // floats into the force-piece scratch
+    int steps = 0;                            // FIRE steps of this launch (0: those of the launch arguments)
 };
 
 // launch-shape choice of one class remembered by the context (autotuner)
