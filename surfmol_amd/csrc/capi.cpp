@@ -1119,7 +1119,7 @@ FireConst make_fire_const(const smr_fire_params *f, int max_steps, double fconv)
 namespace smr {
 
 // FIRE on a coupled layer: steps of [force launches (per size class) ->
-// move + cell-list rebuild] captured S at a time into a HIP graph; the host
+// FIRE decision -> move + cell-list rebuild] captured S at a time into a HIP graph; the host
 // replays the graph until the shared FIRE state leaves RUNNING.
 int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double fconv, smr_relax_result *res) {
     smr_ctx *c = b->ctx;
@@ -1162,13 +1162,16 @@ int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double
     std::vector<KernelArgs> args;
     rc = prepare_plans(b->ls, b->plans, A, args);
     if (rc) return rc;
-    const LayerArgs LA = layer_args(b);
+    LayerArgs LA = layer_args(b);
+    LA.partials = A.partials;
+    LA.fire = A.fire;
     // S steps of launches, captured once into a graph (or issued directly
     // when the batch's graphs are switched off)
     auto issue_steps = [&]() -> hipError_t {
         hipError_t ei = hipSuccess;
         for (int k = 0; k < S && ei == hipSuccess; k++) {
             ei = issue_plans(c->ss, b->ls, b->plans, args, false);
+            if (ei == hipSuccess) ei = launch_layer_decide(LA, st);
             if (ei == hipSuccess) ei = launch_layer_cells(LA, true, st);
         }
         return ei;

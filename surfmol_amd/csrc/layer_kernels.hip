@@ -7,8 +7,8 @@
 // with the uniform cell grid of spacc/src/uniform_grid.rs:87-116), and one
 // FIRE state drives every atom.  One step is
 //   k_relax (KMODE_LAYER)  forces of every molecule, v += F dt, per-molecule
-//                          FIRE sums; the last molecule to finish reduces
-//                          them and takes the FIRE decision
+//                          FIRE sums
+//   k_layer_decide         reduce the sums, take the FIRE decision
 //   k_layer_move_count     apply the decision (mix v, x += v dt), bin atoms,
 //                          write the new position into the atom's sorted slot
 //                          and flag the step if any atom changed cell
@@ -42,6 +42,59 @@ __device__ __forceinline__ int cell_of(const CellGrid &C, float gx, float gy, fl
 }
 
 }  // namespace
+
+// The shared FIRE decision of a step (one wave): reduce the per-molecule sums
+// of the force launch in a fixed order, then the FIRE update of
+// relax_pentacene_uff.rs:162-181 (as in k_relax) and the status; the move
+// is applied by k_layer_move_count.  A layer that has stopped takes no
+// decision (its force launch returned at once).  A launch of its own: a
+// last-finisher ticket in the force launch serialised its 512 workgroups
+// on one device-scope atomic (half the step).
+__global__ __launch_bounds__(64) void k_layer_decide(LayerArgs L) {
+    LayerState *S = L.layer;
+    if (S->status != SMR_SYS_RUNNING) return;
+    const int lane = threadIdx.x;
+    const int nsys = S->nsys;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    // molecules lane, lane+64, ... in order; eight loads in flight
+    for (int q0 = lane; q0 < nsys; q0 += 8 * 64) {
+        double4 p[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (q0 + 64 * u < nsys) p[u] = L.partials[q0 + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (q0 + 64 * u < nsys) { a0 += p[u].x; a1 += p[u].y; a2 += p[u].z; a3 = fmax(a3, p[u].w); }
+    }
+    const double vf = wave_sum(a0), vv = wave_sum(a1), ff = wave_sum(a2), fm = wave_max(a3);
+    if (lane != 0) return;
+    const FireConst &P = L.fire;
+    const double dt_old = S->dt;
+    const bool nonfinite = !(isfinite(vf) && isfinite(vv) && isfinite(ff));
+    if (vf > 0.0 && ff > 1e-30 && vv > 1e-30) {
+        S->mix_a = (float)(1.0 - S->alpha);
+        S->mix_b = (float)(S->alpha * sqrt(vv) / sqrt(ff));
+        S->reset = 0;
+        S->npos++;
+        if (S->npos > P.n_min) {
+            S->dt = fmin(S->dt * P.f_inc, P.dt_max);
+            S->alpha *= P.f_alpha;
+        }
+    } else {
+        S->npos = 0;
+        S->dt *= P.f_dec;
+        S->alpha = P.alpha0;
+        S->mix_a = S->mix_b = 0.f;
+        S->reset = 1;
+    }
+    S->dt_move = (float)dt_old;
+    S->steps++;
+    S->fmax = sqrt(fm);
+    if (nonfinite) S->status = SMR_SYS_NONFINITE;
+    else if (S->fmax < P.fconv) S->status = SMR_SYS_CONVERGED;
+    else if (S->steps >= P.max_steps) S->status = SMR_SYS_MAXSTEPS;
+    S->pending_move = 1;
+}
 
 // Apply the FIRE decision of the step (if one is pending), then bin the atom.
 __global__ void k_layer_move_count(LayerArgs L, int move) {
@@ -140,8 +193,9 @@ __global__ void k_cells_scatter(LayerArgs L) {
 }
 
 // One workgroup per cell: order the cell's atoms by index (bitonic network
-// in LDS over 256 slots; a longer cell is sorted serially), then gather
-// positions (w = system) and REQ rows in that order.
+// in LDS over the next power of two >= the cell's atoms, at most 256; a
+// longer cell is sorted serially), then gather positions (w = system) and
+// REQ rows in that order.
 __global__ __launch_bounds__(256) void k_cells_sort(LayerArgs L) {
     __shared__ int key[256];
     const int c = blockIdx.x, t = threadIdx.x;
@@ -149,9 +203,11 @@ __global__ __launch_bounds__(256) void k_cells_sort(LayerArgs L) {
     const int st = L.start[c], n = L.start[c + 1] - st;
     if (n == 0) return;
     if (n <= 256) {
+        int w = 2;
+        while (w < n) w <<= 1;
         key[t] = t < n ? L.sidx[st + t] : 0x7fffffff;
         __syncthreads();
-        for (int k = 2; k <= 256; k <<= 1)
+        for (int k = 2; k <= w; k <<= 1)
             for (int j = k >> 1; j > 0; j >>= 1) {
                 const int o = t ^ j;
                 if (o > t) {
@@ -350,6 +406,11 @@ hipError_t launch_layer_cells(const LayerArgs &L, bool move, hipStream_t st) {
     hipLaunchKernelGGL(k_cells_scatter, dim3(nb), dim3(256), 0, st, L);
     hipLaunchKernelGGL(k_cells_sort, dim3(ncells), dim3(256), 0, st, L);
     hipLaunchKernelGGL(k_layer_inter, dim3(ncells * L.cells.nox * L.cells.noy * L.cells.noz), dim3(64), 0, st, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_layer_decide(const LayerArgs &L, hipStream_t st) {
+    hipLaunchKernelGGL(k_layer_decide, dim3(1), dim3(64), 0, st, L);
     return hipGetLastError();
 }
 

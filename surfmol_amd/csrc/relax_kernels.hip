@@ -34,63 +34,6 @@ __device__ __forceinline__ void block_sync() {
     }
 }
 
-// The shared FIRE decision of a coupled layer, taken by the workgroup that
-// finished the step's last molecule: reduce the per-molecule sums in a fixed
-// order, then the FIRE update of relax_pentacene_uff.rs:162-181 (as in
-// k_relax) and the status; the move is applied by k_layer_move_count.
-template <int BLOCK>
-__device__ void layer_decide(const KernelArgs &A, double *s_red) {
-    constexpr int NW = BLOCK / 64;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    LayerState *L = A.layer;
-    __threadfence();   // acquire: the other molecules' partial sums
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (int q = tid; q < L->nsys; q += BLOCK) {
-        const double4 p = A.partials[q];
-        a0 += p.x; a1 += p.y; a2 += p.z; a3 = fmax(a3, p.w);
-    }
-    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2); a3 = wave_max(a3);
-    if (lane == 0) {
-        s_red[wid * 8 + 4] = a0; s_red[wid * 8 + 5] = a1;
-        s_red[wid * 8 + 6] = a2; s_red[wid * 8 + 7] = a3;
-    }
-    block_sync<NW>();
-    if (tid == 0) {
-        double vf = 0.0, vv = 0.0, ff = 0.0, fm = 0.0;
-        for (int w = 0; w < NW; w++) {
-            vf += s_red[w * 8 + 4]; vv += s_red[w * 8 + 5]; ff += s_red[w * 8 + 6]; fm = fmax(fm, s_red[w * 8 + 7]);
-        }
-        const FireConst &P = A.fire;
-        const double dt_old = L->dt;
-        const bool nonfinite = !(isfinite(vf) && isfinite(vv) && isfinite(ff));
-        if (vf > 0.0 && ff > 1e-30 && vv > 1e-30) {
-            L->mix_a = (float)(1.0 - L->alpha);
-            L->mix_b = (float)(L->alpha * sqrt(vv) / sqrt(ff));
-            L->reset = 0;
-            L->npos++;
-            if (L->npos > P.n_min) {
-                L->dt = fmin(L->dt * P.f_inc, P.dt_max);
-                L->alpha *= P.f_alpha;
-            }
-        } else {
-            L->npos = 0;
-            L->dt *= P.f_dec;
-            L->alpha = P.alpha0;
-            L->mix_a = L->mix_b = 0.f;
-            L->reset = 1;
-        }
-        L->dt_move = (float)dt_old;
-        L->steps++;
-        L->fmax = sqrt(fm);
-        if (nonfinite) L->status = SMR_SYS_NONFINITE;
-        else if (L->fmax < P.fconv) L->status = SMR_SYS_CONVERGED;
-        else if (L->steps >= P.max_steps) L->status = SMR_SYS_MAXSTEPS;
-        L->pending_move = 1;
-        L->ticket = 0;
-        __threadfence();
-    }
-}
-
 // Value of lane (lane+1) mod 64 (DPP wave_rol:1).
 __device__ __forceinline__ float wave_rol1(float x) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x134, 0xf, 0xf, true));
@@ -732,8 +675,8 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
             // block reduction of (vf, vv, ff, fmax^2)
             if (A.mode == KMODE_LAYER) {
                 // coupled layer: clamped forces and updated velocities go to
-                // memory, this molecule's sums to its partial slot; the last
-                // molecule of the step takes the shared FIRE decision
+                // memory, this molecule's sums to its partial slot; the shared
+                // FIRE decision is taken by the next launch (k_layer_decide)
 #pragma unroll
                 for (int k = 0; k < APT; k++) {
                     const int i = tid + k * BLOCK;
@@ -757,12 +700,7 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                         r2 += s_red[w * 8 + 2]; r3 = fmax(r3, s_red[w * 8 + 3]);
                     }
                     A.partials[s] = make_double4(r0, r1, r2, r3);
-                    __threadfence();
-                    const unsigned t = atomicAdd(&A.layer->ticket, 1u);
-                    s_next[1] = (t == (unsigned)(A.layer->nsys - 1)) ? 1 : 0;
                 }
-                block_sync<NW>();
-                if (s_next[1]) layer_decide<BLOCK>(A, s_red);
                 block_sync<NW>();
                 break;
             }

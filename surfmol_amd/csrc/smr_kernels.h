@@ -46,7 +46,6 @@ struct LayerState {
     float mix_a, mix_b, dt_move;
     int reset;
     int pending_move;        // the decision of this step still has to be applied
-    unsigned ticket;         // systems finished in the running force launch
     int cells_dirty;         // an atom changed cell in this step's move: the cell list is rebuilt
 };
 
@@ -151,8 +150,11 @@ struct LayerArgs {
     LayerState *layer;
     int *work_counters;      // [nwork_counters] reset by the cell scan for the next force launch (or NULL)
     int nwork_counters;
+    const double4 *partials; // [nsys] per-molecule FIRE sums of the step's force launch
+    FireConst fire;          // FIRE parameters of a relaxation (k_layer_decide)
 };
 hipError_t launch_layer_cells(const LayerArgs &L, bool move, hipStream_t st);
+hipError_t launch_layer_decide(const LayerArgs &L, hipStream_t st);
 hipError_t launch_energy64_inter(const LayerArgs &L, SysState *state, const int *list, int n, hipStream_t st);
 hipError_t launch_plain_step_forces(const PlainArgs &P, int natoms, int nang, int ndih, int ninv, hipStream_t st);
 hipError_t launch_plain_reduce(const PlainArgs &P, double dt, double flim, hipStream_t st);
