@@ -12,12 +12,13 @@
 //   k_layer_move_count     apply the decision (mix v, x += v dt), bin atoms,
 //                          write the new position into the atom's sorted slot
 //                          and flag the step if any atom changed cell
-//   k_cells_scan           cell starts (exclusive scan), clears the decision
-//   k_cells_scatter        atoms into their cells
+//   k_cells_build          one workgroup: count (LDS atomics), cell starts
+//                          (exclusive scan), atoms into their cells; clears
+//                          the decision
 //   k_cells_sort           order each cell by atom index, gather the sorted
 //                          positions / REQ rows
-// (scan, scatter and sort do nothing more on a step where no atom changed
-// cell: cells at least the cutoff wide keep their atoms for many steps)
+// (build and sort do nothing more on a step where no atom changed cell:
+// cells at least the cutoff wide keep their atoms for many steps)
 //   k_layer_inter          intermolecular forces: one workgroup per (cell,
 //                          neighbour cell), one lane per atom of the cell; one
 //                          partial force per atom and neighbour cell that the
@@ -135,7 +136,6 @@ __global__ void k_layer_move_count(LayerArgs L, int move) {
     // without a move (set-up, evaluation) the list is always rebuilt
     if (!move || c != L.atom_cell[a]) L.layer->cells_dirty = 1;
     L.atom_cell[a] = c;
-    atomicAdd(&L.count[c], 1);
     if (move) {
         // the sorted slot of the last rebuild (rewritten by the sort when the
         // list is rebuilt this step)
@@ -145,25 +145,40 @@ __global__ void k_layer_move_count(LayerArgs L, int move) {
     }
 }
 
-// Exclusive scan of the cell counts (one workgroup of 1024 threads): each
-// thread scans a contiguous chunk, chunk totals are scanned in LDS.  Zeroes
-// the counts for the next step, clears the applied decision and resets the
-// next force launch's work counters.
-__global__ __launch_bounds__(1024) void k_cells_scan(LayerArgs L) {
-    __shared__ int part[1024];
-    const int n = L.cells.ncx * L.cells.ncy * L.cells.ncz;
+// The cell list of the atoms' cells (one workgroup of 1024 threads): count
+// the atoms of each cell, exclusive scan into the cell starts, scatter the
+// atom indices into their cells (in no particular order inside a cell:
+// k_cells_sort orders them).  Counts and fill cursors live in LDS when the
+// grid has at most kCellsLds cells, else in the global count array.  Runs
+// only on a step where an atom changed cell; every step it clears the
+// applied decision and resets the next force launch's work counters.
+// (Counting here rather than in the move keeps the per-atom device-scope
+// atomics off the steps whose cells do not change.)
+constexpr int kCellsLds = 12288;
+
+template <bool LDS>
+__device__ void build_cells(const LayerArgs &L, int n, int *cnt, int *part) {
     const int t = threadIdx.x;
+    const int N = L.natoms;
+    for (int c = t; c < n; c += 1024) cnt[c] = 0;
+    if (!LDS) __threadfence();
+    __syncthreads();
+    // count: eight cell indices in flight per thread
+    for (int a0 = t; a0 < N; a0 += 8 * 1024) {
+        int cl[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) cl[u] = a0 + u * 1024 < N ? L.atom_cell[a0 + u * 1024] : -1;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (cl[u] >= 0) atomicAdd(&cnt[cl[u]], 1);
+    }
+    if (!LDS) __threadfence();
+    __syncthreads();
+    // exclusive scan: each thread a contiguous chunk, chunk totals in LDS
     const int chunk = (n + 1023) / 1024;
     const int c0 = min(n, t * chunk), c1 = min(n, c0 + chunk);
-    if (!L.layer->cells_dirty) {
-        // no atom changed cell: starts and order stand, the counts restart
-        for (int c = c0; c < c1; c++) L.count[c] = 0;
-        if (t == 0) L.layer->pending_move = 0;
-        if (L.work_counters && t < L.nwork_counters) L.work_counters[t] = 0;
-        return;
-    }
     int sum = 0;
-    for (int c = c0; c < c1; c++) sum += L.count[c];
+    for (int c = c0; c < c1; c++) sum += cnt[c];
     part[t] = sum;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
@@ -174,22 +189,35 @@ __global__ __launch_bounds__(1024) void k_cells_scan(LayerArgs L) {
     }
     int run = part[t] - sum;   // exclusive prefix of this chunk
     for (int c = c0; c < c1; c++) {
-        const int k = L.count[c];
+        const int k = cnt[c];
         L.start[c] = run;
-        L.fill[c] = run;
-        L.count[c] = 0;
+        cnt[c] = run;          // from here on: the cell's fill cursor
         run += k;
     }
     if (t == 1023) L.start[n] = part[1023];
-    if (t == 0) L.layer->pending_move = 0;
-    if (L.work_counters && t < L.nwork_counters) L.work_counters[t] = 0;
+    if (!LDS) __threadfence();
+    __syncthreads();
+    // scatter
+    for (int a0 = t; a0 < N; a0 += 8 * 1024) {
+        int cl[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) cl[u] = a0 + u * 1024 < N ? L.atom_cell[a0 + u * 1024] : -1;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (cl[u] >= 0) L.sidx[atomicAdd(&cnt[cl[u]], 1)] = a0 + u * 1024;
+    }
 }
 
-__global__ void k_cells_scatter(LayerArgs L) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= L.natoms || !L.layer->cells_dirty) return;
-    const int slot = atomicAdd(&L.fill[L.atom_cell[a]], 1);
-    L.sidx[slot] = a;
+__global__ __launch_bounds__(1024) void k_cells_build(LayerArgs L) {
+    __shared__ int part[1024];
+    __shared__ int cnt[kCellsLds];
+    const int t = threadIdx.x;
+    if (t == 0) L.layer->pending_move = 0;
+    if (L.work_counters && t < L.nwork_counters) L.work_counters[t] = 0;
+    if (!L.layer->cells_dirty) return;   // no atom changed cell: starts and order stand
+    const int n = L.cells.ncx * L.cells.ncy * L.cells.ncz;
+    if (n <= kCellsLds) build_cells<true>(L, n, cnt, part);
+    else build_cells<false>(L, n, L.count, part);
 }
 
 // One workgroup per cell: order the cell's atoms by index (bitonic network
@@ -402,8 +430,7 @@ hipError_t launch_layer_cells(const LayerArgs &L, bool move, hipStream_t st) {
     const int nb = (L.natoms + 255) / 256;
     const int ncells = L.cells.ncx * L.cells.ncy * L.cells.ncz;
     hipLaunchKernelGGL(k_layer_move_count, dim3(nb), dim3(256), 0, st, L, move ? 1 : 0);
-    hipLaunchKernelGGL(k_cells_scan, dim3(1), dim3(1024), 0, st, L);
-    hipLaunchKernelGGL(k_cells_scatter, dim3(nb), dim3(256), 0, st, L);
+    hipLaunchKernelGGL(k_cells_build, dim3(1), dim3(1024), 0, st, L);
     hipLaunchKernelGGL(k_cells_sort, dim3(ncells), dim3(256), 0, st, L);
     hipLaunchKernelGGL(k_layer_inter, dim3(ncells * L.cells.nox * L.cells.noy * L.cells.noz), dim3(64), 0, st, L);
     return hipGetLastError();

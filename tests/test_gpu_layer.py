@@ -60,6 +60,26 @@ def test_layer_eval_forces(ctx, surf, oracle, params, rc, w):
     assert (dE / n < 1e-6).all(), (E.sum(axis=0), eo)
 
 
+def test_layer_many_cells(ctx, surf, oracle, params):
+    """Two groups of molecules 700 A apart with a 6 A cutoff: a grid of
+    ~120 x 120 cells, more than the cell build counts in LDS (12 288), so
+    its global-count path runs; forces and energies as the FP64 oracle."""
+    import surfmol_amd as S
+    from surfmol_amd import workloads as W
+    mol, X = W.layer_lattice(2, 2, spacing=11.0, seed=7)
+    X = list(X) + [x + np.array([700.0, 700.0, 0.0]) for x in X]
+    assert ((700.0 + 12.0) / 6.0) ** 2 > 12288
+    hm = S.Molecule.build(mol.elems, X[0], mol.charges, bonds=mol.bonds)
+    b = S.Batch(ctx, [hm], np.zeros(len(X), dtype=np.int32), np.concatenate(X), grid=surf["grid"], rdamp=RDAMP,
+                surface=False, inter_cutoff=6.0)
+    osys = oracle.layer_from_molecules([mol] * len(X), X, params, 6.0, rdamp=RDAMP)
+    F, E = b.eval_forces()
+    Xd = b.positions()
+    fo, eo = oracle.eval_forces(osys, Xd)
+    assert rel_force_err(F, fo) < 1e-5
+    assert (np.abs(E.sum(axis=0) - eo) / len(Xd) < 1e-6).all(), (E.sum(axis=0), eo)
+
+
 def test_layer_without_neighbours_is_independent(ctx, surf, oracle, params):
     """Molecules 40 A apart: the layer equals the independent batch."""
     import surfmol_amd as S
