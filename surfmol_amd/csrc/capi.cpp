@@ -768,7 +768,6 @@ int layer_setup(smr_batch *b, const double *apos) {
     if (e == hipSuccess) e = b->req_atom.upload(areq.data(), sizeof(float4) * na, st);
     if (e == hipSuccess) e = b->cell_count.alloc(sizeof(int) * ncell);
     if (e == hipSuccess) e = hipMemsetAsync(b->cell_count.ptr, 0, sizeof(int) * ncell, st);
-    if (e == hipSuccess) e = b->cell_fill.alloc(sizeof(int) * ncell);
     if (e == hipSuccess) e = b->cell_start.alloc(sizeof(int) * (ncell + 1));
     if (e == hipSuccess) e = b->cell_sidx.alloc(sizeof(int) * na);
     if (e == hipSuccess) e = b->cell_spos.alloc(sizeof(float4) * na);
@@ -809,7 +808,6 @@ LayerArgs layer_args(smr_batch *b) {
     L.natoms = b->total_atoms;
     L.nsys = b->nsys;
     L.count = (int *)b->cell_count.ptr;
-    L.fill = (int *)b->cell_fill.ptr;
     L.start = (int *)b->cell_start.ptr;
     L.sidx = (int *)b->cell_sidx.ptr;
     L.spos = (float4 *)b->cell_spos.ptr;

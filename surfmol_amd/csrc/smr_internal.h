@@ -169,7 +169,7 @@ struct smr_batch {
     smr::DevBuf stencil_key;         // int [total_atoms] cell of each stored stencil
     // coupled layer (opts.inter_cutoff > 0)
     bool layer = false;
-    smr::DevBuf atom_sys, req_atom, cell_count, cell_fill, cell_start, cell_sidx, cell_spos, cell_sreq, atom_cell;
+    smr::DevBuf atom_sys, req_atom, cell_count, cell_start, cell_sidx, cell_spos, cell_sreq, atom_cell;
     smr::DevBuf atom_wrap, cell_swrap;
     smr::DevBuf cell_part, slot_of;
     smr::DevBuf partials, layer_state;

@@ -139,7 +139,7 @@ struct LayerArgs {
     const int *atom_sys;     // [natoms]
     const float4 *req_atom;  // [natoms]
     int natoms, nsys;
-    int *count, *fill, *start, *sidx;   // [ncells], [ncells], [ncells+1], [natoms]
+    int *count, *start, *sidx;   // [ncells] (counts / fill cursors beyond the LDS build), [ncells+1], [natoms]
     float4 *spos, *sreq;
     int *atom_cell;          // [natoms]
     int2 *atom_wrap;         // [natoms] box wraps of each atom (periodic layer)
