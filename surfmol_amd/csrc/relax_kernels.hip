@@ -96,8 +96,10 @@ __device__ __forceinline__ void refresh_stencils(const DevTables &T, bool need, 
 // One atom per thread: at most 256 registers, so that two waves share each
 // SIMD (the LDS of a <= 64-atom system admits two per SIMD as well); wider
 // shapes keep the compiler's choice (their per-thread atom arrays would
-// spill).
-template <int BLOCK, int APT, bool FINT_LDS>
+// spill).  LAYER: the launch carries a coupled layer's cell list (its
+// intermolecular partial forces and, in KMODE_LAYER, the shared FIRE
+// state); the code for it is compiled into these instances only.
+template <int BLOCK, int APT, bool FINT_LDS, bool LAYER>
 __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A) {
     constexpr int NW = BLOCK / 64;
     // one atom per thread: each atom's surface stencil lives in registers
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
     float *fint_x, *fint_y, *fint_z;
 
     // a coupled layer that has converged (or stopped) does no more work
-    if (A.mode == KMODE_LAYER && A.layer->status != SMR_SYS_RUNNING) return;
+    if (LAYER && A.mode == KMODE_LAYER && A.layer->status != SMR_SYS_RUNNING) return;
     // a launch of one step (coupled-layer steps, force evaluations) with a
     // workgroup for every system: each takes its own, without a contended
     // counter; otherwise systems are fetched through the atomic counter in
@@ -562,16 +564,23 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                             acc.add(f);
                         }
                     }
-                    if (A.cells.spos) {
+                    if (LAYER && A.cells.spos) {
                         // coupled layer: intermolecular forces of the
                         // neighbour cells (k_layer_inter), in cell order
                         const CellGrid &C = A.cells;
                         const int slot = C.slot_of[abase + i];
-                        // (cells without partners contribute exact zeros, skipped)
+                        // (cells without partners contribute exact zeros,
+                        // skipped); three loads in flight (K = 1, 3, 9, 27)
                         const int K = C.nox * C.noy * C.noz;
-                        for (int o = 0; o < K; o++) {
-                            const float4 pf4 = C.part[(size_t)o * C.natoms + slot];
-                            if (pf4.x != 0.f || pf4.y != 0.f || pf4.z != 0.f) acc.add(f3(pf4));
+                        for (int o0 = 0; o0 < K; o0 += 3) {
+                            float4 pf4[3];
+#pragma unroll
+                            for (int u = 0; u < 3; u++)
+                                pf4[u] = o0 + u < K ? C.part[(size_t)(o0 + u) * C.natoms + slot]
+                                                    : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                            for (int u = 0; u < 3; u++)
+                                if (pf4[u].x != 0.f || pf4[u].y != 0.f || pf4[u].z != 0.f) acc.add(f3(pf4[u]));
                         }
                     }
                     if (T.grid) {
@@ -631,7 +640,7 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
             // per-atom partial sums (FP64)
             double p_vf = 0.0, p_vv = 0.0, p_ff = 0.0, p_fm = 0.0;
             if (A.mode == KMODE_FIRE || A.mode == KMODE_LAYER) {
-                const float dtf = (float)(A.mode == KMODE_LAYER ? A.layer->dt : st.dt);
+                const float dtf = (float)(LAYER && A.mode == KMODE_LAYER ? A.layer->dt : st.dt);
                 const double flim = A.fire.flim;
 #pragma unroll
                 for (int k = 0; k < APT; k++) {
@@ -673,7 +682,7 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                 }
             }
             // block reduction of (vf, vv, ff, fmax^2)
-            if (A.mode == KMODE_LAYER) {
+            if (LAYER && A.mode == KMODE_LAYER) {
                 // coupled layer: clamped forces and updated velocities go to
                 // memory, this molecule's sums to its partial slot; the shared
                 // FIRE decision is taken by the next launch (k_layer_decide)
@@ -985,13 +994,17 @@ __global__ void k_plain_fire_apply(PlainArgs P, float a, float b, int reset, flo
 // ===========================================================================
 template <int BLOCK, int APT, bool FL>
 static hipError_t launch_one(const KernelArgs &A, int grid, size_t lds, hipStream_t stream) {
-    hipLaunchKernelGGL((k_relax<BLOCK, APT, FL>), dim3(grid), dim3(BLOCK), lds, stream, A);
+    if (A.cells.spos)
+        hipLaunchKernelGGL((k_relax<BLOCK, APT, FL, true>), dim3(grid), dim3(BLOCK), lds, stream, A);
+    else
+        hipLaunchKernelGGL((k_relax<BLOCK, APT, FL, false>), dim3(grid), dim3(BLOCK), lds, stream, A);
     return hipGetLastError();
 }
 
+// (occupancy of a shape: the layer instances use the same launch bounds)
 template <int BLOCK, int APT, bool FL>
 static const void *kernel_ptr() {
-    return reinterpret_cast<const void *>(&k_relax<BLOCK, APT, FL>);
+    return reinterpret_cast<const void *>(&k_relax<BLOCK, APT, FL, false>);
 }
 
 // instantiated shapes: (threads per workgroup, atoms per thread)
