@@ -347,9 +347,10 @@ __global__ __launch_bounds__(64) void k_layer_inter(LayerArgs L) {
             float3 f;
             const float e = ljq_pair(dp, ri, C.sreq[q], T.r2damp, f);
             if (r2 > C.ron2) {
-                // switching shell: E S, force (fr S + E S'/r) dp
-                const float r = sqrtf(r2), x = (r - C.ron) * C.inv_w;
-                const float S = 1.f - x * x * (3.f - 2.f * x), dS_r = 6.f * x * (x - 1.f) * C.inv_w / r;
+                // switching shell: E S, force (fr S + E S'/r) dp (1/r from
+                // one reciprocal square root: no divide, no sqrt)
+                const float ir = frsq(r2), r = r2 * ir, x = (r - C.ron) * C.inv_w;
+                const float S = 1.f - x * x * (3.f - 2.f * x), dS_r = 6.f * x * (x - 1.f) * C.inv_w * ir;
                 f = f * S + dp * (e * dS_r);
             }
             acc.add(f);
