@@ -1192,7 +1192,7 @@ int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double
             return hip_fail(e, "hipGraphInstantiate (layer steps)");
         }
     }
-    // the first step's work counters (each step's cell scan resets them for the next)
+    // the first step's work counters (each step's cell build resets them for the next)
     if (e == hipSuccess) e = launch_zero_i32((int *)b->ls.counters.ptr, NCLASS, st);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&e0);

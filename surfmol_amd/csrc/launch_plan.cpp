@@ -223,7 +223,7 @@ hipError_t issue_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<Cla
                        const std::vector<KernelArgs> &args, bool zero_counters) {
     const int naux = std::min<int>(3, (int)plans.size() - 1);
     hipError_t r = hipSuccess;
-    // the work counters start at zero (a coupled layer's cell scan zeroes
+    // the work counters start at zero (a coupled layer's cell build zeroes
     // them for the next step instead)
     if (zero_counters && (r = launch_zero_i32((int *)ls.counters.ptr, NCLASS, ss.main)) != hipSuccess) return r;
     if (naux > 0) {

@@ -148,7 +148,7 @@ struct LayerArgs {
     int *slot_of;            // [natoms]
     CellGrid cells;
     LayerState *layer;
-    int *work_counters;      // [nwork_counters] reset by the cell scan for the next force launch (or NULL)
+    int *work_counters;      // [nwork_counters] reset by the cell build for the next force launch (or NULL)
     int nwork_counters;
     const double4 *partials; // [nsys] per-molecule FIRE sums of the step's force launch
     FireConst fire;          // FIRE parameters of a relaxation (k_layer_decide)
