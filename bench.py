@@ -18,7 +18,7 @@ Metric: relaxed systems/s at Fmax < 1e-3 eV/A.
   flat arrangement: molecules are squeezed out of the layer and FIRE does not
   reach Fmax < 1e-3 (Fmax 0.71 eV/A after 2000 steps in FP64 and on the
   device alike, 3.6 after 100k device steps); at 13 A the layer relaxes in
-  about 6400 steps.  --layer-spacing 11 measures the step rate at 11 A.
+  about 6500-6700 steps (the count depends on the last bits of the forces).  --layer-spacing 11 measures the step rate at 11 A.
 --workload serve: the serving loop -- relax requests with Poisson arrivals
   (rates --rates, req/s), sizes lognormal (median 60 atoms, ln-sigma 0.5,
   clipped to [20, 600]), on the 48 x 48 x 20 A GridFF; reports p50/p99
