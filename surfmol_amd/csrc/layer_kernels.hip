@@ -20,9 +20,10 @@
 // (build and sort do nothing more on a step where no atom changed cell:
 // cells at least the cutoff wide keep their atoms for many steps)
 //   k_layer_inter          intermolecular forces: one workgroup per (cell,
-//                          neighbour cell), one lane per atom of the cell; one
-//                          partial force per atom and neighbour cell that the
-//                          next force launch adds in a fixed order
+//                          neighbour cell), one to four lanes per atom of the
+//                          cell; one partial force per atom and neighbour
+//                          cell that the next force launch adds in a fixed
+//                          order
 // All kernels are launched every step (no host decision); once the layer has
 // converged the force launch returns at once and the move is a no-op.
 #include "smr_device.h"
@@ -281,8 +282,10 @@ __global__ __launch_bounds__(256) void k_cells_sort(LayerArgs L) {
 // Intermolecular forces on the atoms of cell c from the atoms of its
 // neighbour cell number o (0..K-1, K = 27, or 9 / 3 / 1 when the grid is a
 // single cell thick along some axis): partners of other molecules within
-// the cutoff, in slot order, Kahan-summed.  Writes part[o][slot] for every
-// atom of the cell (zero when the neighbour cell lies outside the grid).
+// the cutoff, Kahan-summed in slot order by each of the atom's 1, 2 or 4
+// lanes (every 1st, 2nd or 4th partner), the lanes' sums then added
+// pairwise.  Writes part[o][slot] for every atom of the cell (zero when the
+// neighbour cell lies outside the grid).
 // Along a periodic axis the neighbour cell wraps around the box and carries
 // an image shift; the displacement of a pair is then
 //   (pj - pi) + ostep * ((gj - gi) + (image - wj + wi) * nL)
@@ -315,48 +318,60 @@ __global__ __launch_bounds__(64) void k_layer_inter(LayerArgs L) {
     const int cc = inside ? (z * C.ncy + y) * C.ncx + x : 0;
     const int q0 = inside ? C.start[cc] : 0, q1 = inside ? C.start[cc + 1] : 0;
     float4 *out = L.part + (size_t)o * L.natoms;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const float4 pi = C.spos[s0 + t];
-        const float4 ri = C.sreq[s0 + t];
-        const int si = __float_as_int(pi.w);
-        const SysDesc di = T.sys[si];
-        const int2 wi = C.swrap[s0 + t];
-        KahanF3 acc;
-        acc.init(make_float3(0.f, 0.f, 0.f));
-        int last = -1, ogx = 0, ogy = 0, ogz = 0;
-        for (int q = q0; q < q1; q++) {
-            const float4 pj = C.spos[q];
-            const int sj = __float_as_int(pj.w);
-            const int2 wj = C.swrap[q];
-            // box lengths between the images (0 on an open axis)
-            const int bx = imx - wj.x + wi.x, by = imy - wj.y + wi.y;
-            if (sj == si && bx == 0 && by == 0) continue;   // the molecule itself: intramolecular
-            if (sj != last) {
-                const SysDesc dj = T.sys[sj];
-                ogx = dj.gx - di.gx;
-                ogy = dj.gy - di.gy;
-                ogz = dj.gz - di.gz;
-                last = sj;
+    // nl adjacent lanes per atom when the cell is small (cells are about the
+    // cutoff wide and hold ~30 atoms): sub-lane k takes partners k, k+nl, ...
+    // of the neighbour cell, so the wave's lanes stay busy
+    const int nl = n <= 16 ? 4 : (n <= 32 ? 2 : 1);
+    const int k = threadIdx.x & (nl - 1);
+    for (int t0 = 0; t0 < n; t0 += 64 / nl) {
+        const int t = t0 + (int)threadIdx.x / nl;
+        float3 v = make_float3(0.f, 0.f, 0.f);
+        if (t < n) {
+            const float4 pi = C.spos[s0 + t];
+            const float4 ri = C.sreq[s0 + t];
+            const int si = __float_as_int(pi.w);
+            const SysDesc di = T.sys[si];
+            const int2 wi = C.swrap[s0 + t];
+            KahanF3 acc;
+            acc.init(make_float3(0.f, 0.f, 0.f));
+            int last = -1, ogx = 0, ogy = 0, ogz = 0;
+            for (int q = q0 + k; q < q1; q += nl) {
+                const float4 pj = C.spos[q];
+                const int sj = __float_as_int(pj.w);
+                const int2 wj = C.swrap[q];
+                // box lengths between the images (0 on an open axis)
+                const int bx = imx - wj.x + wi.x, by = imy - wj.y + wi.y;
+                if (sj == si && bx == 0 && by == 0) continue;   // the molecule itself: intramolecular
+                if (sj != last) {
+                    const SysDesc dj = T.sys[sj];
+                    ogx = dj.gx - di.gx;
+                    ogy = dj.gy - di.gy;
+                    ogz = dj.gz - di.gz;
+                    last = sj;
+                }
+                const float dgx = (float)(ogx + bx * C.nLx) * T.ostep_x;
+                const float dgy = (float)(ogy + by * C.nLy) * T.ostep_y;
+                const float dgz = (float)ogz * T.ostep_z;
+                const float3 dp = make_float3((pj.x - pi.x) + dgx, (pj.y - pi.y) + dgy, (pj.z - pi.z) + dgz);
+                const float r2 = dot3(dp, dp);
+                if (r2 >= C.rc2) continue;
+                float3 f;
+                const float e = ljq_pair(dp, ri, C.sreq[q], T.r2damp, f);
+                if (r2 > C.ron2) {
+                    // switching shell: E S, force (fr S + E S'/r) dp (1/r from
+                    // one reciprocal square root: no divide, no sqrt)
+                    const float ir = frsq(r2), r = r2 * ir, x = (r - C.ron) * C.inv_w;
+                    const float S = 1.f - x * x * (3.f - 2.f * x), dS_r = 6.f * x * (x - 1.f) * C.inv_w * ir;
+                    f = f * S + dp * (e * dS_r);
+                }
+                acc.add(f);
             }
-            const float dgx = (float)(ogx + bx * C.nLx) * T.ostep_x;
-            const float dgy = (float)(ogy + by * C.nLy) * T.ostep_y;
-            const float dgz = (float)ogz * T.ostep_z;
-            const float3 dp = make_float3((pj.x - pi.x) + dgx, (pj.y - pi.y) + dgy, (pj.z - pi.z) + dgz);
-            const float r2 = dot3(dp, dp);
-            if (r2 >= C.rc2) continue;
-            float3 f;
-            const float e = ljq_pair(dp, ri, C.sreq[q], T.r2damp, f);
-            if (r2 > C.ron2) {
-                // switching shell: E S, force (fr S + E S'/r) dp (1/r from
-                // one reciprocal square root: no divide, no sqrt)
-                const float ir = frsq(r2), r = r2 * ir, x = (r - C.ron) * C.inv_w;
-                const float S = 1.f - x * x * (3.f - 2.f * x), dS_r = 6.f * x * (x - 1.f) * C.inv_w * ir;
-                f = f * S + dp * (e * dS_r);
-            }
-            acc.add(f);
+            v = acc.value();
         }
-        const float3 v = acc.value();
-        out[s0 + t] = make_float4(v.x, v.y, v.z, 0.f);
+        // the atom's nl partial sums, added pairwise in a fixed order
+        for (int m = 1; m < nl; m <<= 1)
+            v = v + make_float3(__shfl_xor(v.x, m), __shfl_xor(v.y, m), __shfl_xor(v.z, m));
+        if (t < n && k == 0) out[s0 + t] = make_float4(v.x, v.y, v.z, 0.f);
     }
 }
 
