@@ -312,6 +312,11 @@ class FireParams:
                                   self.flim)
 
 
+# rows of smr_relax_result as numpy sees them (the ctypes layout)
+RESULT_DTYPE = np.dtype(_lib.SmrRelaxResult)
+assert RESULT_DTYPE.itemsize == ctypes.sizeof(_lib.SmrRelaxResult)
+
+
 def _results(arr):
     return [{"steps": r.steps, "status": r.status, "energy": r.energy, "fmax": r.fmax, "terms": list(r.terms)}
             for r in arr]
@@ -364,10 +369,15 @@ class Batch:
         return f, e
 
     def relax_batched(self, fire: FireParams, max_steps: int, fconv: float):
-        res = (_lib.SmrRelaxResult * self.nsys)()
+        """FIRE on every system.  Returns a numpy structured array, one row per
+        system, with the fields of smr_relax_result (steps, status, energy,
+        fmax, terms): the library writes the rows in place, so a large batch
+        pays no per-system conversion (row["steps"] reads like a dict)."""
+        res = np.zeros(self.nsys, dtype=RESULT_DTYPE)
         fp = fire.cstruct()
-        check(lib().smr_relax_batched(self._h, ctypes.byref(fp), int(max_steps), float(fconv), res))
-        return _results(res)
+        check(lib().smr_relax_batched(self._h, ctypes.byref(fp), int(max_steps), float(fconv),
+                                      res.ctypes.data_as(ctypes.POINTER(_lib.SmrRelaxResult))))
+        return res
 
     def relax_plain(self, isys: int, fire: FireParams, max_steps: int, fconv: float):
         res = _lib.SmrRelaxResult()
