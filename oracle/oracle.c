@@ -703,11 +703,72 @@ void orc_eval_forces(const orc_system *s, const double *apos, double *fapos, dou
 /* Integrators                                                          */
 /* ------------------------------------------------------------------ */
 
+/* One FIRE step on given forces f (modified in place by the clamp), the
+ * body of the loop of surfmol/tests/relax_pentacene_uff.rs:142-193 with
+ * FireState semantics from molff/src/raff.rs:1040-1062:
+ *   v += F dt; sums of v.F, |v|^2 over the updated v; if v.F > 0 and
+ *   |F|^2, |v|^2 > 1e-30: v = (1-a) v + a |v| F/|F|, n_pos++, and after
+ *   n_min such steps dt = min(f_inc dt, dt_max), a *= f_alpha; else n_pos = 0,
+ *   dt *= f_dec, a = a0, v = 0; finally x += v dt with the dt the step began
+ *   with.
+ * flim > 0 clamps each |F_i| to flim as step_fire does (raff.rs:1089-1096):
+ * the clamped F enters the velocity update, v.F and the mixing direction,
+ * while |F|^2 (and so |F| in the mixing) sums the unclamped forces.
+ * state[3] = (dt, alpha, n_pos), updated.  Returns max_i |F_i|^2 of the
+ * unclamped forces.                                                     */
+double orc_fire_step(int na, double *f, double *vapos, double *apos, double *state, const orc_fire_params *fp) {
+    double dt = state[0], alpha = state[1];
+    int n_pos = (int)state[2];
+    double v_dot_f = 0.0, v_norm2 = 0.0, f_norm2 = 0.0, f2max = 0.0;
+    for (int ia = 0; ia < na; ia++) {
+        v3 fi = v3_ld(f + 3 * ia);
+        double f2 = v3_dot(fi, fi);
+        if (f2 > f2max) f2max = f2;
+        if (fp->flim > 0.0 && f2 > fp->flim * fp->flim) {
+            fi = v3_mul(fi, fp->flim / sqrt(f2));
+            v3_st(f + 3 * ia, fi);
+        }
+        f_norm2 += f2;
+        v3 v = v3_ld(vapos + 3 * ia);
+        v = v3_add(v, v3_mul(fi, dt));
+        v3_st(vapos + 3 * ia, v);
+        v_dot_f += v3_dot(v, fi);
+        v_norm2 += v3_dot(v, v);
+    }
+    double dt_next = dt;
+    if (v_dot_f > 0.0 && f_norm2 > 1e-30 && v_norm2 > 1e-30) {
+        double v_mag = sqrt(v_norm2);
+        double f_mag = sqrt(f_norm2);
+        for (int ia = 0; ia < na; ia++) {
+            v3 fh = v3_mul(v3_ld(f + 3 * ia), 1.0 / f_mag);
+            v3 v = v3_ld(vapos + 3 * ia);
+            v3_st(vapos + 3 * ia, v3_lincomb(1.0 - alpha, v, alpha * v_mag, fh));
+        }
+        n_pos++;
+        if (n_pos > fp->n_min) {
+            dt_next = fmin(dt * fp->f_inc, fp->dt_max);
+            alpha *= fp->f_alpha;
+        }
+    } else {
+        n_pos = 0;
+        dt_next = dt * fp->f_dec;
+        alpha = fp->alpha0;
+        for (int i = 0; i < 3 * na; i++) vapos[i] = 0.0;
+    }
+    for (int ia = 0; ia < na; ia++) {
+        v3 p = v3_ld(apos + 3 * ia);
+        v3_st(apos + 3 * ia, v3_add(p, v3_mul(v3_ld(vapos + 3 * ia), dt)));
+    }
+    state[0] = dt_next;
+    state[1] = alpha;
+    state[2] = (double)n_pos;
+    return f2max;
+}
+
 /* FIRE loop of surfmol/tests/relax_pentacene_uff.rs:142-193 (identical to
- * relax_xylitol_uff.rs:43-100 and molengine relax_fire main.rs:237-282) with
- * FireState semantics from molff/src/raff.rs:1040-1062.  When flim > 0 the
- * force used for the velocity update and mixing is clamped to flim as in
- * raff.rs:1093 (the convergence test always uses the unclamped force).
+ * relax_xylitol_uff.rs:43-100 and molengine relax_fire main.rs:237-282):
+ * evaluate the forces, take orc_fire_step, stop when max_i |F_i| of the
+ * step's (unclamped) forces is below fconv -- tested after the move.
  *
  * Returns the number of steps taken (itr+1 at convergence, max_steps if not).
  * out[0] = total energy at the last force evaluation (sum of E6),
@@ -718,54 +779,12 @@ int orc_relax_fire(const orc_system *s, double *apos, double *vapos, const orc_f
                    int max_steps, double fconv, double *out) {
     int na = s->natoms;
     double *f = (double *)malloc(sizeof(double) * 3 * (size_t)na);
-    double dt_cur = fp->dt0, alpha = fp->alpha0;
-    int n_pos = 0;
+    double state[3] = {fp->dt0, fp->alpha0, 0.0};
     int steps = max_steps, conv = 0;
     double E6[6] = {0, 0, 0, 0, 0, 0}, fmax = 0.0;
     for (int itr = 0; itr < max_steps; itr++) {
         orc_eval_forces(s, apos, f, E6);
-        double dt = dt_cur;
-        double v_dot_f = 0.0, v_norm2 = 0.0, f_norm2 = 0.0, f2max = 0.0;
-        for (int ia = 0; ia < na; ia++) {
-            v3 fi = v3_ld(f + 3 * ia);
-            double f2 = v3_dot(fi, fi);
-            if (f2 > f2max) f2max = f2;
-            if (fp->flim > 0.0 && f2 > fp->flim * fp->flim) {
-                fi = v3_mul(fi, fp->flim / sqrt(f2));
-                v3_st(f + 3 * ia, fi);
-                f2 = v3_dot(fi, fi);
-            }
-            f_norm2 += f2;
-            v3 v = v3_ld(vapos + 3 * ia);
-            v = v3_add(v, v3_mul(fi, dt));
-            v3_st(vapos + 3 * ia, v);
-            v_dot_f += v3_dot(v, fi);
-            v_norm2 += v3_dot(v, v);
-        }
-        if (v_dot_f > 0.0 && f_norm2 > 1e-30 && v_norm2 > 1e-30) {
-            double v_mag = sqrt(v_norm2);
-            double f_mag = sqrt(f_norm2);
-            for (int ia = 0; ia < na; ia++) {
-                v3 fh = v3_mul(v3_ld(f + 3 * ia), 1.0 / f_mag);
-                v3 v = v3_ld(vapos + 3 * ia);
-                v3_st(vapos + 3 * ia, v3_lincomb(1.0 - alpha, v, alpha * v_mag, fh));
-            }
-            n_pos++;
-            if (n_pos > fp->n_min) {
-                dt_cur = fmin(dt_cur * fp->f_inc, fp->dt_max);
-                alpha *= fp->f_alpha;
-            }
-        } else {
-            n_pos = 0;
-            dt_cur *= fp->f_dec;
-            alpha = fp->alpha0;
-            for (int i = 0; i < 3 * na; i++) vapos[i] = 0.0;
-        }
-        for (int ia = 0; ia < na; ia++) {
-            v3 p = v3_ld(apos + 3 * ia);
-            v3_st(apos + 3 * ia, v3_add(p, v3_mul(v3_ld(vapos + 3 * ia), dt)));
-        }
-        fmax = sqrt(f2max);
+        fmax = sqrt(orc_fire_step(na, f, vapos, apos, state, fp));
         if (fmax < fconv) { steps = itr + 1; conv = 1; break; }
     }
     out[0] = E6[0] + E6[1] + E6[2] + E6[3] + E6[4] + E6[5];
@@ -776,11 +795,34 @@ int orc_relax_fire(const orc_system *s, double *apos, double *vapos, const orc_f
     return steps;
 }
 
-/* MolWorld::run_md (surfmol/src/mol_world.rs:141-164) with
- * DynamicAtoms::move_atom_md (moltopo/src/molecular.rs:118-137): damped MD,
- * velocities zeroed when sum v.F < 0, stop when sum |F|^2 < fconv^2.  The
- * v.F and |F|^2 sums use the pre-move velocity and the unclamped force, as
- * the reference does.  Returns steps taken.                                */
+/* One damped-MD step of MolWorld::run_md (surfmol/src/mol_world.rs:145-160)
+ * on given forces: DynamicAtoms::move_atom_md (moltopo/src/molecular.rs:
+ * 118-137) per atom -- the sums take the velocity before the update and the
+ * unclamped force; |F| is clamped to flim for the update;
+ * v = cdamp v + F_clamped dt; x += v dt -- then, after every atom has moved,
+ * all velocities are zeroed when sum v.F < 0.  sums[3] = (sum v.F, sum |v|^2,
+ * sum |F|^2).  (run_md names these ff, vv and vf: mol_world.rs:150-157.)   */
+void orc_md_step(int na, const double *f, double *vapos, double *apos, double dt, double flim, double cdamp,
+                 double *sums) {
+    double vdotf = 0.0, vv = 0.0, ff = 0.0;
+    for (int i = 0; i < na; i++) {
+        v3 fi = v3_ld(f + 3 * i), v = v3_ld(vapos + 3 * i), p = v3_ld(apos + 3 * i);
+        double a = v3_dot(v, fi), b = v3_dot(v, v), f2 = v3_dot(fi, fi);
+        v3 fc = fi;
+        if (f2 > flim * flim) fc = v3_mul(fc, flim / sqrt(f2));
+        v3 vn = v3_add(v3_mul(v, cdamp), v3_mul(fc, dt));
+        v3 pn = v3_add(p, v3_mul(vn, dt));
+        v3_st(apos + 3 * i, pn);
+        v3_st(vapos + 3 * i, vn);
+        vdotf += a; vv += b; ff += f2;
+    }
+    if (vdotf < 0.0) for (int i = 0; i < 3 * na; i++) vapos[i] = 0.0;
+    sums[0] = vdotf; sums[1] = vv; sums[2] = ff;
+}
+
+/* MolWorld::run_md (surfmol/src/mol_world.rs:141-164): damped MD, velocities
+ * zeroed when sum v.F < 0, stop when sum |F|^2 < fconv^2 (tested after the
+ * move).  cdamp = max(0, 1 - damping).  Returns steps taken.               */
 int orc_run_md(const orc_system *s, double *apos, double *vapos, int niter, double dt, double fconv,
                double flim, double damping) {
     int na = s->natoms;
@@ -790,23 +832,10 @@ int orc_run_md(const orc_system *s, double *apos, double *vapos, int niter, doub
     if (cdamp < 0.0) cdamp = 0.0;
     int ret = niter;
     for (int itr = 0; itr < niter; itr++) {
-        double E6[6];
+        double E6[6], sums[3];
         orc_eval_forces(s, apos, f, E6);
-        double vdotf = 0.0, vv = 0.0, ff = 0.0;
-        for (int i = 0; i < na; i++) {
-            v3 fi = v3_ld(f + 3 * i), v = v3_ld(vapos + 3 * i), p = v3_ld(apos + 3 * i);
-            double a = v3_dot(v, fi), b = v3_dot(v, v), f2 = v3_dot(fi, fi);
-            v3 fc = fi;
-            if (f2 > flim * flim) fc = v3_mul(fc, flim / sqrt(f2));
-            v3 vn = v3_add(v3_mul(v, cdamp), v3_mul(fc, dt));
-            v3 pn = v3_add(p, v3_mul(vn, dt));
-            v3_st(apos + 3 * i, pn);
-            v3_st(vapos + 3 * i, vn);
-            vdotf += a; vv += b; ff += f2;
-        }
-        (void)vv;
-        if (vdotf < 0.0) for (int i = 0; i < 3 * na; i++) vapos[i] = 0.0;
-        if (ff < f2conv) { ret = itr + 1; break; }
+        orc_md_step(na, f, vapos, apos, dt, flim, cdamp, sums);
+        if (sums[2] < f2conv) { ret = itr + 1; break; }
     }
     free(f);
     return ret;

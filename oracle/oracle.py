@@ -20,7 +20,8 @@ Two independent halves:
   8-123), the .dat loaders and UFF formulas (moltopo/src/params.rs:120-657)
   and parameter assignment (molff/src/uff.rs:250-315,699-887).
 
-How each part is pinned (tests/test_oracle_*.py):
+How each part is pinned (tests/test_oracle_pins.py, test_uff_branch_pins.py,
+test_integrator_pins.py, test_layer_oracle.py):
 
 =====================  ======================================================
 part                   pin
@@ -33,21 +34,27 @@ angle/dihedral/inv.    finite-difference gradient of the restated energy
                        sum F = 0, sum r x F = 0; zero dihedral/inversion
                        energy on the planar pentacene (uff.md:91-93)
 UFF params/typing      pentacene golden numbers E_bond=3.83, E_angle=0.01,
-                       E_dih=0, E_inv=0, Fz=-11.7 (uff.md:91-93)
+                       E_dih=0, E_inv=0, Fz=-11.7 (uff.md:91-93); every
+                       dihedral/inversion/angle branch the workload molecules
+                       reach, against values written out from the table
+                       columns and closed forms (tests/test_uff_branch_pins.py)
 non-bonded             closed forms at r=R0 (E=-E0, F=0), Coulomb closed
                        form, FD gradient, conventions cross-checked
 GridFF sampler         partition of unity, exact reproduction of cubic
                        fields, periodic wrap, FD gradient
 grid build / fit       brute force on tiny inputs vs direct Morse/Coulomb
                        sums; S*C = E after the fit
-FIRE / damped MD       momentum conservation (relax_xylitol_uff.rs:228-229),
-                       convergence to Fmax<1e-3, same energy from two starts
+FIRE step              a 30-step 1-D harmonic-well sequence derived in exact
+                       arithmetic (n_min transition, dt growth and cap, uphill
+                       reset); mixing and the flim clamp on one step
+                       (tests/test_integrator_pins.py)
+damped MD step         one-step closed form with clamp, damping and the
+                       after-the-move purge; momentum conservation
+                       (relax_xylitol_uff.rs:228-229)
 =====================  ======================================================
 
-Functions with no such pin: none in the list above; the special-case
-branches of UFF typing/params that the test molecules never reach (O/S sp3-sp3
-dihedrals, sp3-sp2 with an sp2 neighbour, group-15 inversions) are restated
-but "parity unpinned".
+Functions with no such pin: none.  Branches of setup_params that no workload
+reaches (S atoms, P inversions, sp1 centres) are restated but not pinned.
 """
 from __future__ import annotations
 
@@ -101,6 +108,9 @@ def lib():
         L.orc_run_md.argtypes = [P(OrcSystem), P(d), P(d), i, d, d, d, d]
         L.orc_run_md.restype = i
         L.orc_md_steps.argtypes = [P(OrcSystem), P(d), P(d), i, d, d, d]
+        L.orc_fire_step.argtypes = [i, P(d), P(d), P(d), P(d), P(OrcFire)]
+        L.orc_fire_step.restype = d
+        L.orc_md_step.argtypes = [i, P(d), P(d), P(d), d, d, d, P(d)]
         L.orc_make_second_neighs.argtypes = [i, P(i), P(i)]
         L.orc_make_second_neighs.restype = i
         L.orc_fe3d_pbc_comb.argtypes = [P(d), i, i, i, P(d), P(d), P(d)]
@@ -889,6 +899,31 @@ def run_md(sys: OracleSystem, niter, dt, fconv, flim, damping, apos=None, vapos=
     steps = lib().orc_run_md(ctypes.byref(cs), _pd(x), _pd(v), int(niter), float(dt), float(fconv), float(flim),
                              float(damping))
     return steps, x, v
+
+
+def fire_step(forces, apos, vapos, state, fire: FireParams):
+    """One FIRE step (orc_fire_step) on given forces [N,3].  state = [dt,
+    alpha, n_pos] is updated in place.  Returns (apos, vapos, clamped forces,
+    max_i |F_i|^2 of the given forces)."""
+    f = _f64(forces, (-1, 3)).copy()
+    x = _f64(apos, (-1, 3)).copy()
+    v = _f64(vapos, (-1, 3)).copy()
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    fc = fire.cstruct()
+    f2max = lib().orc_fire_step(len(x), _pd(f), _pd(v), _pd(x), _pd(st), ctypes.byref(fc))
+    state[:] = st
+    return x, v, f, f2max
+
+
+def md_step(forces, apos, vapos, dt, flim, cdamp):
+    """One damped-MD step of run_md (orc_md_step) on given forces [N,3].
+    Returns (apos, vapos, (sum v.F, sum |v|^2, sum |F|^2))."""
+    f = _f64(forces, (-1, 3)).copy()
+    x = _f64(apos, (-1, 3)).copy()
+    v = _f64(vapos, (-1, 3)).copy()
+    sums = np.zeros(3)
+    lib().orc_md_step(len(x), _pd(f), _pd(v), _pd(x), float(dt), float(flim), float(cdamp), _pd(sums))
+    return x, v, tuple(sums)
 
 
 def md_steps(sys: OracleSystem, nsteps, dt, flim, cdamp, apos, vapos):

@@ -646,15 +646,17 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                 for (int k = 0; k < APT; k++) {
                     if (tid + k * BLOCK < N) {
                         double fx = F[k].x, fy = F[k].y, fz = F[k].z;
-                        double f2 = fx * fx + fy * fy + fz * fz;
+                        const double f2 = fx * fx + fy * fy + fz * fz;
                         p_fm = fmax(p_fm, f2);
+                        // |F| clamp of step_fire (raff.rs:1089-1096): the
+                        // clamped force drives v and v.F, |F|^2 sums the
+                        // unclamped one
+                        p_ff += f2;
                         if (flim > 0.0 && f2 > flim * flim) {
                             float sc = (float)(flim / sqrt(f2));
                             F[k] = F[k] * sc;
                             fx = F[k].x; fy = F[k].y; fz = F[k].z;
-                            f2 = fx * fx + fy * fy + fz * fz;
                         }
-                        p_ff += f2;
                         my_v[k] = my_v[k] + F[k] * dtf;
                         double vx = my_v[k].x, vy = my_v[k].y, vz = my_v[k].z;
                         p_vf += vx * fx + vy * fy + vz * fz;
@@ -938,15 +940,14 @@ __global__ void k_plain_fire_reduce(PlainArgs P, double dt, double flim) {
         float4 F4 = P.force[i];
         float3 F = f3(F4);
         double fx = F.x, fy = F.y, fz = F.z;
-        double f2 = fx * fx + fy * fy + fz * fz;
+        const double f2 = fx * fx + fy * fy + fz * fz;
         r[3] = fmax(r[3], f2);
+        r[2] += f2;   // unclamped |F|^2, as step_fire (raff.rs:1089-1096)
         if (flim > 0.0 && f2 > flim * flim) {
             F = F * (float)(flim / sqrt(f2));
             P.force[i] = make_float4(F.x, F.y, F.z, 0.f);
             fx = F.x; fy = F.y; fz = F.z;
-            f2 = fx * fx + fy * fy + fz * fz;
         }
-        r[2] += f2;
         float4 v4 = P.vel[i];
         float3 v = f3(v4) + F * dtf;
         P.vel[i] = make_float4(v.x, v.y, v.z, 0.f);
