@@ -76,7 +76,11 @@ typedef struct {
      * interacts as an intermolecular pair -- except an atom with the same
      * image of its own molecule, which is the intramolecular pair. */
     double pbc_lx, pbc_ly;
+    /* optional [natoms]: each force contribution's magnitude is added to its
+     * atom's entry (the scale of the per-atom force sum; NULL = off) */
+    double *fscale;
 } orc_system;
+
 
 typedef struct {
     double dt0, dt_max;       /* FireState::new(dt0, dt_max) raff.rs:1057-1060 */
@@ -105,6 +109,9 @@ static inline v3 v3_lincomb(double a, v3 u, double b, v3 w) {
     v3 r = {a * u.x + b * w.x, a * u.y + b * w.y, a * u.z + b * w.z};
     return r;
 }
+/* per-atom force scale: adds |f| of one contribution to atom ia's entry */
+#define ORC_SCALE(s, ia, f) do { if ((s)->fscale) (s)->fscale[(ia)] += sqrt(v3_dot((f), (f))); } while (0)
+
 /* Vec2d::mul_cmplx */
 static inline void cmul(double *x, double *y, double cx, double cy) {
     double nx = (*x) * cx - (*y) * cy;
@@ -144,6 +151,7 @@ static double orc_eval_atom_bonds(const orc_system *s, int ia, const double *apo
         v3 f = v3_mul(dp, fr);
         v3 fa = v3_ld(fapos + 3 * ia);
         v3_st(fapos + 3 * ia, v3_add(fa, f));
+        ORC_SCALE(s, ia, f);
     }
     return e;
 }
@@ -273,7 +281,11 @@ static void orc_assemble(const orc_system *s, const double *fint, double *fapos)
     for (int i = 0; i < s->nangles; i++) for (int k = 0; k < 3; k++) { int a = s->ang_atoms[3 * i + k]; obj[off[a] + cnt[a]++] = i0ang + 3 * i + k; }
     for (int ia = 0; ia < na; ia++) {
         v3 f = v3_ld(fapos + 3 * ia);
-        for (int i = off[ia]; i < off[ia + 1]; i++) f = v3_add(f, v3_ld(fint + 3 * obj[i]));
+        for (int i = off[ia]; i < off[ia + 1]; i++) {
+            v3 fp = v3_ld(fint + 3 * obj[i]);
+            f = v3_add(f, fp);
+            ORC_SCALE(s, ia, fp);
+        }
         v3_st(fapos + 3 * ia, f);
     }
     free(cnt); free(off); free(obj);
@@ -419,6 +431,7 @@ static double orc_nb_eval_layer(const orc_system *s, const double *apos, double 
             v3 f;
             e += orc_ljqh_gpu(v3_sub(v3_ld(apos + 3 * ja), pi), ri[0] + rj[0], ri[1] * rj[1], ri[2] * rj[2], r2damp, &f);
             fe = v3_add(fe, f);
+            ORC_SCALE(s, ia, f);
         }
         const int cx = cell[ia] % n[0], cy = (cell[ia] / n[0]) % n[1], cz = cell[ia] / (n[0] * n[1]);
         for (int z = cz - 1; z <= cz + 1; z++) {
@@ -436,6 +449,7 @@ static double orc_nb_eval_layer(const orc_system *s, const double *apos, double 
                         v3 f;
                         e += orc_inter_pair(s, dp, ri, s->reqs + 4 * ja, r2damp, &f);
                         fe = v3_add(fe, f);
+                        ORC_SCALE(s, ia, f);
                     }
                 }
             }
@@ -490,6 +504,7 @@ double orc_nb_eval(const orc_system *s, const double *apos, double *fapos) {
                             e += orc_inter_pair(s, dp, ri, rj, r2damp, &f);
                         }
                         fe = v3_add(fe, f);
+                        ORC_SCALE(s, ia, f);
                     }
                 }
             }
@@ -519,6 +534,7 @@ double orc_nb_eval(const orc_system *s, const double *apos, double *fapos) {
                     e += orc_ljqh_gpu(dp, ri[0] + rj[0], ri[1] * rj[1], ri[2] * rj[2], r2damp, &f);
                 }
                 fe = v3_add(fe, f);
+                ORC_SCALE(s, ia, f);
             }
             etot += e;
             v3_st(fapos + 3 * ia, v3_add(v3_ld(fapos + 3 * ia), fe));
@@ -554,6 +570,7 @@ double orc_nb_eval(const orc_system *s, const double *apos, double *fapos) {
                 }
                 e += eij;
                 fi = v3_add(fi, f);
+                ORC_SCALE(s, ia, f);
             }
             etot += e;
             v3_st(fapos + 3 * ia, v3_add(v3_ld(fapos + 3 * ia), fi));
@@ -682,7 +699,11 @@ double orc_grid_eval(const orc_system *s, const double *apos, double *fapos) {
         orc_plqh(s->reqs + 4 * ia, s->alpha_morse, plqh);
         for (int k = 0; k < 3; k++) u[k] = (apos[3 * ia + k] - s->gp0[k]) * s->ginv_step[k];
         orc_fe3d_pbc_comb(u, s->gnx, s->gny, s->gnz, s->grid, plqh, fe);
-        for (int k = 0; k < 3; k++) fapos[3 * ia + k] += -fe[k] * s->ginv_step[k];
+        v3 fs = {-fe[0] * s->ginv_step[0], -fe[1] * s->ginv_step[1], -fe[2] * s->ginv_step[2]};
+        fapos[3 * ia + 0] += fs.x;
+        fapos[3 * ia + 1] += fs.y;
+        fapos[3 * ia + 2] += fs.z;
+        ORC_SCALE(s, ia, fs);
         etot += fe[3];
     }
     return etot;
@@ -875,38 +896,56 @@ void orc_md_steps(const orc_system *s, double *apos, double *vapos, int nsteps, 
  *   Coul   += kC * Q / sqrt(r^2 + rdamp^2)
  * Output: out[4*(iz + nz*(iy + ny*ix)) + {0,1,2}] (z-fastest, channel 3 = 0),
  * i.e. already in the packed layout of GridFF.h pack_Bspline_d.            */
+/* Reference-grid values (Pauli, London, Coulomb) at node (ix, iy, iz): the
+ * body of make_GridFF's per-node loop (opencl/gridff_build.cl:1006-1052). */
+static void orc_gridff_node(int natoms, const double *apos, const double *reqs, const double *lvec,
+                            int nx, int ny, int nz, const double *p0, const int *npbc, double alpha,
+                            double r2damp, int ix, int iy, int iz, double *o) {
+    double pos[3];
+    for (int k = 0; k < 3; k++)
+        pos[k] = p0[k] + lvec[0 + k] * ix / (double)nx + lvec[3 + k] * iy / (double)ny + lvec[6 + k] * iz / (double)nz;
+    double eP = 0.0, eL = 0.0, eC = 0.0;
+    for (int ja = 0; ja < natoms; ja++) {
+        const double *R = reqs + 4 * ja;
+        for (int jz = -npbc[2]; jz <= npbc[2]; jz++)
+            for (int jy = -npbc[1]; jy <= npbc[1]; jy++)
+                for (int jx = -npbc[0]; jx <= npbc[0]; jx++) {
+                    double d[3];
+                    for (int k = 0; k < 3; k++)
+                        d[k] = pos[k] - apos[3 * ja + k] + lvec[0 + k] * jx + lvec[3 + k] * jy + lvec[6 + k] * jz;
+                    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    double r = sqrt(r2 + 1e-32);
+                    double ir2 = 1.0 / (r2 + r2damp);
+                    eC += ORC_COULOMB_CONST * R[2] * sqrt(ir2);
+                    double e = exp(-alpha * (r - R[0]));
+                    double eM = R[1] * e;
+                    eP += eM * e;
+                    eL += -2.0 * eM;
+                }
+    }
+    o[0] = eP; o[1] = eL; o[2] = eC; o[3] = 0.0;
+}
+
 void orc_gridff_build(int natoms, const double *apos, const double *reqs, const double *lvec,
                       int nx, int ny, int nz, const double *p0, const int *npbc,
                       double alpha, double rdamp, double *out) {
     double r2damp = rdamp * rdamp;
     for (int ix = 0; ix < nx; ix++)
         for (int iy = 0; iy < ny; iy++)
-            for (int iz = 0; iz < nz; iz++) {
-                double pos[3];
-                for (int k = 0; k < 3; k++)
-                    pos[k] = p0[k] + lvec[0 + k] * ix / (double)nx + lvec[3 + k] * iy / (double)ny + lvec[6 + k] * iz / (double)nz;
-                double eP = 0.0, eL = 0.0, eC = 0.0;
-                for (int ja = 0; ja < natoms; ja++) {
-                    const double *R = reqs + 4 * ja;
-                    for (int jz = -npbc[2]; jz <= npbc[2]; jz++)
-                        for (int jy = -npbc[1]; jy <= npbc[1]; jy++)
-                            for (int jx = -npbc[0]; jx <= npbc[0]; jx++) {
-                                double d[3];
-                                for (int k = 0; k < 3; k++)
-                                    d[k] = pos[k] - apos[3 * ja + k] + lvec[0 + k] * jx + lvec[3 + k] * jy + lvec[6 + k] * jz;
-                                double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                                double r = sqrt(r2 + 1e-32);
-                                double ir2 = 1.0 / (r2 + r2damp);
-                                eC += ORC_COULOMB_CONST * R[2] * sqrt(ir2);
-                                double e = exp(-alpha * (r - R[0]));
-                                double eM = R[1] * e;
-                                eP += eM * e;
-                                eL += -2.0 * eM;
-                            }
-                }
-                double *o = out + 4 * ((size_t)iz + (size_t)nz * ((size_t)iy + (size_t)ny * (size_t)ix));
-                o[0] = eP; o[1] = eL; o[2] = eC; o[3] = 0.0;
-            }
+            for (int iz = 0; iz < nz; iz++)
+                orc_gridff_node(natoms, apos, reqs, lvec, nx, ny, nz, p0, npbc, alpha, r2damp, ix, iy, iz,
+                                out + 4 * ((size_t)iz + (size_t)nz * ((size_t)iy + (size_t)ny * (size_t)ix)));
+}
+
+/* The same values at a list of nodes only: nodes[3*q] = (ix, iy, iz),
+ * out[4*q] = (Pauli, London, Coulomb, 0) -- spot checks of full-size grids. */
+void orc_gridff_build_nodes(int natoms, const double *apos, const double *reqs, const double *lvec,
+                            int nx, int ny, int nz, const double *p0, const int *npbc,
+                            double alpha, double rdamp, int nnodes, const int *nodes, double *out) {
+    double r2damp = rdamp * rdamp;
+    for (int q = 0; q < nnodes; q++)
+        orc_gridff_node(natoms, apos, reqs, lvec, nx, ny, nz, p0, npbc, alpha, r2damp, nodes[3 * q],
+                        nodes[3 * q + 1], nodes[3 * q + 2], out + 4 * (size_t)q);
 }
 
 /* Apply the B-spline node operator S = Sx (x) Sy (x) Sz with the 1-D stencil

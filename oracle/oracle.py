@@ -116,6 +116,7 @@ def lib():
         L.orc_fe3d_pbc_comb.argtypes = [P(d), i, i, i, P(d), P(d), P(d)]
         L.orc_plqh.argtypes = [P(d), d, P(d)]
         L.orc_gridff_build.argtypes = [i, P(d), P(d), P(d), i, i, i, P(d), P(i), d, d, P(d)]
+        L.orc_gridff_build_nodes.argtypes = [i, P(d), P(d), P(d), i, i, i, P(d), P(i), d, d, i, P(i), P(d)]
         L.orc_bspline_apply.argtypes = [i, i, i, P(d), P(d)]
         L.orc_bspline_fit.argtypes = [i, i, i, P(d), P(d)]
         _lib = L
@@ -144,6 +145,7 @@ class OrcSystem(ctypes.Structure):
         ("alpha_morse", ctypes.c_double),
         ("mol_id", ctypes.POINTER(ctypes.c_int)), ("rcut_inter2", ctypes.c_double), ("layer_cells", ctypes.c_int),
         ("rswitch_inter", ctypes.c_double), ("pbc_lx", ctypes.c_double), ("pbc_ly", ctypes.c_double),
+        ("fscale", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -807,6 +809,38 @@ def system_from_geometry(elems, apos, params: Params, charges=None, rcut_bond=1.
         inv_ngs=bake_inversion_neighs(inversions, nl), reqs=reqs, **kw)
 
 
+def system_from_terms(types, apos, bonds, angles, dihedrals, inversions, params: Params, charges=None,
+                      **kw) -> OracleSystem:
+    """A system from given term lists, kept in their order: Uff::from_topology
+    (molff/src/uff.rs:238-240) takes the topology's bonds, angles, dihedrals
+    and inversions as they are, then the set-up of relax_pentacene_uff.rs:
+    53-78 follows (neighbour tables, setup_params with the given UFF types,
+    hneigh slots, REQH).  Used for topologies read from the reference's file
+    formats, whose dihedrals come in the writer's HashSet order."""
+    apos = np.asarray(apos, dtype=np.float64)
+    n = len(types)
+    bonds = [tuple(int(x) for x in b) for b in bonds]
+    angles = [tuple(int(x) for x in a) for a in angles]
+    dihedrals = [tuple(int(x) for x in d) for d in dihedrals]
+    inversions = [tuple(int(x) for x in v) for v in inversions]
+    neighs, neigh_bs = make_neigh_bs(n, bonds)
+    nl = neighs.tolist()
+    bp, ap, dp, ip = setup_params(params, list(types), nl, bonds, angles, dihedrals, inversions)
+    reqs = np.array([get_reqh(params, t) for t in types], dtype=np.float64)
+    if charges is not None:
+        reqs[:, 2] = np.asarray(charges, dtype=np.float64)
+    elems = [params.atom_types[t].element if t in params.atom_types else _elem_sym(t) for t in types]
+    return OracleSystem(
+        elems=elems, apos=apos.copy(), bonds=np.array(bonds, dtype=np.int32).reshape(-1, 2),
+        angles=np.array(angles, dtype=np.int32).reshape(-1, 3),
+        dihedrals=np.array(dihedrals, dtype=np.int32).reshape(-1, 4),
+        inversions=np.array(inversions, dtype=np.int32).reshape(-1, 4),
+        neighs=neighs, neigh_bs=neigh_bs, types=list(types),
+        bon_params=bp, ang_params=ap, dih_params=dp, inv_params=ip,
+        ang_ngs=bake_angle_neighs(angles, nl), dih_ngs=bake_dihedral_neighs(dihedrals, nl),
+        inv_ngs=bake_inversion_neighs(inversions, nl), reqs=reqs, **kw)
+
+
 def layer_from_molecules(mols, positions, params: Params, rcut_inter: float, **kw) -> OracleSystem:
     """A coupled layer: the union of several molecules as one system whose
     pairs between different molecules interact within rcut_inter only (the
@@ -851,6 +885,22 @@ def grid_eval(sys: OracleSystem, apos=None):
     f = np.zeros_like(x)
     e = lib().orc_grid_eval(ctypes.byref(cs), _pd(x), _pd(f))
     return f, e
+
+
+def force_scale(sys: OracleSystem, apos=None):
+    """Per atom, the sum of the magnitudes of every force contribution the
+    atom's total is summed from (bond, each angle/dihedral/inversion piece,
+    each pair, the surface): the scale against which the rounding error of
+    that sum is measured.  Near a minimum the total goes to zero while these
+    contributions stay of order eV/A."""
+    cs = sys.cstruct()
+    x = _f64(sys.apos if apos is None else apos, (-1, 3))
+    scale = np.zeros(len(x))
+    cs.fscale = _pd(scale)
+    f = np.zeros_like(x)
+    E = np.zeros(6)
+    lib().orc_eval_forces(ctypes.byref(cs), _pd(x), _pd(f), _pd(E))
+    return scale
 
 
 def eval_forces(sys: OracleSystem, apos=None):
@@ -963,6 +1013,18 @@ def gridff_build(sub_pos, sub_reqs, lvec, ns, p0, npbc, alpha, rdamp):
     lib().orc_gridff_build(len(sp), _pd(sp), _pd(sr), _pd(lv), nx, ny, nz, _pd(p), _pi(nb), float(alpha),
                            float(rdamp), _pd(out))
     return out.reshape(nx, ny, nz, 4)
+
+
+def gridff_build_nodes(sub_pos, sub_reqs, lvec, ns, p0, npbc, alpha, rdamp, nodes):
+    """Reference-grid values at the listed nodes [(ix, iy, iz), ...] -> [n, 4]."""
+    sp = _f64(sub_pos, (-1, 3))
+    sr = _f64(sub_reqs, (-1, 4))
+    nd = _i32(nodes, (-1, 3))
+    out = np.zeros((len(nd), 4))
+    nx, ny, nz = (int(x) for x in ns)
+    lib().orc_gridff_build_nodes(len(sp), _pd(sp), _pd(sr), _pd(_f64(lvec, (9,))), nx, ny, nz, _pd(_f64(p0, (3,))),
+                                 _pi(_i32(npbc, (3,))), float(alpha), float(rdamp), len(nd), _pi(nd), _pd(out))
+    return out
 
 
 def bspline_apply(C, ns):

@@ -33,6 +33,7 @@ from ._lib import (DATA_DIR, SMR_SYS_CONVERGED, SMR_SYS_MAXSTEPS, SMR_SYS_NONFIN
                    check, lib)
 
 __all__ = ["Molecule", "Context", "GridFF", "GridSpec", "Batch", "Server", "FireParams", "substrate_reqs", "SmrError",
+           "RESULT_DTYPE", "to_dicts",
            "DATA_DIR", "SMR_SYS_CONVERGED",
            "SMR_SYS_MAXSTEPS", "SMR_SYS_NONFINITE", "SMR_SYS_RUNNING", "lib"]
 
@@ -322,6 +323,15 @@ def _results(arr):
             for r in arr]
 
 
+def to_dicts(results):
+    """Relax results (a RESULT_DTYPE array or row, as relax_batched and
+    relax_plain return) as a list of plain Python dicts -- int/float fields and
+    a list of six energy terms -- e.g. for json.dumps."""
+    rows = np.atleast_1d(results)
+    return [{"steps": int(r["steps"]), "status": int(r["status"]), "energy": float(r["energy"]),
+             "fmax": float(r["fmax"]), "terms": [float(x) for x in r["terms"]]} for r in rows]
+
+
 class Batch:
     """A set of systems on the device (shared topologies, per-system state)."""
 
@@ -380,11 +390,13 @@ class Batch:
         return res
 
     def relax_plain(self, isys: int, fire: FireParams, max_steps: int, fconv: float):
-        res = _lib.SmrRelaxResult()
+        """The plain method on system isys.  Returns one row of the type
+        relax_batched returns (RESULT_DTYPE; ``to_dicts`` converts)."""
+        res = np.zeros(1, dtype=RESULT_DTYPE)
         fp = fire.cstruct()
         check(lib().smr_relax_plain(self._h, int(isys), ctypes.byref(fp), int(max_steps), float(fconv),
-                                    ctypes.byref(res)))
-        return _results([res])[0]
+                                    res.ctypes.data_as(ctypes.POINTER(_lib.SmrRelaxResult))))
+        return res[0]
 
     def run_md(self, niter, dt, fconv, flim, damping):
         steps = np.zeros(self.nsys, dtype=np.int32)
@@ -479,7 +491,9 @@ class Server:
         return rid.value
 
     def poll(self, rid: int, positions=True):
-        """None while the request runs; else a result dict (forgets the request)."""
+        """None while the request runs; else a dict (forgets the request) with
+        plain Python values: steps, status, energy, fmax, terms (list), the
+        submit/retire times and latency [s], and positions [N,3] when asked."""
         done = ctypes.c_int()
         res = _lib.SmrRelaxResult()
         n = self._natoms.get(rid)

@@ -1514,6 +1514,10 @@ extern "C" int smr_batch_autotune(smr_batch *b, const smr_fire_params *fire, int
     (void)hipMemcpyAsync(b->pos.ptr, kpos.ptr, apos_b, hipMemcpyDeviceToDevice, st);
     (void)hipMemcpyAsync(b->vel.ptr, kvel.ptr, apos_b, hipMemcpyDeviceToDevice, st);
     (void)hipMemcpyAsync(b->state.ptr, kst.ptr, st_b, hipMemcpyDeviceToDevice, st);
+    // the trials ran without cell keys and left the stencils of their own
+    // last cells in the stored-stencil buffer: every stored stencil is void
+    if (b->stencil_key.ptr)
+        (void)hipMemsetAsync(b->stencil_key.ptr, 0xff, sizeof(int) * (size_t)b->total_atoms, st);
     hipError_t e = hipStreamSynchronize(st);
     if (rc2 != SMR_OK) return rc2;
     if (e != hipSuccess) return hip_fail(e, "smr_batch_autotune: restore");

@@ -132,15 +132,22 @@ struct KahanF3 {
 
 // Bond between atom i (at pi) and neighbour at pj: writes hneigh (h, 1/l),
 // adds 2k(l-l0) h to f, returns k (l-l0)^2.   uff.rs:440-468 / UFF.cl:223-256
+// The stretch l - l0 is formed as (|d|^2 - l0^2) / (l + l0) with |d|^2 and
+// the subtraction in FP64 from the exact difference of the FP32 positions:
+// l - l0 taken from an FP32 length carries an error of ~1e-7 l (not of
+// l - l0), i.e. ~1e-5 eV/A of force on a stiff bond whatever the strain.
 __device__ __forceinline__ float bond_term(float3 pi, float3 pj, float k, float l0, float4 &hn, float3 &f) {
     float3 dp = pj - pi;
-    float l2 = dot3(dp, dp);
-    l2 = fmaxf(l2, 1e-16f);
+    const double dx = (double)pj.x - (double)pi.x, dy = (double)pj.y - (double)pi.y,
+                 dz = (double)pj.z - (double)pi.z;
+    const double l2d = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double l0d = (double)l0;
+    float l2 = fmaxf((float)l2d, 1e-16f);
     float inv_l = frsq(l2);
     float l = l2 * inv_l;
     float3 h = dp * inv_l;
     hn = make_float4(h.x, h.y, h.z, inv_l);
-    float dl = l - l0;
+    float dl = (float)fma(-l0d, l0d, l2d) * frcp(fmaxf(l + l0, 1e-8f));
     f = f + h * (2.0f * k * dl);
     return k * dl * dl;
 }

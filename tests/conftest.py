@@ -44,3 +44,13 @@ def rel_force_err(f, fref):
     d = np.linalg.norm(np.asarray(f) - np.asarray(fref), axis=1).max()
     s = np.linalg.norm(np.asarray(fref), axis=1).max()
     return d / s
+
+
+def scaled_force_err(f, fref, scale):
+    """max_i |dF_i| / scale_i, with scale_i the sum of the magnitudes of the
+    force contributions summed into atom i (oracle.force_scale): the relative
+    error of each atom's force sum.  Used along trajectories, where the net
+    forces fall towards Fmax < 1e-3 eV/A while the contributions stay of order
+    eV/A, so that max_i |F_i| stops measuring the arithmetic's precision."""
+    d = np.linalg.norm(np.asarray(f) - np.asarray(fref), axis=1)
+    return float((d / np.asarray(scale)).max())

@@ -405,3 +405,52 @@ def test_pentacene_fire_returns_to_plane(oracle, pentacene):
     r2 = oracle.relax_fire(s, fp, 5000, 1e-3, apos=x)
     zr2 = np.sqrt((r2["apos"][:, 2] ** 2).sum()) / math.sqrt(len(s.elems))
     assert zr2 < 0.05
+
+
+def test_system_from_terms_any_order(oracle, params):
+    """A system built from given term lists (Uff::from_topology) in shuffled
+    order has the parameters of the geometry-built one per term, and the
+    same forces and energies to FP64 rounding."""
+    from surfmol_amd import workloads as W
+    m = W.ptcda_like()
+    a = oracle.system_from_geometry(m.elems, m.apos, params, charges=m.charges, bonds=m.bonds)
+    rng = np.random.default_rng(3)
+    ang = a.angles[rng.permutation(len(a.angles))]
+    dih = a.dihedrals[rng.permutation(len(a.dihedrals))]
+    inv = a.inversions[rng.permutation(len(a.inversions))]
+    b = oracle.system_from_terms(a.types, m.apos, a.bonds, ang, dih, inv, params, charges=m.charges)
+    pa = {tuple(d): tuple(p) for d, p in zip(a.dihedrals, a.dih_params)}
+    assert all(pa[tuple(d)] == tuple(p) for d, p in zip(b.dihedrals, b.dih_params))
+    fa, ea = oracle.eval_forces(a)
+    fb, eb = oracle.eval_forces(b)
+    assert np.abs(fa - fb).max() < 1e-12 * np.abs(fa).max()
+    assert np.abs(ea - eb).max() < 1e-12 * np.abs(ea).max()
+
+
+def test_gridff_build_nodes_equals_full_build(oracle, params):
+    from surfmol_amd import workloads as W
+    from surface_setup import slab_reqs
+    el, pos, q = W.rocksalt_slab(6.0, 6.0)
+    reqs = slab_reqs(params, oracle, el, q)
+    ns, lvec = (8, 6, 5), [6.0, 0, 0, 0, 6.0, 0, 0, 0, 2.5]
+    E = oracle.gridff_build(pos, reqs, lvec, ns, (0.0, 0.0, 1.0), (1, 1, 0), 1.5, 1.0)
+    nodes = np.array([[0, 0, 0], [7, 5, 4], [3, 2, 1], [5, 0, 3]])
+    En = oracle.gridff_build_nodes(pos, reqs, lvec, ns, (0.0, 0.0, 1.0), (1, 1, 0), 1.5, 1.0, nodes)
+    assert np.array_equal(En, E[nodes[:, 0], nodes[:, 1], nodes[:, 2]])
+
+
+def test_force_scale(oracle, params):
+    """The force scale of the H2 known answer is |F| = 20 on each atom; in
+    general it bounds the net force (triangle inequality) and is additive
+    over the terms."""
+    s = _two_atom_system(oracle)
+    assert oracle.force_scale(s) == pytest.approx([20.0, 20.0], abs=1e-12)
+    from surfmol_amd import workloads as W
+    m = W.ptcda_like()
+    o = oracle.system_from_geometry(m.elems, m.apos, params, charges=m.charges, bonds=m.bonds)
+    f, _ = oracle.eval_forces(o)
+    sc = oracle.force_scale(o)
+    assert (sc >= np.linalg.norm(f, axis=1) * (1 - 1e-12)).all()
+    o2 = oracle.system_from_geometry(m.elems, m.apos, params, charges=m.charges, bonds=m.bonds, nb_mode=oracle.NB_OFF)
+    sc2 = oracle.force_scale(o2)
+    assert (sc > sc2).all()
