@@ -239,6 +239,19 @@ int smr_relax_batched(smr_batch *b, const smr_fire_params *fire, int max_steps, 
 int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fire, int max_steps, double fconv,
                     smr_relax_result *result);
 
+/* smr_relax_plain without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during the
+ * call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels, results and errors as smr_relax_plain; the
+ * kernels' scratch (hneigh, force pieces, per-atom energies, reduction output)
+ * is a fixed device array of 600 KiB per GPU, kept until the process ends,
+ * instead of buffers allocated on the batch.  A system with more than 1024
+ * atoms or more than 32768 force pieces (4 per dihedral and inversion, 3 per
+ * angle) is rejected with SMR_ERR_UNSUPPORTED.  Calls on one GPU share that
+ * array: calls that overlap on one GPU race. */
+int smr_relax_plain_noalloc(smr_batch *b, int isys, const smr_fire_params *fire, int max_steps, double fconv,
+                            smr_relax_result *result);
+
 /* Launch variant of the fused kernel for this batch: threads per workgroup
  * (64/128/256/512; 0 = automatic by system size; atoms per thread follow),
  * the stored per-atom surface stencils (1 on, 0 off, -1 automatic: with one

@@ -91,6 +91,7 @@ _SIGNATURES = [
     ("smr_batch_eval_forces", c_int, [c_void_p, P(c_double), P(c_double)]),
     ("smr_relax_batched", c_int, [c_void_p, P(SmrFireParams), c_int, c_double, P(SmrRelaxResult)]),
     ("smr_relax_plain", c_int, [c_void_p, c_int, P(SmrFireParams), c_int, c_double, P(SmrRelaxResult)]),
+    ("smr_relax_plain_noalloc", c_int, [c_void_p, c_int, P(SmrFireParams), c_int, c_double, P(SmrRelaxResult)]),
     ("smr_run_md", c_int, [c_void_p, c_int, c_double, c_double, c_double, c_double, P(c_int)]),
     ("smr_batch_last_launch", c_int, [c_void_p, P(c_int), P(c_float)]),
     ("smr_batch_set_launch", c_int, [c_void_p, c_int, c_int, c_int]),

@@ -87,6 +87,8 @@ extern "C" int smr_ctx_create(int device, smr_ctx **out) {
     c->arch = prop.gcnArchName;
     e = c->ss.create();
     if (e != hipSuccess) { delete c; return hip_fail(e, "smr_ctx_create: streams"); }
+    int rc = noalloc_setup(device);
+    if (rc) { c->ss.destroy(); delete c; return rc; }
     c->stream = c->ss.main;
     *out = c;
     return SMR_OK;
@@ -1293,29 +1295,56 @@ extern "C" int smr_run_md(smr_batch *b, int niter, double dt, double fconv, doub
     return SMR_OK;
 }
 
-extern "C" int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fire, int max_steps, double fconv,
-                               smr_relax_result *result) {
-    if (!b || !result) return fail(SMR_ERR_INVALID, "smr_relax_plain: NULL argument");
-    int rc = check_fire(fire, "smr_relax_plain");
+namespace smr {
+// The plain method on system isys.  Its kernels' scratch (hneigh, force
+// pieces, per-atom energies, the reduction output) comes from `ext` when
+// given (no allocation on this path), else from the batch's own buffers,
+// allocated on the first call.
+int relax_plain_run(smr_batch *b, int isys, const smr_fire_params *fire, int max_steps, double fconv,
+                    smr_relax_result *result, const PlainScratch *ext, const char *who) {
+    if (!b || !result) return fail(SMR_ERR_INVALID, std::string(who) + ": NULL argument");
+    int rc = check_fire(fire, who);
     if (rc) return rc;
     if (isys < 0 || isys >= b->nsys)
-        return fail(SMR_ERR_INVALID, "smr_relax_plain: isys " + std::to_string(isys) + " not in [0," + std::to_string(b->nsys) + ")");
-    if (max_steps <= 0 || !(fconv > 0.0)) return fail(SMR_ERR_INVALID, "smr_relax_plain: need max_steps > 0, fconv > 0");
-    if (b->layer) return fail(SMR_ERR_UNSUPPORTED, "smr_relax_plain: not available for a coupled layer");
+        return fail(SMR_ERR_INVALID, std::string(who) + ": isys " + std::to_string(isys) + " not in [0," +
+                                         std::to_string(b->nsys) + ")");
+    if (max_steps <= 0 || !(fconv > 0.0)) return fail(SMR_ERR_INVALID, std::string(who) + ": need max_steps > 0, fconv > 0");
+    if (b->layer) return fail(SMR_ERR_UNSUPPORTED, std::string(who) + ": not available for a coupled layer");
     smr_ctx *c = b->ctx;
     HIPCHK(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     const SysDesc &sd = b->hp.sys[isys];
     const TopoDesc &td = b->hp.topo[sd.topo];
     const int N = td.natoms;
-    if (b->plain_hneigh.bytes < sizeof(float4) * 4 * (size_t)b->max_atoms) {
-        hipError_t e = b->plain_hneigh.alloc(sizeof(float4) * 4 * (size_t)b->max_atoms);
-        if (e == hipSuccess) e = b->plain_fint.alloc(sizeof(float4) * (size_t)std::max(1, b->nf_max));
-        if (e == hipSuccess) e = b->plain_e.alloc((sizeof(float) * 2 + sizeof(double)) * (size_t)b->max_atoms);
-        if (e == hipSuccess) e = b->plain_cvf.alloc(sizeof(double) * 16);
-        if (e != hipSuccess) return hip_fail(e, "smr_relax_plain: scratch");
-    }
     PlainArgs P;
+    if (ext) {
+        if (N > ext->max_atoms || td.nf > ext->max_pieces)
+            return fail(SMR_ERR_UNSUPPORTED, std::string(who) + ": system " + std::to_string(isys) + " has " +
+                                                 std::to_string(N) + " atoms and " + std::to_string(td.nf) +
+                                                 " force pieces; the fixed scratch holds at most " +
+                                                 std::to_string(ext->max_atoms) + " atoms and " +
+                                                 std::to_string(ext->max_pieces) + " pieces");
+        P.hneigh = ext->hneigh;
+        P.fint = ext->fint;
+        P.eatom = ext->eatom;
+        P.esurf = ext->esurf;
+        P.epair = ext->epair;
+        P.cvf = ext->cvf;
+    } else {
+        if (b->plain_hneigh.bytes < sizeof(float4) * 4 * (size_t)b->max_atoms) {
+            hipError_t e = b->plain_hneigh.alloc(sizeof(float4) * 4 * (size_t)b->max_atoms);
+            if (e == hipSuccess) e = b->plain_fint.alloc(sizeof(float4) * (size_t)std::max(1, b->nf_max));
+            if (e == hipSuccess) e = b->plain_e.alloc((sizeof(float) * 2 + sizeof(double)) * (size_t)b->max_atoms);
+            if (e == hipSuccess) e = b->plain_cvf.alloc(sizeof(double) * 16);
+            if (e != hipSuccess) return hip_fail(e, "smr_relax_plain: scratch");
+        }
+        P.hneigh = (float4 *)b->plain_hneigh.ptr;
+        P.fint = (float4 *)b->plain_fint.ptr;
+        P.eatom = (float *)b->plain_e.ptr;
+        P.esurf = P.eatom + b->max_atoms;
+        P.epair = (double *)(P.esurf + b->max_atoms);
+        P.cvf = (double *)b->plain_cvf.ptr;
+    }
     P.tab = b->tab;
     P.topo = sd.topo;
     P.iux = sd.iux;
@@ -1323,12 +1352,6 @@ extern "C" int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fi
     P.pos = (float4 *)b->pos.ptr + sd.atom_base;
     P.vel = (float4 *)b->vel.ptr + sd.atom_base;
     P.force = (float4 *)b->force.ptr + sd.atom_base;
-    P.hneigh = (float4 *)b->plain_hneigh.ptr;
-    P.fint = (float4 *)b->plain_fint.ptr;
-    P.eatom = (float *)b->plain_e.ptr;
-    P.esurf = P.eatom + b->max_atoms;
-    P.epair = (double *)(P.esurf + b->max_atoms);
-    P.cvf = (double *)b->plain_cvf.ptr;
     HIPCHK(hipMemsetAsync(P.vel, 0, sizeof(float4) * N, st));
     double dt = fire->dt0, alpha = fire->alpha0;
     int npos = 0, steps = max_steps, status = SMR_SYS_MAXSTEPS;
@@ -1370,7 +1393,8 @@ extern "C" int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fi
             break;
         }
     }
-    // energies of the final positions, FP64 (as for the batched path)
+    // energies of the final positions, FP64 (as for the batched path); the
+    // system index travels in the reduction scratch
     HIPCHK(hipMemcpyAsync(P.cvf + 12, &isys, sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(launch_energy64(b->tab, (const float4 *)b->pos.ptr, (SysState *)b->state.ptr, (const int *)(P.cvf + 12), 1, st));
     SysState hs;
@@ -1383,6 +1407,12 @@ extern "C" int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fi
     for (int q = 0; q < 6; q++) { result->terms[q] = hs.E[q]; E += hs.E[q]; }
     result->energy = E;
     return SMR_OK;
+}
+}  // namespace smr
+
+extern "C" int smr_relax_plain(smr_batch *b, int isys, const smr_fire_params *fire, int max_steps, double fconv,
+                               smr_relax_result *result) {
+    return relax_plain_run(b, isys, fire, max_steps, fconv, result, nullptr, "smr_relax_plain");
 }
 
 extern "C" int smr_batch_last_launch(const smr_batch *b, int *shape4, float *kernel_ms) {

@@ -30,6 +30,16 @@ struct DevBuf {
     hipError_t upload(const void *src, size_t nbytes, hipStream_t st);
 };
 
+// scratch of the plain method's kernels for one system: hneigh [4N],
+// force pieces [nf], per-atom bond/surface/pair energies [N], reduction
+// output [16]
+struct PlainScratch {
+    float4 *hneigh, *fint;
+    float *eatom, *esurf;
+    double *epair, *cvf;
+    int max_atoms, max_pieces;
+};
+
 struct HostPools {
     std::vector<TopoDesc> topo;
     std::vector<SysDesc> sys;
@@ -185,3 +195,13 @@ struct smr_batch {
     std::vector<int> work_host;      // the work list as uploaded (class order, atom count inside a class)
     float last_kernel_ms = 0.f;
 };
+
+namespace smr {
+// the plain method (capi.cpp): scratch from `ext`, or the batch's own
+// buffers (allocated on first use) when ext == nullptr
+int relax_plain_run(smr_batch *b, int isys, const smr_fire_params *fire, int max_steps, double fconv,
+                    smr_relax_result *result, const PlainScratch *ext, const char *who);
+// the fixed scratch of smr_relax_plain_noalloc on `device` (current):
+// resolves its device address once per device (plain_noalloc.hip)
+int noalloc_setup(int device);
+}  // namespace smr
