@@ -113,64 +113,71 @@ def workload_uneven(args, S, W):
     return mols, np.arange(nsys, dtype=np.int32), X, 48.0, desc
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["uniform", "uneven", "layer", "serve"], default="uniform")
-    ap.add_argument("--rates", type=str, default="200,1000,5000", help="serve: arrival rates [req/s]")
-    ap.add_argument("--requests", type=int, default=1000, help="serve: requests per rate")
-    ap.add_argument("--chunk-steps", type=int, default=64, help="serve: FIRE steps per persistent launch (<= 64 atoms)")
-    ap.add_argument("--nsys", type=int, default=None)
-    ap.add_argument("--max-steps", type=int, default=None, help="FIRE step cap (2000; 20000 for the layer)")
-    ap.add_argument("--layer-spacing", type=float, default=13.0,
-                    help="layer: square lattice spacing [A]; at 11 A the molecule has no flat arrangement with "
-                         "contacts above 2.05 A and the layer does not relax (FP64 oracle and device agree)")
-    ap.add_argument("--autotune", action="store_true", help="time every launch shape per size class first "
-                    "(always on for the uneven workload)")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="issue the launches directly instead of replaying captured HIP graphs (same results)")
-    ap.add_argument("--plain-sample", type=int, default=4)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--copy-probe", type=int, default=0,
-                    help="run only a plain device copy of this many bytes (HBM counter calibration) and exit")
-    args = ap.parse_args()
-    args.nsys_set = args.nsys is not None
-    if args.max_steps is None:
-        args.max_steps = 20000 if args.workload == "layer" else 2000
-    if args.nsys is None:
-        args.nsys = 4096
-    if args.copy_probe:
-        import torch
-        n = args.copy_probe // 4
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": 4 * n}))
-        return
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); the systems are independent, "
-              f"so N GPUs would each relax their own share with no exchange.", file=sys.stderr)
-        sys.exit(2)
+def host_info():
+    """Model name and usable core count of this host (the CPU baseline's)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    return {"cpu_model": model or "unknown", "cores_usable": cores, "logical_cpus": os.cpu_count()}
 
-    import surfmol_amd as S
-    from surfmol_amd import workloads as W
 
+def cpu_relax_rate(O, osys_list, X_of, nsys, ofire, max_steps, fconv, seconds, threads):
+    """Converged systems/s of the FP64 C restatement (oracle/oracle.c) relaxing
+    systems 0, 1, 2, ... of the batch until `seconds` have passed, on
+    `threads` host threads (the C code runs outside the GIL; each thread owns
+    one oracle system whose positions are replaced per system)."""
+    import threading
+    lock = threading.Lock()
+    state = {"next": 0, "done": 0, "conv": 0}
+    t_end = time.perf_counter() + seconds
+
+    def worker(osys):
+        while True:
+            with lock:
+                if state["next"] >= nsys or time.perf_counter() > t_end:
+                    return
+                s = state["next"]
+                state["next"] += 1
+            r = O.relax_fire(osys, ofire, max_steps, fconv, apos=X_of(s))
+            with lock:
+                state["done"] += 1
+                state["conv"] += int(r["converged"])
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker, args=(osys_list[k],)) for k in range(threads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    T = time.perf_counter() - t0
+    return state["conv"] / T, state["done"], T
+
+
+def run_workload(args, S, W, workload, steps, warmup, cpu_seconds, want_plain, want_cpu, nsys=None):
+    """One batched-relaxation workload: `steps` timed relaxations of every
+    system (positions restored on the device before each) and the result
+    dictionary of the bench line."""
     alpha, rdamp = 1.5, 1.0
-    if args.workload == "serve":
-        serve(args, S, W, alpha, rdamp)
-        return
+    layer = workload == "layer"
+    max_steps = args.max_steps if args.max_steps is not None else (20000 if layer else 2000)
+    wargs = argparse.Namespace(**vars(args))
+    wargs.max_steps = max_steps
+    wargs.nsys = nsys if nsys is not None else (args.nsys if args.nsys is not None else 4096)
+    wargs.nsys_set = nsys is not None or args.nsys is not None
     t0 = time.time()
-    wl = {"uniform": workload_uniform, "uneven": workload_uneven, "layer": workload_layer}[args.workload]
-    mols, sys_topo, Xs, L, desc = wl(args, S, W)
-    layer = args.workload == "layer"
+    wl = {"uniform": workload_uniform, "uneven": workload_uneven, "layer": workload_layer}[workload]
+    mols, sys_topo, Xs, L, desc = wl(wargs, S, W)
     rc = 12.0 if layer else 0.0
-    t_gen = time.time() - t0
     nsys = len(sys_topo)
     ctx = S.Context(0)
     grid, spec, t_grid = grid_for(ctx, S, W, L, alpha=alpha, rdamp=rdamp, npbc=10 if layer else 2)
@@ -182,28 +189,29 @@ def main():
     if args.no_graph:
         batch.set_graph(False)
     tuned = None
-    if args.autotune or args.workload == "uneven":
+    if args.autotune or workload == "uneven":
         t0 = time.time()
         tuned = batch.autotune(fire, 50)
         t_tune = time.time() - t0
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         batch.reset()
-        batch.relax_batched(fire, args.max_steps, fconv)
+        batch.relax_batched(fire, max_steps, fconv)
     times, kms, results = [], [], None
-    for _ in range(args.steps):
+    for _ in range(steps):
         batch.reset()
         t0 = time.perf_counter()
-        results = batch.relax_batched(fire, args.max_steps, fconv)   # synchronises the stream
+        results = batch.relax_batched(fire, max_steps, fconv)   # synchronises the stream
         times.append(time.perf_counter() - t0)
         kms.append(batch.last_launch()["kernel_ms"])
     T = sum(times)
+    X_dev = batch.positions()
     # the same relaxation with the boundary's host buffers: FP64 positions in
     # (smr_batch_set_positions, over PCIe), relaxed, positions out
     batch.reset()
     t0 = time.perf_counter()
     batch.set_positions(X)
-    r_io = batch.relax_batched(fire, args.max_steps, fconv)
+    r_io = batch.relax_batched(fire, max_steps, fconv)
     _ = batch.positions()
     t_io = time.perf_counter() - t0
     nconv_io = sum(1 for r in r_io if r["status"] == S.SMR_SYS_CONVERGED)
@@ -211,7 +219,7 @@ def main():
     steps_arr = np.array([r["steps"] for r in results])
     if layer:
         nconv = 1 if nconv == nsys else 0     # the layer is one relaxed system
-    value = nconv * args.steps / T
+    value = nconv * steps / T
 
     # algorithmic FLOP and grid-lookup bytes of one relaxation (sum over systems of per-step work x steps taken)
     counts = [hm.counts for hm in hms]
@@ -231,14 +239,14 @@ def main():
         flop_launch += 41.0 * n_inter * float(steps_arr[0])
     kavg_s = (sum(kms) / len(kms)) / 1e3
     achieved_tf = flop_launch / kavg_s / 1e12
-    traffic, traffic_src = measured_traffic(args)
+    traffic, traffic_src = measured_traffic(wargs) if workload == "uniform" else (None, None)
     natoms_sys = np.array(natoms_sys)
 
     cfg = {
         "workload": desc,
         "nsys": nsys,
         "relaxed_unit": "layer (all systems, one FIRE state)" if layer else "system",
-        "natoms_per_system": int(natoms_sys[0]) if args.workload == "uniform" else
+        "natoms_per_system": int(natoms_sys[0]) if workload == "uniform" else
         {"min": int(natoms_sys.min()), "mean": round(float(natoms_sys.mean()), 1), "max": int(natoms_sys.max())},
         "converged": nconv,
         "fire_steps_mean": round(float(steps_arr.mean()), 1),
@@ -254,9 +262,9 @@ def main():
         "value": round(value, 4 if layer else 2),
         "unit": "layers/s" if layer else "systems/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * T / args.steps, 3),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(1e3 * T / steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -264,8 +272,8 @@ def main():
         "data": "synthetic",
         "config": cfg,
         "roofline": {
-            "bound": "mfma",
-            "note": "compute roof = FP32 vector peak; the kernel runs on the VALU (no dense contraction on the path)",
+            "bound": "valu",
+            "note": "compute roof = FP32 vector (VALU) peak: no step is a dense contraction, so MFMA is unused",
             "achieved": round(achieved_tf, 3),
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -288,14 +296,14 @@ def main():
         cfg["fmax_final"] = round(float(results[0]["fmax"]), 6)
         # the 11 A square lattice keeps rearranging for more than 20k FIRE steps,
         # so the step rate of the coupled layer is reported beside `value`
-        out["layer_fire_steps_per_s"] = round(float(steps_arr[0]) * args.steps / T, 1)
-        out["atom_steps_per_s"] = round(float(steps_arr[0]) * args.steps / T * len(X), 1)
-    if not args.no_plain and not layer:
+        out["layer_fire_steps_per_s"] = round(float(steps_arr[0]) * steps / T, 1)
+        out["atom_steps_per_s"] = round(float(steps_arr[0]) * steps / T * len(X), 1)
+    if want_plain and not layer:
         pts = []
         for s in range(min(args.plain_sample, nsys)):
             batch.reset()
             t0 = time.perf_counter()
-            r = batch.relax_plain(s, fire, args.max_steps, fconv)
+            r = batch.relax_plain(s, fire, max_steps, fconv)
             dt = time.perf_counter() - t0
             pts.append((dt, r["status"] == S.SMR_SYS_CONVERGED, r["steps"]))
         tp = sum(p[0] for p in pts)
@@ -305,93 +313,239 @@ def main():
                             round(1e3 * tp / sum(p[2] for p in pts), 4)}
         out["speedup_vs_plain"] = round(value / plain_rate, 1) if plain_rate > 0 else None
 
-    if not args.no_cpu and layer:
+    if want_cpu:
+        # the oracle here is the checker and the timed CPU baseline only
         from oracle import oracle as O
         O.build()
         P = O.Params(S.DATA_DIR)
         C = grid.coefficients().astype(np.float64)
         ofire = O.FireParams(dt0=fire.dt0, dt_max=fire.dt_max)
-        osys = O.layer_from_molecules([mols[0]] * nsys, Xs, P, rc, rdamp=rdamp, layer_cells=True, rswitch_inter=2.0,
-                                      grid=C,
-                                      grid_ns=(spec.nx, spec.ny, spec.nz), grid_p0=spec.p0, grid_step=spec.step,
-                                      alpha_morse=alpha)
-        osys.cstruct()
-        nst, t_cpu = 0, 0.0
-        x, v = None, None
-        while t_cpu < args.cpu_seconds:
-            t0 = time.perf_counter()
-            r = O.relax_fire(osys, ofire, 10, 1e-3, apos=x, vapos=v)
-            t_cpu += time.perf_counter() - t0
-            x, v = r["apos"], r["vapos"]
-            nst += 10
-        # the FIRE state restarts every 10 steps in this timing loop; the cost per step is what is measured
-        t_step = t_cpu / nst
-        cpu_rate = 1.0 / (t_step * float(steps_arr[0]))
-        out["cpu_baseline"] = {"value": round(cpu_rate, 6), "unit": "layers/s", "cores": 1, "kind": "port",
-                               "sample": f"{nst} FIRE steps of the same 19,456-atom layer on the FP64 C restatement "
-                                         f"(oracle/oracle.c, cell-grid pair search), single thread, {t_cpu:.1f} s; "
-                                         f"{1e3 * t_step:.1f} ms/step x the {int(steps_arr[0])} steps the device took",
-                               "fire_steps_per_s": round(1.0 / t_step, 2)}
-        out["speedup_vs_cpu"] = round(out["layer_fire_steps_per_s"] * t_step, 1)
-    if not args.no_cpu and not layer:
-        from oracle import oracle as O
-        O.build()
-        P = O.Params(S.DATA_DIR)
-        C = grid.coefficients().astype(np.float64)
-        ofire = O.FireParams(dt0=fire.dt0, dt_max=fire.dt_max)
-        done, conv, t_cpu = 0, 0, 0.0
-        # a sample that follows the workload's size mix: systems in batch order
-        while t_cpu < args.cpu_seconds and done < nsys:
-            m = mols[sys_topo[done]]
-            osys = O.system_from_geometry(m.elems, Xs[done], P, charges=m.charges, bonds=m.bonds, nb_mode=O.NB_GPU,
-                                          rdamp=rdamp, grid=C, grid_ns=(spec.nx, spec.ny, spec.nz), grid_p0=spec.p0,
-                                          grid_step=spec.step, alpha_morse=alpha)
-            osys.cstruct()  # set-up outside the timed region
-            t0 = time.perf_counter()
-            r = O.relax_fire(osys, ofire, args.max_steps, fconv)
-            t_cpu += time.perf_counter() - t0
-            conv += int(r["converged"])
-            done += 1
-        cpu_rate = conv / t_cpu
-        out["cpu_baseline"] = {"value": round(cpu_rate, 3), "unit": "systems/s", "cores": 1, "kind": "port",
-                               "sample": f"first {done} systems of the same batch, FP64 C restatement of the "
-                                         f"reference CPU path (oracle/oracle.c), single thread, {t_cpu:.1f} s"}
-        out["speedup_vs_cpu"] = round(value / cpu_rate, 1) if cpu_rate > 0 else None
+        host = host_info()
+        gkw = dict(rdamp=rdamp, grid=C, grid_ns=(spec.nx, spec.ny, spec.nz), grid_p0=spec.p0, grid_step=spec.step,
+                   alpha_morse=alpha)
+        if layer:
+            osys = O.layer_from_molecules([mols[0]] * nsys, Xs, P, rc, layer_cells=True, rswitch_inter=2.0, **gkw)
+            osys.cstruct()
+            nst, t_cpu = 0, 0.0
+            x, v = None, None
+            while t_cpu < cpu_seconds:
+                t0 = time.perf_counter()
+                r = O.relax_fire(osys, ofire, 10, 1e-3, apos=x, vapos=v)
+                t_cpu += time.perf_counter() - t0
+                x, v = r["apos"], r["vapos"]
+                nst += 10
+            # the FIRE state restarts every 10 steps in this timing loop; the cost per step is what is measured
+            t_step = t_cpu / nst
+            cpu_rate = 1.0 / (t_step * float(steps_arr[0]))
+            out["cpu_baseline"] = {"value": round(cpu_rate, 6), "unit": "layers/s", "cores": 1, "kind": "port",
+                                   "sample": f"{nst} FIRE steps of the same 19,456-atom layer on the FP64 C "
+                                             f"restatement (oracle/oracle.c, cell-grid pair search), single thread, "
+                                             f"{t_cpu:.1f} s; {1e3 * t_step:.1f} ms/step x the {int(steps_arr[0])} "
+                                             f"steps the device took", "fire_steps_per_s": round(1.0 / t_step, 2),
+                                   **host}
+            out["speedup_vs_cpu"] = round(out["layer_fire_steps_per_s"] * t_step, 1)
+        else:
+            def make_osys(s):
+                m = mols[sys_topo[s]]
+                o = O.system_from_geometry(m.elems, Xs[s], P, charges=m.charges, bonds=m.bonds, nb_mode=O.NB_GPU, **gkw)
+                o.cstruct()
+                return o
+            uniform = all(t == sys_topo[0] for t in sys_topo)
+            ncores = host["cores_usable"]
+            if uniform:
+                # one oracle system per thread (replicas: only the positions change)
+                pool = [make_osys(0) for _ in range(ncores)]
+                r1, d1, T1 = cpu_relax_rate(O, pool[:1], lambda s: Xs[s], nsys, ofire, max_steps, fconv, cpu_seconds, 1)
+                rN, dN, TN = cpu_relax_rate(O, pool, lambda s: Xs[s], nsys, ofire, max_steps, fconv, cpu_seconds,
+                                            ncores)
+                what = ("FP64 C restatement of the reference CPU path (oracle/oracle.c: UFF, O(N^2) LJ+Coulomb, "
+                        "tricubic GridFF, the reference FIRE loop), whole systems per thread")
+                out["cpu_baseline"] = {"value": round(rN, 3), "unit": "systems/s", "cores": ncores, "kind": "port",
+                                       "sample": f"systems 0..{dN - 1} of the same batch on {ncores} threads, "
+                                                 f"{TN:.1f} s; {what}", **host}
+                out["cpu_baseline_1thread"] = {"value": round(r1, 3), "unit": "systems/s", "cores": 1,
+                                               "sample": f"systems 0..{d1 - 1}, one thread, {T1:.1f} s (the "
+                                                         f"reference CPU path is single-threaded)"}
+                out["speedup_vs_cpu"] = round(value / rN, 1) if rN > 0 else None
+                out["speedup_vs_cpu_1thread"] = round(value / r1, 1) if r1 > 0 else None
+            else:
+                done, conv, t_cpu = 0, 0, 0.0
+                # a sample that follows the workload's size mix: systems in batch order
+                while t_cpu < cpu_seconds and done < nsys:
+                    osys = make_osys(done)   # set-up outside the timed region
+                    t0 = time.perf_counter()
+                    r = O.relax_fire(osys, ofire, max_steps, fconv)
+                    t_cpu += time.perf_counter() - t0
+                    conv += int(r["converged"])
+                    done += 1
+                cpu_rate = conv / t_cpu
+                out["cpu_baseline"] = {"value": round(cpu_rate, 3), "unit": "systems/s", "cores": 1, "kind": "port",
+                                       "sample": f"first {done} systems of the same batch, FP64 C restatement of the "
+                                                 f"reference CPU path (oracle/oracle.c), single thread, {t_cpu:.1f} s",
+                                       **host}
+                out["speedup_vs_cpu"] = round(value / cpu_rate, 1) if cpu_rate > 0 else None
+            # error against FP64 on a sample of the timed relaxation's systems
+            out["fp64_error"] = fp64_error(O, S, batch, results, X_dev, make_osys, mols, sys_topo, ofire, fconv,
+                                           min(8, nsys))
+    return out
 
+
+def fp64_error(O, S, batch, results, X_dev, make_osys, mols, sys_topo, ofire, fconv, nsample):
+    """Errors of the device against the FP64 oracle on `nsample` systems:
+    forces at the start (max_i|dF_i| / max_i|F_i|) and at the relaxed
+    positions (per atom, against the sum of the magnitudes of the atom's force
+    contributions); the reported energy against the FP64 energy of the
+    returned positions; and how far FP64 FIRE restarted from the device's
+    minimum moves the energy (eV/atom)."""
+    batch.reset()
+    F0, _ = batch.eval_forces()
+    X0 = batch.positions()
+    off = batch.offsets
+    e_start, e_final, e_energy, e_basin = 0.0, 0.0, 0.0, 0.0
+    for s in range(nsample):
+        sl = slice(off[s], off[s + 1])
+        o = make_osys(s)
+        n = off[s + 1] - off[s]
+        fo, _ = O.eval_forces(o, X0[sl])
+        e_start = max(e_start, float(np.linalg.norm(F0[sl] - fo, axis=1).max() / np.linalg.norm(fo, axis=1).max()))
+        ff, eo = O.eval_forces(o, X_dev[sl])
+        e_energy = max(e_energy, abs(float(eo.sum()) - float(results[s]["energy"])) / n)
+        rf = O.relax_fire(o, ofire, 3000, fconv, apos=X_dev[sl])
+        e_basin = max(e_basin, abs(rf["E"] - float(results[s]["energy"])) / n)
+    # forces at the relaxed positions
+    batch.set_positions(X_dev)
+    Ff, _ = batch.eval_forces()
+    for s in range(nsample):
+        sl = slice(off[s], off[s + 1])
+        o = make_osys(s)
+        ff, _ = O.eval_forces(o, X_dev[sl])
+        sc = O.force_scale(o, X_dev[sl])
+        e_final = max(e_final, float((np.linalg.norm(Ff[sl] - ff, axis=1) / sc).max()))
+    batch.reset()
+    return {"systems": nsample, "force_rel_start": float("%.3g" % e_start),
+            "force_rel_to_contributions_relaxed": float("%.3g" % e_final),
+            "energy_eV_per_atom": float("%.3g" % e_energy),
+            "fp64_restart_dE_eV_per_atom": float("%.3g" % e_basin),
+            "note": "energies are FP64 evaluations of the device's FP32 positions; the restart term is the energy "
+                    "change when FP64 FIRE continues from the device's relaxed state"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["uniform", "uneven", "layer", "serve"], default="uniform")
+    ap.add_argument("--rates", type=str, default="200,1000,5000", help="serve: arrival rates [req/s]")
+    ap.add_argument("--requests", type=int, default=1000, help="serve: requests per rate")
+    ap.add_argument("--chunk-steps", type=int, default=64, help="serve: FIRE steps per persistent launch (<= 64 atoms)")
+    ap.add_argument("--nsys", type=int, default=None)
+    ap.add_argument("--max-steps", type=int, default=None, help="FIRE step cap (2000; 20000 for the layer)")
+    ap.add_argument("--layer-spacing", type=float, default=13.0,
+                    help="layer: square lattice spacing [A]; at 11 A the molecule has no flat arrangement with "
+                         "contacts above 2.05 A and the layer does not relax (FP64 oracle and device agree)")
+    ap.add_argument("--autotune", action="store_true", help="time every launch shape per size class first "
+                    "(always on for the uneven workload)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="issue the launches directly instead of replaying captured HIP graphs (same results)")
+    ap.add_argument("--plain-sample", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="uniform workload: skip the short uneven, layer and serving measurements added to the line")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="run only a plain device copy of this many bytes (HBM counter calibration) and exit")
+    args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); the systems are independent, "
+              f"so N GPUs would each relax their own share with no exchange.", file=sys.stderr)
+        sys.exit(2)
+
+    import surfmol_amd as S
+    from surfmol_amd import workloads as W
+
+    if args.workload == "serve":
+        print(json.dumps(serve(args, S, W, 1.5, 1.0, [float(r) for r in args.rates.split(",") if r],
+                               args.requests)))
+        return
+    out = run_workload(args, S, W, args.workload, args.steps, args.warmup, args.cpu_seconds, not args.no_plain,
+                       not args.no_cpu)
+    if args.workload == "uniform" and not args.no_extras:
+        # the other configurations, each measured once in the same run (driver-timed with the headline)
+        extras = {}
+        t0 = time.time()
+        un = run_workload(args, S, W, "uneven", 1, 1, 0.0, False, False, nsys=2048)
+        extras["uneven"] = {"value": un["value"], "unit": un["unit"], "ms_per_relaxation": un["ms_per_step"],
+                            "converged": un["config"]["converged"], "nsys": un["config"]["nsys"],
+                            "roofline_frac": un["roofline"]["frac"], "kernel_ms": un["roofline"]["kernel_ms_avg"],
+                            "workload": un["config"]["workload"], "wall_s": round(time.time() - t0, 1)}
+        print(json.dumps({"progress": "uneven", **extras["uneven"]}), file=sys.stderr, flush=True)
+        t0 = time.time()
+        largs = argparse.Namespace(**vars(args))
+        largs.max_steps = None
+        la = run_workload(largs, S, W, "layer", 1, 0, 0.0, False, False)
+        extras["layer"] = {"value": la["value"], "unit": la["unit"], "ms_per_relaxation": la["ms_per_step"],
+                           "fire_steps": la["config"]["fire_steps_max"],
+                           "layer_fire_steps_per_s": la["layer_fire_steps_per_s"],
+                           "roofline_frac": la["roofline"]["frac"], "workload": la["config"]["workload"],
+                           "wall_s": round(time.time() - t0, 1)}
+        print(json.dumps({"progress": "layer", **extras["layer"]}), file=sys.stderr, flush=True)
+        t0 = time.time()
+        sargs = argparse.Namespace(**vars(args))
+        sargs.max_steps = None
+        sv = serve(sargs, S, W, 1.5, 1.0, [1000.0], 500)
+        r = sv["per_rate"][0]
+        extras["serve"] = {"rate_req_s": r["rate_req_s"], "requests": r["requests"], "p50_ms": r["p50_ms"],
+                           "p99_ms": r["p99_ms"], "served_per_s": r["served_per_s"], "converged": r["converged"],
+                           "served_vs_plain": r["served_vs_plain"], "workload": sv["config"]["workload"],
+                           "wall_s": round(time.time() - t0, 1)}
+        out["other_configs"] = extras
     print(json.dumps(out))
 
 
-def serve(args, S, W, alpha, rdamp):
-    """Serving loop under Poisson arrivals; one JSON line with p50/p99 per rate."""
-    rates = [float(r) for r in args.rates.split(",") if r]
+def serve(args, S, W, alpha, rdamp, rates, nreq):
+    """Serving loop under Poisson arrivals: the result dictionary (one JSON
+    line) with p50/p99 latency and served throughput per rate."""
+    max_steps = args.max_steps if args.max_steps is not None else 2000
     ctx = S.Context(0)
     grid, spec, t_grid = grid_for(ctx, S, W, 48.0, alpha=alpha, rdamp=rdamp)
     fire = S.FireParams(dt0=W.fs_to_internal(0.1), dt_max=W.fs_to_internal(1.0))
     per_rate = []
     for k, rate in enumerate(rates):
         seed = W.SEED_BASE + 4 + 16 * k
-        sizes = W.lognormal_sizes(args.requests, median=60.0, sigma=0.5, lo=20, hi=600, seed=seed)
+        sizes = W.lognormal_sizes(nreq, median=60.0, sigma=0.5, lo=20, hi=600, seed=seed)
         rng = np.random.Generator(np.random.PCG64(seed + 1))
         mols = [W.random_polyhex(rng, int(n)) for n in sizes]
-        ang, xy, h = W.uniform_batch_poses(args.requests, L=48.0, hmin=3.0, hmax=5.0, seed=seed + 2)
+        ang, xy, h = W.uniform_batch_poses(nreq, L=48.0, hmin=3.0, hmax=5.0, seed=seed + 2)
         X = [W.place_on_surface(m, ang[s], xy[s], h[s]) for s, m in enumerate(mols)]
         hm = [S.Molecule.build(m.elems, X[s], m.charges, bonds=m.bonds) for s, m in enumerate(mols)]
-        arrivals = W.poisson_arrivals(rate, args.requests, seed + 3)
+        arrivals = W.poisson_arrivals(rate, nreq, seed + 3)
         # the plain method on the same requests: one request at a time, one
         # launch sequence + host FIRE decision per step; its service times on
         # a sample feed a FIFO single-server queue over the same arrivals
         plain_t = []
-        for s in range(min(args.plain_sample, args.requests)):
+        for s in range(min(args.plain_sample, nreq)):
             pb = S.Batch(ctx, [hm[s]], [0], X[s], grid=grid, rdamp=rdamp)
             tq = time.perf_counter()
-            pb.relax_plain(0, fire, args.max_steps, 1e-3)
+            pb.relax_plain(0, fire, max_steps, 1e-3)
             plain_t.append(time.perf_counter() - tq)
             pb.close()
         srv = S.Server(ctx, grid, fire, max_resident=4096, max_atoms=600, hbm_fraction=0.25,
-                       chunk_steps=args.chunk_steps, max_steps=args.max_steps, fconv=1e-3, rdamp=rdamp)
+                       chunk_steps=args.chunk_steps, max_steps=max_steps, fconv=1e-3, rdamp=rdamp)
         ids = []
         t0 = time.perf_counter()
-        for s in range(args.requests):
+        for s in range(nreq):
             wait = arrivals[s] - (time.perf_counter() - t0)
             if wait > 0:
                 time.sleep(wait)
@@ -405,19 +559,19 @@ def serve(args, S, W, alpha, rdamp):
         span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
         steps = np.array([r["steps"] for r in res])
         free, plat = 0.0, []
-        for s in range(args.requests):
+        for s in range(nreq):
             start = max(free, arrivals[s])
             free = start + plain_t[s % len(plain_t)]
             plat.append(free - arrivals[s])
         plat = np.array(plat)
         plain = {"p50_ms": round(1e3 * float(np.percentile(plat, 50)), 1),
                  "p99_ms": round(1e3 * float(np.percentile(plat, 99)), 1),
-                 "served_per_s": round(args.requests / (free - arrivals[0]), 2),
+                 "served_per_s": round(nreq / (free - arrivals[0]), 2),
                  "service_ms_mean": round(1e3 * float(np.mean(plain_t)), 2),
                  "model": f"FIFO single-server queue over the request arrivals with the plain method's service "
                           f"times measured on the first {len(plain_t)} requests"}
         per_rate.append({
-            "rate_req_s": rate, "requests": args.requests, "converged": conv,
+            "rate_req_s": rate, "requests": nreq, "converged": conv,
             "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
             "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
             "mean_ms": round(1e3 * float(lat.mean()), 3),
@@ -440,11 +594,11 @@ def serve(args, S, W, alpha, rdamp):
         "config": {"workload": "serving: Poisson arrivals, request sizes lognormal (median 60 atoms, ln-sigma 0.5, "
                                "clipped [20,600]) random fused-ring molecules on a 48x48x20 A rock-salt GridFF at "
                                "0.1 A; FIRE dt0 0.1 fs, dt_max 1 fs, Fmax < 1e-3 eV/A, <= %d steps; %d FIRE steps "
-                               "per chunk" % (args.max_steps, args.chunk_steps),
+                               "per chunk" % (max_steps, args.chunk_steps),
                    "grid_build_fit_s": round(t_grid, 3)},
         "per_rate": per_rate,
     }
-    print(json.dumps(out))
+    return out
 
 
 if __name__ == "__main__":
