@@ -78,6 +78,36 @@ int smh_molecule_write_json(const smh_molecule *m, const char *path);
 /* Positions the molecule was built from or read with, [3*natoms]. */
 int smh_molecule_positions(const smh_molecule *m, double *apos);
 
+/* ---- one replica of the multi-replica rigid-atom port relaxation ----
+ * (surfmol_raff.h).  smh_raff_build restates, for one molecule, the set-up
+ * of the reference's RAFF benchmark (crates/libs/oclff/src/bin/
+ * raff_ocl_benchmark.rs:78-97) with oclff::pack (crates/libs/oclff/src/
+ * pack.rs): pack_molecules (nodes = atoms of bond degree > 1 first, then
+ * caps, then padding atoms "X" up to group_size; nnode < 0 infers the node
+ * count, otherwise it must equal it), masses_from_elems (inverse masses,
+ * padding 0), build_neighs_from_bonds, make_exclusions_1st_2nd,
+ * make_ports_from_neighs (port = initial neighbour displacement, stiffness
+ * k_stiff) and make_bk_slots_clustered.  group_size 0 means no padding.
+ * pos3 [3*natoms], bonds [2*nbonds] in the caller's atom order; the tables
+ * are in packed order (perm[k] = caller index of packed atom k).  Errors
+ * (no abort): bad indices, more than 4 bonds or back slots per atom, a node
+ * count that disagrees with the bonds. */
+typedef struct smh_raff_mol smh_raff_mol;
+typedef struct {
+    int natoms, nreal, nnode;
+    const float *pos3, *inv_mass;          /* [3*natoms], [natoms]          */
+    const int *neighs, *excl1, *excl2;     /* [4*natoms]                    */
+    const int *bk_slots;                   /* [4*natoms]                    */
+    const float *port_local, *kflat;       /* [16*natoms], [4*natoms]       */
+    const int *is_padding, *perm;          /* [natoms], [nreal]             */
+    const char *elems;                     /* space-separated, packed order */
+} smh_raff_tables;
+const char *smh_raff_last_error(void);
+int smh_raff_build(int natoms, const char *const *elems, const float *pos3, int nbonds, const int *bonds, int nnode,
+                   int group_size, float k_stiff, smh_raff_mol **out);
+int smh_raff_tables_of(const smh_raff_mol *m, smh_raff_tables *t);
+void smh_raff_free(smh_raff_mol *m);
+
 #ifdef __cplusplus
 }
 #endif

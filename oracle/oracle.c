@@ -1056,3 +1056,171 @@ void orc_bspline_fit(int nx, int ny, int nz, const double *E, double *C) {
 #undef IDX
     free(A); free(b);
 }
+
+/* ------------------------------------------------------------------ */
+/* Multi-replica rigid-atom port relaxation (RAFF "RRsp3" cluster step) */
+/* ------------------------------------------------------------------ */
+
+/* Shared topology of one replica (oclff::RRsp3Multi uploads,
+ * crates/libs/oclff/src/rrsp3.rs:829-929). */
+typedef struct {
+    int n, nnode;
+    const double *radius;      /* [n]                                   */
+    const int *neighs;         /* [4n] local, -1 padded                 */
+    const int *excl1, *excl2;  /* [4n]                                  */
+    const int *bk_slots;       /* [4n] node-port index inode*4+k        */
+    const int *fixmask;        /* [n] or NULL                           */
+    const double *port_local;  /* [nnode*4*3] port arms (body frame)    */
+    const double *kflat;       /* [nnode*4]                             */
+    const double *inv_mass;    /* [n]; <= 1e-12: padding, never moves   */
+} orc_raff;
+
+static inline v3 orc_qrot(const double *q, v3 v) {
+    v3 u = {q[0], q[1], q[2]};
+    v3 t = v3_mul(v3_cross(u, v), 2.0);
+    return v3_add(v3_add(v, v3_mul(t, q[3])), v3_cross(u, t));
+}
+
+static inline void orc_qmul(const double *a, const double *b, double *o) {
+    double r[4] = {a[3] * b[0] + a[0] * b[3] + a[1] * b[2] - a[2] * b[1],
+                   a[3] * b[1] - a[0] * b[2] + a[1] * b[3] + a[2] * b[0],
+                   a[3] * b[2] + a[0] * b[1] - a[1] * b[0] + a[2] * b[3],
+                   a[3] * b[3] - a[0] * b[0] - a[1] * b[1] - a[2] * b[2]};
+    o[0] = r[0]; o[1] = r[1]; o[2] = r[2]; o[3] = r[3];
+}
+
+static inline void orc_qnormalize(double *q) {
+    double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (n2 < 1e-16) { q[0] = q[1] = q[2] = 0.0; q[3] = 1.0; return; }
+    double s = 1.0 / sqrt(n2);
+    q[0] *= s; q[1] *= s; q[2] *= s; q[3] *= s;
+}
+
+/* One step of RRsp3Multi::step_cluster_multi (rrsp3.rs:1000-1026) for one
+ * replica, in FP64: compute_collision_multi (opencl/RRsp3.cl:1774-1831),
+ * compute_ports_current_multi (:1835-1933) and apply_corrections_multi
+ * (:1937-2047, massless_rot = 0), with the kernels' epsilons.  All reads of
+ * the first two phases see the positions and orientations at the start of
+ * the step.  Updates pos [3n], quat [4n] (x,y,z,w), mom [3n], qmom [4n];
+ * writes this step's dcoll [3n], dnode / drot [3*nnode], dneigh
+ * [12*nnode].  Returns the largest |component| of dcoll over the atoms and
+ * of dnode over the nodes. */
+double orc_raff_step(const orc_raff *t, double dt, double rot_mass_scale, double relaxation, double beta,
+                     int collide, double *pos, double *quat, double *mom, double *qmom, double *dcoll,
+                     double *dnode, double *drot, double *dneigh) {
+    const int n = t->n, nn = t->nnode;
+    double metric = 0.0;
+    /* collision: Jacobi sphere-pair corrections */
+    for (int i = 0; i < n; i++) {
+        v3 sum = {0, 0, 0};
+        const double wi = t->inv_mass[i], ri = t->radius[i];
+        if (collide && wi > 1e-12 && ri > 0.0) {
+            v3 p = v3_ld(pos + 3 * i);
+            for (int j = 0; j < n; j++) {
+                if (j == i) continue;
+                int ex = 0;
+                for (int k = 0; k < 4; k++) ex |= (t->excl1[4 * i + k] == j) | (t->excl2[4 * i + k] == j);
+                if (ex) continue;
+                const double wj = t->inv_mass[j], rj = t->radius[j];
+                if (wj <= 1e-12 || rj <= 0.0) continue;
+                v3 d = v3_sub(p, v3_ld(pos + 3 * j));
+                double d2 = v3_dot(d, d), rs = ri + rj;
+                if (d2 < rs * rs && d2 > 1e-16) {
+                    double dist = sqrt(d2);
+                    v3 nv = v3_mul(d, 1.0 / dist);
+                    double dl = 0.5 * ((rs - dist) / (wi + wj + 1e-12));
+                    sum = v3_add(sum, v3_mul(nv, dl * wi));
+                }
+            }
+        }
+        v3_st(dcoll + 3 * i, sum);
+        double m = fmax(fabs(sum.x), fmax(fabs(sum.y), fabs(sum.z)));
+        if (m > metric) metric = m;
+    }
+    /* ports (massfull XPBD, the Current variant) */
+    const double dt2 = dt * dt + 1e-16;
+    for (int i = 0; i < nn; i++) {
+        v3 sdp = {0, 0, 0}, sdt = {0, 0, 0};
+        const double wi = t->inv_mass[i];
+        for (int k = 0; k < 4; k++) {
+            v3 rec = {0, 0, 0};
+            const int j = t->neighs[4 * i + k];
+            if (wi > 1e-12 && j >= 0 && j < n && t->kflat[4 * i + k] > 0.0 && t->inv_mass[j] > 1e-12) {
+                const double wj = t->inv_mass[j], K = t->kflat[4 * i + k];
+                const double mi = 1.0 / wi, invI = rot_mass_scale / (0.4 * mi + 1e-12);
+                v3 arm = {t->port_local[12 * i + 3 * k], t->port_local[12 * i + 3 * k + 1], t->port_local[12 * i + 3 * k + 2]};
+                v3 ra = orc_qrot(quat + 4 * i, arm);
+                v3 diff = v3_sub(v3_ld(pos + 3 * j), v3_add(v3_ld(pos + 3 * i), ra));
+                double d2 = v3_dot(diff, diff);
+                if (d2 >= 1e-16) {
+                    double dist = sqrt(d2);
+                    v3 nv = v3_mul(diff, 1.0 / dist);
+                    v3 rxn = v3_cross(ra, nv);
+                    double wang = v3_dot(rxn, rxn) * invI;
+                    double alpha = 1.0 / (K * dt2);
+                    int jnode = j < nn;
+                    double wt = wi + wj + wang + alpha + 1e-12;
+                    if (jnode) wt += wang;
+                    double imp = dist / wt;
+                    if (jnode) imp *= 0.5;
+                    v3 P = v3_mul(nv, imp);
+                    sdp = v3_add(sdp, v3_mul(P, wi));
+                    sdt = v3_add(sdt, v3_mul(v3_cross(ra, P), invI));
+                    rec = v3_mul(P, -wj);
+                }
+            }
+            v3_st(dneigh + 12 * i + 3 * k, rec);
+        }
+        v3_st(dnode + 3 * i, sdp);
+        v3_st(drot + 3 * i, sdt);
+        double m = fmax(fabs(sdp.x), fmax(fabs(sdp.y), fabs(sdp.z)));
+        if (m > metric) metric = m;
+    }
+    /* corrections */
+    for (int i = 0; i < n; i++) {
+        if (t->inv_mass[i] <= 1e-12) continue;
+        const int isnode = i < nn;
+        v3 dxp = isnode ? v3_ld(dnode + 3 * i) : (v3){0, 0, 0};
+        for (int k = 0; k < 4; k++) {
+            const int sl = t->bk_slots[4 * i + k];
+            if (sl >= 0) dxp = v3_add(dxp, v3_ld(dneigh + 3 * sl));
+        }
+        v3 dx = v3_add(v3_ld(dcoll + 3 * i), dxp);
+        const int msk = t->fixmask ? t->fixmask[i] : 0;
+        if (msk & 1) dx.x = 0.0;
+        if (msk & 2) dx.y = 0.0;
+        if (msk & 4) dx.z = 0.0;
+        v3 mv = v3_mul(dx, relaxation);
+        if (beta != 0.0) mv = v3_add(mv, v3_mul(v3_ld(mom + 3 * i), beta));
+        if (msk & 8) {
+            double zold = pos[3 * i + 2];
+            if (fabs(zold) > 1e-6) mv.z -= zold;
+            else mv.z = 0.0;
+        }
+        v3_st(pos + 3 * i, v3_add(v3_ld(pos + 3 * i), mv));
+        v3_st(mom + 3 * i, mv);
+        if (!isnode || (msk & 7)) continue;
+        v3 dth = v3_mul(v3_ld(drot + 3 * i), relaxation);
+        double ang = sqrt(v3_dot(dth, dth));
+        double *q = quat + 4 * i;
+        double qold[4] = {q[0], q[1], q[2], q[3]}, qn[4] = {q[0], q[1], q[2], q[3]};
+        if (ang > 1e-8) {
+            /* quat_from_axis_angle with the axis dtheta/|dtheta| */
+            v3 ax = v3_mul(dth, 1.0 / ang);
+            double s = sin(0.5 * ang), c = cos(0.5 * ang);
+            double dq[4] = {ax.x * s, ax.y * s, ax.z * s, c};
+            orc_qmul(dq, qold, qn);
+            orc_qnormalize(qn);
+        }
+        if (beta != 0.0)
+            for (int k = 0; k < 4; k++) qn[k] += qmom[4 * i + k] * beta;
+        orc_qnormalize(qn);
+        if (qn[0] * qold[0] + qn[1] * qold[1] + qn[2] * qold[2] + qn[3] * qold[3] < 0.0)
+            for (int k = 0; k < 4; k++) qn[k] = -qn[k];
+        for (int k = 0; k < 4; k++) {
+            qmom[4 * i + k] = qn[k] - qold[k];
+            q[k] = qn[k];
+        }
+    }
+    return metric;
+}

@@ -116,6 +116,8 @@ def lib():
         L.orc_fe3d_pbc_comb.argtypes = [P(d), i, i, i, P(d), P(d), P(d)]
         L.orc_plqh.argtypes = [P(d), d, P(d)]
         L.orc_gridff_build.argtypes = [i, P(d), P(d), P(d), i, i, i, P(d), P(i), d, d, P(d)]
+        L.orc_raff_step.argtypes = [P(OrcRaff), d, d, d, d, i, P(d), P(d), P(d), P(d), P(d), P(d), P(d), P(d)]
+        L.orc_raff_step.restype = d
         L.orc_gridff_build_nodes.argtypes = [i, P(d), P(d), P(d), i, i, i, P(d), P(i), d, d, i, P(i), P(d)]
         L.orc_bspline_apply.argtypes = [i, i, i, P(d), P(d)]
         L.orc_bspline_fit.argtypes = [i, i, i, P(d), P(d)]
@@ -146,6 +148,16 @@ class OrcSystem(ctypes.Structure):
         ("mol_id", ctypes.POINTER(ctypes.c_int)), ("rcut_inter2", ctypes.c_double), ("layer_cells", ctypes.c_int),
         ("rswitch_inter", ctypes.c_double), ("pbc_lx", ctypes.c_double), ("pbc_ly", ctypes.c_double),
         ("fscale", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+class OrcRaff(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("nnode", ctypes.c_int), ("radius", ctypes.POINTER(ctypes.c_double)),
+        ("neighs", ctypes.POINTER(ctypes.c_int)), ("excl1", ctypes.POINTER(ctypes.c_int)),
+        ("excl2", ctypes.POINTER(ctypes.c_int)), ("bk_slots", ctypes.POINTER(ctypes.c_int)),
+        ("fixmask", ctypes.POINTER(ctypes.c_int)), ("port_local", ctypes.POINTER(ctypes.c_double)),
+        ("kflat", ctypes.POINTER(ctypes.c_double)), ("inv_mass", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -1041,3 +1053,146 @@ def bspline_fit(E, ns):
     out = np.zeros_like(e)
     lib().orc_bspline_fit(nx, ny, nz, _pd(e), _pd(out))
     return out.reshape(nx, ny, nz, 4)
+
+
+# ---------------------------------------------------------------------------
+# multi-replica rigid-atom port relaxation (RAFF "RRsp3" cluster step)
+# ---------------------------------------------------------------------------
+_RAFF_MASS = {"H": 1.008, "C": 12.011, "N": 14.007, "O": 15.999, "F": 18.998, "P": 30.974, "S": 32.06, "Cl": 35.45,
+              "Br": 79.904, "I": 126.904, "Na": 22.990, "K": 39.098, "Mg": 24.305, "Ca": 40.078, "Fe": 55.845,
+              "Cu": 63.546, "Zn": 65.38, "Al": 26.982, "Si": 28.085, "B": 10.81, "Li": 6.94, "Be": 9.012,
+              "Ne": 20.180, "Ar": 39.948}
+
+
+def raff_pack(elems, pos, bonds, nnode=-1, group_size=0, k_stiff=200.0):
+    """One replica's tables, restating oclff::pack for one molecule
+    (crates/libs/oclff/src/pack.rs): pack_molecules (:54-116, nodes = bond
+    degree > 1 first, caps, padding "X" to group_size), masses_from_elems
+    (:132-150, float32 inverse masses), build_neighs_from_bonds (:155-170),
+    make_exclusions_1st_2nd (:176-211), make_ports_from_neighs (:293-316) and
+    make_bk_slots_clustered (:218-244).  Float tables are float32 as there."""
+    n = len(elems)
+    pos = np.asarray(pos, dtype=np.float32).reshape(n, 3)
+    N = group_size or n
+    if n > N:
+        raise ValueError("more atoms than group_size")
+    deg = [0] * n
+    for i, j in bonds:
+        if i != j:
+            deg[i] += 1
+            deg[j] += 1
+    ninf = sum(1 for x in deg if x > 1)
+    if nnode < 0:
+        nnode = ninf
+    if nnode != ninf:
+        raise ValueError(f"nnode {nnode} but {ninf} atoms of degree > 1")
+    perm = [i for i in range(n) if deg[i] > 1] + [i for i in range(n) if deg[i] <= 1]
+    inv = {p: k for k, p in enumerate(perm)}
+    el = [elems[p] for p in perm] + ["X"] * (N - n)
+    P = np.zeros((N, 3), dtype=np.float32)
+    P[:n] = pos[perm]
+    inv_mass = np.zeros(N, dtype=np.float32)
+    for k in range(n):
+        inv_mass[k] = np.float32(1.0) / np.float32(_RAFF_MASS.get(el[k], 12.0))
+    pb = [(inv[i], inv[j]) for i, j in bonds]
+    neighs = -np.ones((N, 4), dtype=np.int32)
+    d = [0] * N
+    for i, j in pb:
+        if d[i] >= 4 or d[j] >= 4:
+            raise ValueError("more than 4 bonds")
+        neighs[i, d[i]] = j
+        neighs[j, d[j]] = i
+        d[i] += 1
+        d[j] += 1
+    excl1 = -np.ones((N, 4), dtype=np.int32)
+    excl2 = -np.ones((N, 4), dtype=np.int32)
+    for i in range(N):
+        n1 = [int(x) for x in neighs[i] if x >= 0]
+        excl1[i, :len(n1)] = n1
+        s2 = []
+        for j in n1:
+            for t in neighs[j]:
+                t = int(t)
+                if t < 0 or t == i or t in n1 or t in s2:
+                    continue
+                s2.append(t)
+                if len(s2) >= 4:
+                    break
+            if len(s2) >= 4:
+                break
+        excl2[i, :len(s2)] = s2
+    port = np.zeros((N, 4, 3), dtype=np.float32)
+    kflat = np.zeros((N, 4), dtype=np.float32)
+    for i in range(N):
+        for k in range(4):
+            j = neighs[i, k]
+            if j >= 0:
+                port[i, k] = P[j] - P[i]
+                kflat[i, k] = k_stiff
+    bk = -np.ones((N, 4), dtype=np.int32)
+    cnt = [0] * N
+    for il in range(nnode):
+        for k in range(4):
+            ja = neighs[il, k]
+            if ja < 0:
+                continue
+            if cnt[ja] >= 4:
+                raise ValueError("more than 4 back slots")
+            bk[ja, cnt[ja]] = 4 * il + k
+            cnt[ja] += 1
+    return dict(natoms=N, nreal=n, nnode=nnode, perm=np.array(perm, dtype=np.int32), elems=el, pos3=P,
+                inv_mass=inv_mass, neighs=neighs, excl1=excl1, excl2=excl2, port_local=port, kflat=kflat, bk_slots=bk)
+
+
+class RaffReplica:
+    """One replica's shared topology for orc_raff_step (FP64 copies of the
+    float32 tables)."""
+
+    def __init__(self, tab, radius=None, fixmask=None):
+        N, nn = tab["natoms"], tab["nnode"]
+        self.n, self.nnode = N, nn
+        self.radius = np.zeros(N) if radius is None else np.asarray(radius, dtype=np.float64).copy()
+        self.neighs = _i32(tab["neighs"], (-1,))
+        self.excl1 = _i32(tab["excl1"], (-1,))
+        self.excl2 = _i32(tab["excl2"], (-1,))
+        self.bk_slots = _i32(tab["bk_slots"], (-1,))
+        self.fixmask = None if fixmask is None else _i32(fixmask, (-1,))
+        self.port_local = _f64(np.asarray(tab["port_local"], dtype=np.float64)[:nn].reshape(-1), (-1,))
+        self.kflat = _f64(np.asarray(tab["kflat"], dtype=np.float64)[:nn].reshape(-1), (-1,))
+        if self.port_local.size == 0:
+            self.port_local = np.zeros(1)
+            self.kflat = np.zeros(1)
+        self.inv_mass = _f64(tab["inv_mass"], (-1,))
+        self.collide = bool((self.radius > 0).any())
+        s = OrcRaff()
+        s.n, s.nnode = N, nn
+        s.radius, s.neighs, s.excl1, s.excl2 = _pd(self.radius), _pi(self.neighs), _pi(self.excl1), _pi(self.excl2)
+        s.bk_slots = _pi(self.bk_slots)
+        s.fixmask = _pi(self.fixmask) if self.fixmask is not None else None
+        s.port_local, s.kflat, s.inv_mass = _pd(self.port_local), _pd(self.kflat), _pd(self.inv_mass)
+        self.cs = s
+
+    def run(self, pos, quat=None, nsteps=1, dt=0.1, relaxation=0.5, beta=0.0, rot_mass_scale=1.0, tol=None,
+            mom=None, qmom=None):
+        """nsteps steps (or until the step's largest correction component is
+        below tol).  Returns dict(pos, quat, mom, qmom, dcoll, dnode, drot,
+        dneigh, metric, steps)."""
+        N, nn = self.n, self.nnode
+        x = _f64(pos, (N, 3)).copy()
+        q = np.tile([0.0, 0.0, 0.0, 1.0], (N, 1)) if quat is None else _f64(quat, (N, 4)).copy()
+        m = np.zeros((N, 3)) if mom is None else _f64(mom, (N, 3)).copy()
+        qm = np.zeros((N, 4)) if qmom is None else _f64(qmom, (N, 4)).copy()
+        dc = np.zeros((N, 3))
+        dn = np.zeros((max(nn, 1), 3))
+        dr = np.zeros((max(nn, 1), 3))
+        dg = np.zeros((max(nn, 1) * 4, 3))
+        metric, steps = 0.0, 0
+        for _ in range(nsteps):
+            metric = lib().orc_raff_step(ctypes.byref(self.cs), float(dt), float(rot_mass_scale), float(relaxation),
+                                         float(beta), 1 if self.collide else 0, _pd(x), _pd(q), _pd(m), _pd(qm),
+                                         _pd(dc), _pd(dn), _pd(dr), _pd(dg))
+            steps += 1
+            if tol is not None and metric < tol:
+                break
+        return dict(pos=x, quat=q, mom=m, qmom=qm, dcoll=dc, dnode=dn[:nn], drot=dr[:nn], dneigh=dg[:4 * nn],
+                    metric=metric, steps=steps)

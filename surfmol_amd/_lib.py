@@ -1,4 +1,5 @@
-"""ctypes binding of libsurfmol_relax.so (include/surfmol_relax.h, surfmol_host.h).
+"""ctypes binding of libsurfmol_relax.so (include/surfmol_relax.h, surfmol_host.h,
+surfmol_raff.h).
 
 The library is built in-tree by ``surfmol_amd/build.py``.  Loading fails
 loudly when it is missing: there is no CPU fallback anywhere in the package.
@@ -22,9 +23,25 @@ SMR_SYS_RUNNING = 0
 SMR_SYS_CONVERGED = 1
 SMR_SYS_MAXSTEPS = 2
 SMR_SYS_NONFINITE = 3
+SMR_RAFF_RUNNING = 0
+SMR_RAFF_CONVERGED = 1
+SMR_RAFF_MAXSTEPS = 2
 
 P = ctypes.POINTER
 c_int, c_double, c_float, c_char_p, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_char_p, ctypes.c_void_p
+
+
+class SmrRaffStepConfig(ctypes.Structure):
+    _fields_ = [("dt", c_float), ("k_coll", c_float), ("relaxation", c_float), ("bbox_margin", c_float),
+                ("momentum_beta", c_float), ("rot_mass_scale", c_float), ("n_rot_substeps", c_int),
+                ("rot_eps", c_float), ("theta_max", c_float), ("damp", c_float)]
+
+
+class SmhRaffTables(ctypes.Structure):
+    _fields_ = [("natoms", c_int), ("nreal", c_int), ("nnode", c_int), ("pos3", P(c_float)), ("inv_mass", P(c_float)),
+                ("neighs", P(c_int)), ("excl1", P(c_int)), ("excl2", P(c_int)), ("bk_slots", P(c_int)),
+                ("port_local", P(c_float)), ("kflat", P(c_float)), ("is_padding", P(c_int)), ("perm", P(c_int)),
+                ("elems", c_char_p)]
 
 
 class SmrTopology(ctypes.Structure):
@@ -119,6 +136,26 @@ _SIGNATURES = [
     ("smh_molecule_write_ufftopo", c_int, [c_void_p, c_char_p]),
     ("smh_molecule_write_json", c_int, [c_void_p, c_char_p]),
     ("smh_molecule_positions", c_int, [c_void_p, P(c_double)]),
+    ("smh_raff_last_error", c_char_p, []),
+    ("smh_raff_build", c_int, [c_int, P(c_char_p), P(c_float), c_int, P(c_int), c_int, c_int, c_float, P(c_void_p)]),
+    ("smh_raff_tables_of", c_int, [c_void_p, P(SmhRaffTables)]),
+    ("smh_raff_free", None, [c_void_p]),
+    ("smr_raff_create", c_int, [c_void_p, c_int, c_int, P(c_void_p)]),
+    ("smr_raff_destroy", c_int, [c_void_p]),
+    ("smr_raff_upload_radius", c_int, [c_void_p, P(c_float)]),
+    ("smr_raff_upload_neighs_and_exclusions", c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),
+    ("smr_raff_upload_fixmask", c_int, [c_void_p, P(c_int)]),
+    ("smr_raff_upload_cluster_ports", c_int, [c_void_p, P(c_float), P(c_float), c_int]),
+    ("smr_raff_upload_bk_slots", c_int, [c_void_p, P(c_int)]),
+    ("smr_raff_upload_state", c_int, [c_void_p, P(c_float), P(c_float), P(c_float)]),
+    ("smr_raff_step", c_int, [c_void_p, P(SmrRaffStepConfig), c_int]),
+    ("smr_raff_relax", c_int, [c_void_p, P(SmrRaffStepConfig), c_int, c_float, P(c_int), P(c_int), P(c_float)]),
+    ("smr_raff_download_pos", c_int, [c_void_p, P(c_float)]),
+    ("smr_raff_download_pos_replica", c_int, [c_void_p, c_int, P(c_float)]),
+    ("smr_raff_download_quat_replica", c_int, [c_void_p, c_int, P(c_float)]),
+    ("smr_raff_download_dpos_coll_replica", c_int, [c_void_p, c_int, P(c_float)]),
+    ("smr_raff_download_dpos_node_replica", c_int, [c_void_p, c_int, P(c_float)]),
+    ("smr_raff_download_metric", c_int, [c_void_p, P(c_float)]),
 ]
 
 EXPORTED_SYMBOLS = [s[0] for s in _SIGNATURES]

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/surfmol_relax.h"
+#include "../../include/surfmol_raff.h"
 #include "smr_device.h"
 
 namespace smr {
@@ -159,6 +160,26 @@ hipError_t launch_energy64_inter(const LayerArgs &L, SysState *state, const int 
 hipError_t launch_plain_step_forces(const PlainArgs &P, int natoms, int nang, int ndih, int ninv, hipStream_t st);
 hipError_t launch_plain_reduce(const PlainArgs &P, double dt, double flim, hipStream_t st);
 hipError_t launch_plain_apply(const PlainArgs &P, int natoms, float a, float b, int reset, float dt, hipStream_t st);
+
+// multi-replica rigid-atom port relaxation (raff_kernels.hip)
+struct RaffArgs {
+    int natoms, nnode, nsys, nsteps;
+    // shared topology (local atom index)
+    const float *radius;       // [natoms]
+    const int4 *neighs, *excl1, *excl2, *bk_slots;   // [natoms]
+    const int *fixmask;        // [natoms] or NULL
+    const float4 *port_local;  // [nnode*4]
+    const float *kflat;        // [nnode*4]
+    // per-replica state [nsys*natoms] and correction buffers
+    float4 *pos, *quat, *dpos_mom, *dquat_mom, *dpos_coll;
+    float4 *dpos_node, *drot_node;   // [nsys*nnode]
+    float4 *dpos_neigh;              // [nsys*nnode*4]
+    float *metric;                   // [nsys] max |component| of the last step's corrections
+    int *steps, *status;             // [nsys] relax mode
+    float dt, rot_mass_scale, relaxation, beta, tol;
+    int collide, relax, max_steps;
+};
+hipError_t launch_raff(const RaffArgs &A, int seg, hipStream_t st);
 
 // GridFF build / fit (gridff_kernels.hip)
 struct GridBuildArgs {

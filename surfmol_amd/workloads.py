@@ -360,3 +360,51 @@ def lognormal_sizes(n: int, median: float = 60.0, sigma: float = 0.5, lo: int = 
     rng = np.random.Generator(np.random.PCG64(seed))
     s = np.exp(rng.normal(math.log(median), sigma, n))
     return np.clip(np.round(s), lo, hi).astype(int)
+
+
+def raff_perturbed_replicas(pos, is_h, is_padding, nsys, perturb=0.2, seed=12345):
+    """Replicas of a packed RAFF molecule with the random perturbations of the
+    reference's RAFF benchmark (crates/libs/oclff/src/bin/
+    raff_ocl_benchmark.rs:99-116): a 64-bit LCG (multiplier
+    6364136223846793005, increment 1442695040888963407) draws, per replica,
+    atom and coordinate, r = (state >> 33) / 2^31 - 0.5 and adds
+    2 amp r with amp = perturb (1.5 perturb for H); padding atoms stay at 0.
+    Returns float32 [nsys, natoms, 3]."""
+    pos = np.asarray(pos, dtype=np.float32)
+    n = len(pos)
+    out = np.zeros((nsys, n, 3), dtype=np.float32)
+    mask = (1 << 64) - 1
+    s = seed
+    for k in range(nsys):
+        for i in range(n):
+            if is_padding[i]:
+                continue
+            amp = np.float32(perturb * 1.5 if is_h[i] else perturb)
+            for d in range(3):
+                s = (s * 6364136223846793005 + 1442695040888963407) & mask
+                r = np.float32(np.float32(s >> 33) / np.float32(1 << 31)) - np.float32(0.5)
+                out[k, i, d] = pos[i, d] + amp * r * np.float32(2.0)
+    return out
+
+
+def xylitol_raff_molecule(golden_xyz):
+    """Elements, float32 positions and bonds of xylitol as the reference's RAFF
+    benchmark builds them (raff_ocl_benchmark.rs:24-47): a bond wherever
+    d < r_i + r_j + 0.4 with covalent radii H 0.31, C 0.76, N 0.71, O 0.66."""
+    rc = {"H": 0.31, "C": 0.76, "N": 0.71, "O": 0.66, "F": 0.57, "P": 1.07, "S": 1.05, "Cl": 1.02}
+    el, x = [], []
+    with open(golden_xyz) as f:
+        lines = f.read().splitlines()
+    n = int(lines[0].split()[0])
+    for ln in lines[2:2 + n]:
+        w = ln.split()
+        el.append(w[0])
+        x.append([float(w[1]), float(w[2]), float(w[3])])
+    x = np.array(x)
+    bonds = []
+    for i in range(n):
+        for j in range(i + 1, n):
+            cut = rc.get(el[i], 0.7) + rc.get(el[j], 0.7) + 0.4
+            if ((x[j] - x[i]) ** 2).sum() < cut * cut:
+                bonds.append((i, j))
+    return el, x.astype(np.float32), bonds

@@ -8,7 +8,7 @@ from conftest import ROOT
 
 def _declared():
     names = set()
-    for h in ["surfmol_relax.h", "surfmol_host.h"]:
+    for h in ["surfmol_relax.h", "surfmol_host.h", "surfmol_raff.h"]:
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"\b(smr_[a-z0-9_]+|smh_[a-z0-9_]+)\s*\(", txt):
