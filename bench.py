@@ -113,6 +113,65 @@ def workload_uneven(args, S, W):
     return mols, np.arange(nsys, dtype=np.int32), X, 48.0, desc
 
 
+def raff_bench(args, S, W, nsys, nsteps=100, warmup=10, cpu_seconds=0.0):
+    """The reference's RAFF benchmark (crates/libs/oclff/src/bin/
+    raff_ocl_benchmark.rs): xylitol (22 atoms, 10 nodes) packed into 32-atom
+    groups, port stiffness 200, its LCG perturbations, StepConfig dt 0.1,
+    relaxation 0.5, no collisions; `nsteps` steps of every replica timed after
+    `warmup` steps.  The unit is the reference report's: real atom-steps/s."""
+    from surfmol_amd.raff import RaffMolecule, RaffReplicas, StepConfig
+    el, x, bonds = W.xylitol_raff_molecule(os.path.join(ROOT, "tests", "golden", "xylitol.xyz"))
+    m = RaffMolecule.build(el, x, bonds, group_size=32)
+    X = W.raff_perturbed_replicas(m.pos3, [e == "H" for e in m.elems], m.is_padding, nsys)
+    ctx = S.Context(0)
+    rep = RaffReplicas.from_molecule(ctx, m, X, nsys)
+    cfg = StepConfig(dt=0.1, k_coll=0.0, relaxation=0.5)
+    rep.step(cfg, warmup)
+    t0 = time.perf_counter()
+    rep.step(cfg, nsteps)                      # one persistent launch (synchronises)
+    T = time.perf_counter() - t0
+    # the reference's shape: one call (and launch) per step
+    t0 = time.perf_counter()
+    for _ in range(20):
+        rep.step(cfg, 1)
+    T1 = (time.perf_counter() - t0) / 20
+    nports = int((m.neighs[:m.nnode] >= 0).sum())
+    flop_step = 100.0 * nports + 30.0 * m.nreal + 60.0 * m.nnode    # per replica-step (stated in DESIGN)
+    ach = flop_step * nsys * nsteps / T / 1e12
+    out = {"metric": "RAFF replica relaxation throughput (real atom-steps/s)",
+           "value": round(m.nreal * nsys * nsteps / T, 1), "unit": "atom-steps/s", "n_gpus": 1,
+           "steps": nsteps, "warmup": warmup, "ms_per_step": round(1e3 * T / nsteps, 5), "higher_is_better": True,
+           "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": f"RAFF multi-replica: {nsys} xylitol replicas (22 atoms, 10 nodes, 21 ports, "
+                                  f"packed to 32), K 200, LCG perturbations 0.2 A (H 0.3 A), dt 0.1, relaxation 0.5, "
+                                  f"no collisions; {nsteps} steps in one persistent launch",
+                      "nsys": nsys, "natoms_real": m.nreal},
+           "one_launch_per_step": {"ms_per_step": round(1e3 * T1, 4),
+                                   "value": round(m.nreal * nsys / T1, 1), "unit": "atom-steps/s"},
+           "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(ach / FP32_PEAK_TFLOPS, 4), "flop_per_replica_step": flop_step,
+                        "traffic": None, "note": "state stays in registers/LDS during the launch"},
+           "reference_published": {"value": 5.6e8, "unit": "atom-steps/s", "hardware": "NVIDIA GTX 1650, 5000 "
+                                   "replicas", "source": "notes/reports/2026-08-29_gpu_raff_benchmark.md:24"}}
+    if cpu_seconds > 0:
+        from oracle import oracle as O
+        O.build()
+        t = O.raff_pack(el, x, bonds, group_size=32)
+        orc = O.RaffReplica(t)
+        n, tc = 0, 0.0
+        while tc < cpu_seconds:
+            t0 = time.perf_counter()
+            orc.run(X[n % nsys], nsteps=100)
+            tc += time.perf_counter() - t0
+            n += 1
+        out["cpu_baseline"] = {"value": round(m.nreal * 100 * n / tc, 1), "unit": "atom-steps/s", "cores": 1,
+                               "kind": "port", "sample": f"{n} replicas x 100 steps on the FP64 C restatement "
+                                                         f"(oracle/oracle.c orc_raff_step), one thread, {tc:.1f} s",
+                               **host_info()}
+    rep.close()
+    return out
+
+
 def host_info():
     """Model name and usable core count of this host (the CPU baseline's)."""
     model = None
@@ -435,7 +494,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["uniform", "uneven", "layer", "serve"], default="uniform")
+    ap.add_argument("--workload", choices=["uniform", "uneven", "layer", "serve", "raff"], default="uniform")
     ap.add_argument("--rates", type=str, default="200,1000,5000", help="serve: arrival rates [req/s]")
     ap.add_argument("--requests", type=int, default=1000, help="serve: requests per rate")
     ap.add_argument("--chunk-steps", type=int, default=64, help="serve: FIRE steps per persistent launch (<= 64 atoms)")
@@ -479,6 +538,10 @@ def main():
         print(json.dumps(serve(args, S, W, 1.5, 1.0, [float(r) for r in args.rates.split(",") if r],
                                args.requests)))
         return
+    if args.workload == "raff":
+        print(json.dumps(raff_bench(args, S, W, args.nsys or 5000, nsteps=max(args.steps, 100), warmup=10,
+                                    cpu_seconds=0.0 if args.no_cpu else args.cpu_seconds)))
+        return
     out = run_workload(args, S, W, args.workload, args.steps, args.warmup, args.cpu_seconds, not args.no_plain,
                        not args.no_cpu)
     if args.workload == "uniform" and not args.no_extras:
@@ -510,6 +573,12 @@ def main():
                            "p99_ms": r["p99_ms"], "served_per_s": r["served_per_s"], "converged": r["converged"],
                            "served_vs_plain": r["served_vs_plain"], "workload": sv["config"]["workload"],
                            "wall_s": round(time.time() - t0, 1)}
+        t0 = time.time()
+        ra = raff_bench(args, S, W, 5000)
+        extras["raff"] = {"value": ra["value"], "unit": ra["unit"], "ms_per_step": ra["ms_per_step"],
+                          "one_launch_per_step_ms": ra["one_launch_per_step"]["ms_per_step"],
+                          "roofline_frac": ra["roofline"]["frac"], "reference_published": ra["reference_published"],
+                          "workload": ra["config"]["workload"], "wall_s": round(time.time() - t0, 1)}
         out["other_configs"] = extras
     print(json.dumps(out))
 
