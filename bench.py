@@ -153,6 +153,35 @@ def raff_bench(args, S, W, nsys, nsteps=100, warmup=10, cpu_seconds=0.0):
                         "traffic": None, "note": "state stays in registers/LDS during the launch"},
            "reference_published": {"value": 5.6e8, "unit": "atom-steps/s", "hardware": "NVIDIA GTX 1650, 5000 "
                                    "replicas", "source": "notes/reports/2026-08-29_gpu_raff_benchmark.md:24"}}
+    # serving: Poisson arrivals of single-replica relax requests (tol 2e-4,
+    # <= 3000 steps) into a 4096-slot server, 64-step chunks
+    from surfmol_amd.raff import RaffServer
+    nreq, rate = min(nsys, 4000), 20000.0
+    tmpl = RaffReplicas.from_molecule(ctx, m, X[:1], 1)
+    srv = RaffServer(tmpl, capacity=4096, chunk_steps=64, max_steps=3000, tol=2e-4, cfg=cfg)
+    arr = W.poisson_arrivals(rate, nreq, W.SEED_BASE + 77)
+    ids = []
+    t0 = time.perf_counter()
+    for k in range(nreq):
+        wait = arr[k] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        ids.append(srv.submit(X[k]))
+    srv.drain()
+    res = [srv.poll(i) for i in ids]
+    sst = srv.stats()
+    srv.close()
+    tmpl.close()
+    lat = np.array([r["latency"] for r in res])
+    span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    out["serve"] = {"rate_req_s": rate, "requests": nreq,
+                    "converged": int(sum(1 for r in res if r["status"] == 1)),
+                    "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+                    "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+                    "served_per_s": round(nreq / span, 1),
+                    "steps_mean": round(float(np.mean([r["steps"] for r in res])), 1),
+                    "chunks": int(sst["chunks"]), "peak_resident": int(sst["peak_resident"]),
+                    "config": "4096 slots, 64-step chunks, tol 2e-4 on the step's largest correction, <= 3000 steps"}
     if cpu_seconds > 0:
         from oracle import oracle as O
         O.build()
@@ -578,6 +607,7 @@ def main():
         extras["raff"] = {"value": ra["value"], "unit": ra["unit"], "ms_per_step": ra["ms_per_step"],
                           "one_launch_per_step_ms": ra["one_launch_per_step"]["ms_per_step"],
                           "roofline_frac": ra["roofline"]["frac"], "reference_published": ra["reference_published"],
+                          "serve": ra["serve"],
                           "workload": ra["config"]["workload"], "wall_s": round(time.time() - t0, 1)}
         out["other_configs"] = extras
     print(json.dumps(out))

@@ -99,6 +99,42 @@ int smr_raff_download_dpos_coll_replica(const smr_raff *r, int isys, float *d3);
 int smr_raff_download_dpos_node_replica(const smr_raff *r, int isys, float *d3);
 int smr_raff_download_metric(const smr_raff *r, float *metric);
 
+/* ----------------------------------------------------------- serving */
+typedef struct smr_raff_server smr_raff_server;
+typedef struct {
+    int capacity;              /* replicas resident at once (slots)              */
+    int chunk_steps;           /* steps per persistent launch                     */
+    int max_steps;             /* step cap per request                            */
+    float tol;                 /* retire when the step's largest |correction
+                                  component| < tol (as smr_raff_relax)            */
+    smr_raff_step_config step;
+} smr_raff_server_config;
+
+/* Continuous-batching relax server for replicas of one molecule (no
+ * counterpart in the reference, whose multi-replica harness steps a fixed
+ * set).  The template supplies the shared topology and the inverse masses
+ * (its first replica's): its uploads must be complete.  A worker thread
+ * admits queued requests into free slots, relaxes every resident replica by
+ * chunked persistent launches and retires each one when it converges or
+ * reaches max_steps; requests that arrive meanwhile join at the next chunk.
+ * A request's result equals smr_raff_relax's on a one-replica set. */
+int smr_raff_server_create(const smr_raff *tmpl, const smr_raff_server_config *cfg, smr_raff_server **out);
+/* One replica's positions [n*3] (float32, packed order) and orientations
+ * [n*4] (NULL = identity).  Thread-safe. */
+int smr_raff_server_submit(smr_raff_server *s, const float *pos3, const float *quat, long long *id);
+/* *done = 1 when the request has retired: steps, status (SMR_RAFF_*), the
+ * last step's largest |correction component|, final positions [n*3] and the
+ * submit / retire times (seconds on the server's steady clock) are filled
+ * (each may be NULL) and the request is forgotten. */
+int smr_raff_server_poll(smr_raff_server *s, long long id, int *done, int *steps, int *status, float *metric,
+                         float *pos3, double *t_submit, double *t_done);
+/* Block until every submitted request has retired. */
+int smr_raff_server_drain(smr_raff_server *s);
+/* out[7] = (submitted, retired, resident, queued, chunks, peak resident,
+ * seconds of device time in chunks). */
+int smr_raff_server_stats(smr_raff_server *s, double *out);
+int smr_raff_server_destroy(smr_raff_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,11 @@ class SmrRaffStepConfig(ctypes.Structure):
                 ("rot_eps", c_float), ("theta_max", c_float), ("damp", c_float)]
 
 
+class SmrRaffServerConfig(ctypes.Structure):
+    _fields_ = [("capacity", c_int), ("chunk_steps", c_int), ("max_steps", c_int), ("tol", c_float),
+                ("step", SmrRaffStepConfig)]
+
+
 class SmhRaffTables(ctypes.Structure):
     _fields_ = [("natoms", c_int), ("nreal", c_int), ("nnode", c_int), ("pos3", P(c_float)), ("inv_mass", P(c_float)),
                 ("neighs", P(c_int)), ("excl1", P(c_int)), ("excl2", P(c_int)), ("bk_slots", P(c_int)),
@@ -156,6 +161,13 @@ _SIGNATURES = [
     ("smr_raff_download_dpos_coll_replica", c_int, [c_void_p, c_int, P(c_float)]),
     ("smr_raff_download_dpos_node_replica", c_int, [c_void_p, c_int, P(c_float)]),
     ("smr_raff_download_metric", c_int, [c_void_p, P(c_float)]),
+    ("smr_raff_server_create", c_int, [c_void_p, P(SmrRaffServerConfig), P(c_void_p)]),
+    ("smr_raff_server_submit", c_int, [c_void_p, P(c_float), P(c_float), P(ctypes.c_longlong)]),
+    ("smr_raff_server_poll", c_int, [c_void_p, ctypes.c_longlong, P(c_int), P(c_int), P(c_int), P(c_float), P(c_float),
+                                     P(c_double), P(c_double)]),
+    ("smr_raff_server_drain", c_int, [c_void_p]),
+    ("smr_raff_server_stats", c_int, [c_void_p, P(c_double)]),
+    ("smr_raff_server_destroy", c_int, [c_void_p]),
 ]
 
 EXPORTED_SYMBOLS = [s[0] for s in _SIGNATURES]

@@ -10,7 +10,7 @@
 #include <vector>
 
 #include "../../include/surfmol_raff.h"
-#include "smr_internal.h"
+#include "raff_internal.h"
 
 using namespace smr;
 
@@ -20,25 +20,17 @@ using namespace smr;
         if (_e != hipSuccess) return hip_fail(_e, #expr);         \
     } while (0)
 
-struct smr_raff {
-    smr_ctx *ctx = nullptr;
-    int n = 0, nsys = 0, nnode = -1, seg = 0;
-    bool collide = false;
-    DevBuf radius, neighs, excl1, excl2, bk_slots, fixmask, port_local, kflat;
-    DevBuf pos, quat, dpos_mom, dquat_mom, dpos_coll;
-    DevBuf dpos_node, drot_node, dpos_neigh;
-    DevBuf metric, steps, status;
-    std::vector<int> bk_host, neigh_host;
-    bool have_state = false, have_neighs = false, have_bk = false;
-};
+namespace smr {
 
-namespace {
-
-int seg_for(int n) {
+int raff_seg_for(int n) {
     int s = 16;
     while (s < n) s <<= 1;
     return s;
 }
+
+}  // namespace smr
+
+namespace {
 
 hipError_t zero(DevBuf &b, hipStream_t st) { return b.bytes ? hipMemsetAsync(b.ptr, 0, b.bytes, st) : hipSuccess; }
 
@@ -51,7 +43,10 @@ int check_rows(const int *v, int count, int lo, int hi, const char *who, const c
     return SMR_OK;
 }
 
-int check_cfg(const smr_raff_step_config *c, const char *who) {
+}  // namespace
+
+namespace smr {
+int raff_check_cfg(const smr_raff_step_config *c, const char *who) {
     if (!c) return fail(SMR_ERR_INVALID, std::string(who) + ": cfg == NULL");
     if (!(c->dt > 0.f) || !std::isfinite(c->dt) || !std::isfinite(c->relaxation) || !std::isfinite(c->momentum_beta) ||
         !(c->rot_mass_scale >= 0.f) || !std::isfinite(c->rot_mass_scale))
@@ -60,7 +55,7 @@ int check_cfg(const smr_raff_step_config *c, const char *who) {
     return SMR_OK;
 }
 
-int ready(const smr_raff *r, const char *who) {
+int raff_ready(const smr_raff *r, const char *who) {
     if (!r->have_state || !r->have_neighs || !r->have_bk || r->nnode < 0)
         return fail(SMR_ERR_STATE, std::string(who) + ": upload the neighbours, cluster ports, back slots and state "
                                                       "first");
@@ -73,7 +68,7 @@ int ready(const smr_raff *r, const char *who) {
     return SMR_OK;
 }
 
-RaffArgs args_of(smr_raff *r, const smr_raff_step_config *c, int nsteps) {
+RaffArgs raff_args(smr_raff *r, const smr_raff_step_config *c, int nsteps) {
     RaffArgs A;
     memset(&A, 0, sizeof(A));
     A.natoms = r->n;
@@ -107,6 +102,9 @@ RaffArgs args_of(smr_raff *r, const smr_raff_step_config *c, int nsteps) {
     return A;
 }
 
+}  // namespace smr
+
+namespace {
 // float4 rows of a buffer -> 3 floats per row
 int download3(const DevBuf &b, size_t row0, size_t nrows, float *out, hipStream_t st) {
     std::vector<float> h(4 * nrows);
@@ -132,7 +130,7 @@ extern "C" int smr_raff_create(smr_ctx *ctx, int natoms_per_sys, int nsys, smr_r
     r->ctx = ctx;
     r->n = natoms_per_sys;
     r->nsys = nsys;
-    r->seg = seg_for(natoms_per_sys);
+    r->seg = raff_seg_for(natoms_per_sys);
     const size_t n = (size_t)natoms_per_sys, na = n * (size_t)nsys;
     hipError_t e = r->radius.alloc(sizeof(float) * n);
     if (e == hipSuccess) e = r->neighs.alloc(sizeof(int4) * n);
@@ -303,13 +301,13 @@ extern "C" int smr_raff_upload_state(smr_raff *r, const float *pos3, const float
 
 extern "C" int smr_raff_step(smr_raff *r, const smr_raff_step_config *cfg, int nsteps) {
     if (!r) return fail(SMR_ERR_INVALID, "smr_raff_step: NULL argument");
-    int rc = check_cfg(cfg, "smr_raff_step");
-    if (!rc) rc = ready(r, "smr_raff_step");
+    int rc = raff_check_cfg(cfg, "smr_raff_step");
+    if (!rc) rc = raff_ready(r, "smr_raff_step");
     if (rc) return rc;
     if (nsteps <= 0) return fail(SMR_ERR_INVALID, "smr_raff_step: nsteps must be > 0");
     RCHK(hipSetDevice(r->ctx->device));
     hipStream_t st = r->ctx->stream;
-    RaffArgs A = args_of(r, cfg, nsteps);
+    RaffArgs A = raff_args(r, cfg, nsteps);
     RCHK(launch_raff(A, r->seg, st));
     RCHK(hipStreamSynchronize(st));
     return SMR_OK;
@@ -318,15 +316,15 @@ extern "C" int smr_raff_step(smr_raff *r, const smr_raff_step_config *cfg, int n
 extern "C" int smr_raff_relax(smr_raff *r, const smr_raff_step_config *cfg, int max_steps, float tol, int *steps,
                               int *status, float *metric) {
     if (!r) return fail(SMR_ERR_INVALID, "smr_raff_relax: NULL argument");
-    int rc = check_cfg(cfg, "smr_raff_relax");
-    if (!rc) rc = ready(r, "smr_raff_relax");
+    int rc = raff_check_cfg(cfg, "smr_raff_relax");
+    if (!rc) rc = raff_ready(r, "smr_raff_relax");
     if (rc) return rc;
     if (max_steps <= 0 || !(tol > 0.f)) return fail(SMR_ERR_INVALID, "smr_raff_relax: need max_steps > 0, tol > 0");
     RCHK(hipSetDevice(r->ctx->device));
     hipStream_t st = r->ctx->stream;
     RCHK(zero(r->steps, st));
     RCHK(zero(r->status, st));   // SMR_RAFF_RUNNING
-    RaffArgs A = args_of(r, cfg, max_steps);
+    RaffArgs A = raff_args(r, cfg, max_steps);
     A.relax = 1;
     A.max_steps = max_steps;
     A.tol = tol;

@@ -8,6 +8,8 @@ the C ABI of include/surfmol_raff.h and the host set-up of surfmol_host.h.
 * ``RaffReplicas``        -- oclff::RRsp3Multi (crates/libs/oclff/src/
                               rrsp3.rs:699-1048): shared topology, per-replica
                               state on the device, step / relax / downloads.
+* ``RaffServer``          -- continuous-batching relax server for replicas of
+                              one molecule.
 """
 from __future__ import annotations
 
@@ -198,6 +200,61 @@ class RaffReplicas:
     def close(self):
         if self._h:
             lib().smr_raff_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RaffServer:
+    """smr_raff_server_*: submit replicas while others relax; each retires at
+    convergence (tol) or at max_steps."""
+
+    def __init__(self, template: RaffReplicas, capacity=1024, chunk_steps=64, max_steps=5000, tol=1e-4,
+                 cfg: StepConfig | None = None):
+        c = _lib.SmrRaffServerConfig()
+        c.capacity, c.chunk_steps, c.max_steps, c.tol = int(capacity), int(chunk_steps), int(max_steps), float(tol)
+        c.step = (cfg or StepConfig()).cstruct()
+        h = ctypes.c_void_p()
+        check(lib().smr_raff_server_create(template._h, ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+        self.n = template.n
+
+    def submit(self, pos3, quat=None) -> int:
+        p = np.ascontiguousarray(pos3, dtype=np.float32).reshape(self.n, 3)
+        q = None if quat is None else np.ascontiguousarray(quat, dtype=np.float32).reshape(self.n, 4)
+        rid = ctypes.c_longlong()
+        check(lib().smr_raff_server_submit(self._h, _fp(p), None if q is None else _fp(q), ctypes.byref(rid)))
+        return rid.value
+
+    def poll(self, rid: int):
+        """None while the request runs; else a dict (forgets the request)."""
+        done, steps, status = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        met = ctypes.c_float()
+        x = np.zeros((self.n, 3), np.float32)
+        ts, td = ctypes.c_double(), ctypes.c_double()
+        check(lib().smr_raff_server_poll(self._h, ctypes.c_longlong(rid), ctypes.byref(done), ctypes.byref(steps),
+                                         ctypes.byref(status), ctypes.byref(met), _fp(x), ctypes.byref(ts),
+                                         ctypes.byref(td)))
+        if not done.value:
+            return None
+        return {"steps": steps.value, "status": status.value, "metric": met.value, "positions": x,
+                "t_submit": ts.value, "t_done": td.value, "latency": td.value - ts.value}
+
+    def drain(self):
+        check(lib().smr_raff_server_drain(self._h))
+
+    def stats(self):
+        o = (ctypes.c_double * 7)()
+        check(lib().smr_raff_server_stats(self._h, o))
+        return dict(zip(["submitted", "retired", "resident", "queued", "chunks", "peak_resident", "device_s"], list(o)))
+
+    def close(self):
+        if self._h:
+            lib().smr_raff_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -165,3 +165,39 @@ def test_errors(ctx):
     with pytest.raises(SmrError, match="neighs"):
         r.upload_neighs_and_exclusions(np.full((8, 4), 9), -np.ones((8, 4)), -np.ones((8, 4)))
     r.close()
+
+
+def test_server_continuous_batching(ctx, oracle):
+    """200 requests arrive while others relax (64 slots, 32-step chunks):
+    every result equals the one-shot relaxation of the same replica bit for
+    bit (positions, steps, status, metric), whatever it shared the device
+    with; the server queued requests and ran many chunks."""
+    import time
+    from surfmol_amd.raff import RaffReplicas, RaffServer, StepConfig
+    m, X, dev, orc = _setup(ctx, oracle, 200)
+    cfg = StepConfig(dt=0.1, relaxation=0.5)
+    tol = 2e-4
+    steps, status, metric = dev.relax(cfg, 3000, tol)
+    Xb = dev.positions()
+    tmpl = RaffReplicas.from_molecule(ctx, m, X[:1], 1)
+    srv = RaffServer(tmpl, capacity=64, chunk_steps=32, max_steps=3000, tol=tol, cfg=cfg)
+    rng = np.random.default_rng(3)
+    ids = []
+    for s in range(200):
+        ids.append(srv.submit(X[s]))
+        if s % 10 == 0:
+            time.sleep(float(rng.uniform(0, 0.002)))
+    srv.drain()
+    st = srv.stats()
+    assert st["retired"] == 200 and st["resident"] == 0 and st["peak_resident"] <= 64 and st["chunks"] > 5
+    for s, rid in enumerate(ids):
+        r = srv.poll(rid)
+        assert r is not None
+        assert r["steps"] == steps[s] and r["status"] == status[s] and r["metric"] == metric[s], s
+        assert np.array_equal(r["positions"][:m.nreal], Xb[s, :m.nreal]), s
+        assert r["latency"] >= 0
+    with pytest.raises(Exception):
+        srv.poll(ids[0])                      # forgotten after the first poll
+    srv.close()
+    tmpl.close()
+    dev.close()
