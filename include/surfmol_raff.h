@@ -7,7 +7,7 @@
  * (the directions to their bonded neighbours at the reference geometry), the
  * neighbours are pulled onto the port tips, and optional sphere collisions
  * push non-bonded atoms apart.  This is the drop-in for oclff::RRsp3Multi
- * (crates/libs/oclff/src/rrsp3.rs:699-1048, kernels opencl/RRsp3.cl:1774-
+ * (crates/libs/oclff/src/rrsp3.rs:699-1088, kernels opencl/RRsp3.cl:1774-
  * 2065): every entry point below names the method it replaces, takes the
  * same arrays in the same layout (float32, as the reference uploads them)
  * and runs the same step.  All replicas share one topology; the per-replica
@@ -45,7 +45,7 @@ typedef struct {
     float rot_eps, theta_max, damp;
 } smr_raff_step_config;
 
-/* RRsp3Multi::new(natoms_per_sys, group_size, nsys, _) (rrsp3.rs:746-812).
+/* RRsp3Multi::new(natoms_per_sys, group_size, nsys, _) (rrsp3.rs:742-839).
  * natoms_per_sys counts the replica's atoms as packed (nodes first, then
  * caps, then padding atoms of inverse mass 0); at most SMR_RAFF_MAX_ATOMS.
  * The reference's group_size is a launch detail here (chosen by the
@@ -54,7 +54,7 @@ typedef struct {
 int smr_raff_create(smr_ctx *ctx, int natoms_per_sys, int nsys, smr_raff **out);
 int smr_raff_destroy(smr_raff *r);
 
-/* Shared topology uploads (rrsp3.rs:829-889), one row per atom of one
+/* Shared topology uploads (rrsp3.rs:858-934), one row per atom of one
  * replica: radius [n] (>= 0; all zero disables collisions), neighs / excl1 /
  * excl2 / bk_slots [4n] (-1 padded, local indices; bk_slots index node ports
  * inode*4 + k), fixmask [n] (bits 1,2,4 freeze x,y,z and the node's
@@ -68,14 +68,14 @@ int smr_raff_upload_cluster_ports(smr_raff *r, const float *port_local_atoms, co
                                   int nnode_per_group);
 int smr_raff_upload_bk_slots(smr_raff *r, const int *bk_slots);
 
-/* RRsp3Multi::upload_state_multi (rrsp3.rs:899-929): positions
+/* RRsp3Multi::upload_state_multi (rrsp3.rs:936-974): positions
  * pos3 [nsys*n*3], inverse masses [n] (shared; padding atoms 0, their
  * positions become NaN as in the reference) and orientations [nsys*n*4]
  * (x,y,z,w; NULL = identity).  Zeroes both heavy-ball momenta. */
 int smr_raff_upload_state(smr_raff *r, const float *pos3, const float *inv_mass, const float *quat);
 
 /* nsteps consecutive calls of RRsp3Multi::step_cluster_multi with the
- * Current port kernel (rrsp3.rs:1000-1026), as one persistent launch. */
+ * Current port kernel (rrsp3.rs:1028-1050), as one persistent launch. */
 int smr_raff_step(smr_raff *r, const smr_raff_step_config *cfg, int nsteps);
 
 /* Relaxation with per-replica retirement (no counterpart in the reference,
@@ -88,7 +88,7 @@ int smr_raff_step(smr_raff *r, const smr_raff_step_config *cfg, int nsteps);
 int smr_raff_relax(smr_raff *r, const smr_raff_step_config *cfg, int max_steps, float tol, int *steps, int *status,
                    float *metric);
 
-/* Downloads (rrsp3.rs:934-994,1029-1056): positions of all replicas
+/* Downloads (rrsp3.rs:976-1025,1051-1083): positions of all replicas
  * [nsys*n*3] or of one [n*3]; orientations of one replica [n*4]; the last
  * step's collision corrections [n*3] and node translations [nnode*3]; the
  * largest |component| of each replica's last step corrections [nsys]. */

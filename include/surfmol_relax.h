@@ -179,7 +179,7 @@ typedef struct {
 /* Create a batch of nsys systems.  Systems reference one of ntopo
  * topologies (sys_topo[s] in [0,ntopo)), so replicas share device tables
  * (the per-replica/shared split of oclff::RRsp3Multi,
- * crates/libs/oclff/src/rrsp3.rs:742-830).  apos holds the concatenated
+ * crates/libs/oclff/src/rrsp3.rs:742-839).  apos holds the concatenated
  * initial positions [sum_s natoms(s), 3].  grid may be NULL when
  * opts->surface == 0.  Replaces the per-call buffer set-up of
  * UffOcl::eval_bonds (crates/libs/oclff/src/uff.rs:30-58). */

@@ -2,7 +2,7 @@
 // cluster step) on gfx950.
 //
 // Reference: oclff::RRsp3Multi::step_cluster_multi (crates/libs/oclff/src/
-// rrsp3.rs:1000-1026) with the kernels compute_collision_multi,
+// rrsp3.rs:1028-1050) with the kernels compute_collision_multi,
 // compute_ports_current_multi and apply_corrections_multi
 // (opencl/RRsp3.cl:1774-2047).  One reference step is three launches over a
 // (group, replica) grid, exchanging every correction through global memory.

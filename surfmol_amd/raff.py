@@ -6,7 +6,7 @@ the C ABI of include/surfmol_raff.h and the host set-up of surfmol_host.h.
                               (crates/libs/oclff/src/pack.rs), in the
                               library's C++ host code.
 * ``RaffReplicas``        -- oclff::RRsp3Multi (crates/libs/oclff/src/
-                              rrsp3.rs:699-1048): shared topology, per-replica
+                              rrsp3.rs:699-1088): shared topology, per-replica
                               state on the device, step / relax / downloads.
 * ``RaffServer``          -- continuous-batching relax server for replicas of
                               one molecule.
