@@ -95,7 +95,8 @@ def workload_layer(args, S, W):
     mol, X = W.layer_lattice(16, 32, spacing=sp, seed=W.SEED_BASE + 3)
     desc = (f"coupled layer: 512 PTCDA-like molecules (19,456 atoms) on a 16x32 lattice at {sp:g} A (0/90 degree "
             "checkerboard orientations, +-2 degree jitter, height U(3,4) A); UFF + intra-molecular LJ/Coulomb + "
-            "intermolecular LJ/Coulomb within 12 A (device cell list, energy switched off over 10-12 A) + GridFF B-spline on the 6x6x20 A rock-salt "
+            "intermolecular LJ/Coulomb within 12 A (energy switched off over 10-12 A; Verlet lists with a 1 A skin in "
+            "one cooperative launch, or the device cell list of the step launches) + GridFF B-spline on the 6x6x20 A rock-salt "
             "cell at 0.1 A (60x60x200 nodes, images to +-60 A); one FIRE state, dt0 0.1 fs, dt_max 1 fs, "
             f"<= {args.max_steps} steps")
     return [mol], np.zeros(len(X), dtype=np.int32), X, 6.0, desc
@@ -276,6 +277,8 @@ def run_workload(args, S, W, workload, steps, warmup, cpu_seconds, want_plain, w
     fconv = 1e-3
     if args.no_graph:
         batch.set_graph(False)
+    if layer and args.layer_step_launches:
+        batch.set_layer_persistent(False)
     tuned = None
     if args.autotune or workload == "uneven":
         t0 = time.time()
@@ -386,6 +389,27 @@ def run_workload(args, S, W, workload, steps, warmup, cpu_seconds, want_plain, w
         # so the step rate of the coupled layer is reported beside `value`
         out["layer_fire_steps_per_s"] = round(float(steps_arr[0]) * steps / T, 1)
         out["atom_steps_per_s"] = round(float(steps_arr[0]) * steps / T * len(X), 1)
+        path = batch.layer_path()
+        cfg["layer_path"] = ("one cooperative launch (Verlet lists, grid barriers)" if path["persistent"] else
+                             "step launches (force launch, decision, cell list)" +
+                             (f"; one launch not taken: {path['fallback']}" if path["fallback"] else ""))
+        cfg["us_per_fire_step"] = round(1e6 * T / (float(steps_arr[0]) * steps), 2)
+        cfg["energy_eV_per_atom"] = round(float(sum(r["energy"] for r in results)) / len(X), 9)
+        if path["persistent"]:
+            cfg["neighbour_list_builds"] = path["rebuilds"]
+            # the other path on the same layer, one relaxation
+            batch.set_layer_persistent(False)
+            batch.reset()
+            t0 = time.perf_counter()
+            r_s = batch.relax_batched(fire, max_steps, fconv)
+            t_s = time.perf_counter() - t0
+            batch.set_layer_persistent(True)
+            ok_s = all(r["status"] == S.SMR_SYS_CONVERGED for r in r_s)
+            out["step_launch_path"] = {"value": round((1.0 if ok_s else 0.0) / t_s, 4), "unit": "layers/s",
+                                       "fire_steps": int(r_s[0]["steps"]),
+                                       "us_per_fire_step": round(1e6 * t_s / float(r_s[0]["steps"]), 2),
+                                       "energy_eV_per_atom": round(float(sum(r["energy"] for r in r_s)) / len(X), 9),
+                                       "fmax_final": round(float(r_s[0]["fmax"]), 6)}
     if want_plain and not layer:
         pts = []
         for s in range(min(args.plain_sample, nsys)):
@@ -529,6 +553,9 @@ def main():
     ap.add_argument("--chunk-steps", type=int, default=64, help="serve: FIRE steps per persistent launch (<= 64 atoms)")
     ap.add_argument("--nsys", type=int, default=None)
     ap.add_argument("--max-steps", type=int, default=None, help="FIRE step cap (2000; 20000 for the layer)")
+    ap.add_argument("--layer-step-launches", action="store_true",
+                    help="layer: relax by step launches (force launch, decision, cell list) instead of one "
+                         "cooperative launch")
     ap.add_argument("--layer-spacing", type=float, default=13.0,
                     help="layer: square lattice spacing [A]; at 11 A the molecule has no flat arrangement with "
                          "contacts above 2.05 A and the layer does not relax (FP64 oracle and device agree)")
@@ -590,6 +617,8 @@ def main():
         extras["layer"] = {"value": la["value"], "unit": la["unit"], "ms_per_relaxation": la["ms_per_step"],
                            "fire_steps": la["config"]["fire_steps_max"],
                            "layer_fire_steps_per_s": la["layer_fire_steps_per_s"],
+                           "us_per_fire_step": la["config"]["us_per_fire_step"], "path": la["config"]["layer_path"],
+                           "step_launch_path": la.get("step_launch_path"),
                            "roofline_frac": la["roofline"]["frac"], "workload": la["config"]["workload"],
                            "wall_s": round(time.time() - t0, 1)}
         print(json.dumps({"progress": "layer", **extras["layer"]}), file=sys.stderr, flush=True)

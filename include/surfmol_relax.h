@@ -297,6 +297,26 @@ int smr_batch_autotune(smr_batch *b, const smr_fire_params *fire, int trial_step
  * Results are the same either way.  No counterpart in the reference. */
 int smr_batch_set_graph(smr_batch *b, int use_graph);
 
+/* Coupled layers: 1 (default) run the whole relaxation as one cooperative
+ * launch when the layer fits the chip at once (one 64-thread workgroup per
+ * molecule of <= 64 atoms; per step the workgroups exchange positions,
+ * velocities, forces and FIRE sums through device-coherent memory between
+ * grid barriers, and take intermolecular partners from Verlet lists of
+ * radius cutoff + skin_A, rebuilt when an atom may have moved skin_A / 2);
+ * 0 use the step launches (force launch, decision, cell list).  Results
+ * agree to FP32 summation order.  A layer that does not fit, or whose
+ * neighbourhoods overflow the lists, runs the step launches.  skin_A <= 0
+ * keeps the current skin (default 1 A).  No counterpart in the reference. */
+int smr_batch_set_layer_persistent(smr_batch *b, int on, double skin_A);
+
+/* Which path the last coupled-layer relaxation took: *persistent 1 for the
+ * one-launch path; *fallback the reason it was not (0 none or switched off,
+ * 1 a grid barrier timed out, 2/3/4 neighbour instances / staged atoms /
+ * list entries overflowed, 5 the layer does not fit the chip at once, 6 its
+ * molecules need another launch shape); *rebuilds the neighbour-list builds.
+ * Each pointer may be NULL.  No counterpart in the reference. */
+int smr_batch_layer_path(const smr_batch *b, int *persistent, int *fallback, int *rebuilds);
+
 /* Order in which smr_relax_batched starts the systems of a size class:
  * 1 (default) larger systems first and, among systems of one size, highest
  * FP64 energy per atom of the starting positions first

@@ -444,6 +444,26 @@ class Batch:
     def set_graph(self, use_graph: bool):
         check(lib().smr_batch_set_graph(self._h, 1 if use_graph else 0))
 
+    LAYER_FALLBACK = {0: None, 1: "grid barrier timed out", 2: "neighbour instances overflowed",
+                      3: "staged neighbour atoms overflowed", 4: "neighbour list overflowed",
+                      5: "the layer does not fit the chip at once", 6: "molecules need another launch shape"}
+
+    def set_layer_persistent(self, on: bool = True, skin: float = 0.0):
+        """Coupled layers: run the relaxation as one cooperative launch where
+        the layer fits the chip (default), with Verlet lists of radius
+        cutoff + skin (A; 0 keeps the current skin, 1 A at first); False uses
+        the step launches.  Results agree to FP32 summation order."""
+        check(lib().smr_batch_set_layer_persistent(self._h, 1 if on else 0, float(skin)))
+
+    def layer_path(self) -> dict:
+        """Which path the last coupled-layer relaxation took: persistent
+        (one launch), the reason it fell back to the step launches, and the
+        number of neighbour-list builds."""
+        p, f, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().smr_batch_layer_path(self._h, ctypes.byref(p), ctypes.byref(f), ctypes.byref(r)))
+        return {"persistent": bool(p.value), "fallback": self.LAYER_FALLBACK.get(f.value, f.value),
+                "rebuilds": r.value}
+
     def close(self):
         if self._h:
             lib().smr_batch_destroy(self._h)

@@ -120,6 +120,8 @@ _SIGNATURES = [
     ("smr_batch_plan", c_int, [c_void_p, P(c_int), P(c_int)]),
     ("smr_batch_autotune", c_int, [c_void_p, P(SmrFireParams), c_int]),
     ("smr_batch_set_graph", c_int, [c_void_p, c_int]),
+    ("smr_batch_set_layer_persistent", c_int, [c_void_p, c_int, c_double]),
+    ("smr_batch_layer_path", c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),
     ("smr_batch_set_order", c_int, [c_void_p, c_int]),
     ("smr_server_create", c_int, [c_void_p, c_void_p, P(SmrServerConfig), P(c_void_p)]),
     ("smr_server_submit", c_int, [c_void_p, P(SmrTopology), P(c_double), P(ctypes.c_longlong)]),

@@ -1118,9 +1118,127 @@ FireConst make_fire_const(const smr_fire_params *f, int max_steps, double fconv)
 
 namespace smr {
 
-// FIRE on a coupled layer: steps of [force launches (per size class) ->
-// FIRE decision -> move + cell-list rebuild] captured S at a time into a HIP graph; the host
-// replays the graph until the shared FIRE state leaves RUNNING.
+// FP64 energies of the final positions and the layer's shared step count,
+// status and Fmax in every molecule's result.
+int layer_results(smr_batch *b, const LayerState &hs, smr_relax_result *res) {
+    int rc = eval_energies64(b);
+    if (rc) return rc;
+    rc = read_results(b, res);
+    if (rc) return rc;
+    const int status = hs.status == SMR_SYS_RUNNING ? SMR_SYS_MAXSTEPS : hs.status;
+    for (int s = 0; s < b->nsys; s++) {
+        res[s].steps = hs.steps;
+        res[s].status = status;
+        res[s].fmax = hs.fmax;
+    }
+    return SMR_OK;
+}
+
+// The coupled layer as one cooperative launch (KMODE_LAYER_P of k_relax,
+// layer_persist.h), from the state layer_relax has set up.  *done = false
+// when the layer does not take this path: switched off, molecules of
+// another launch shape, more molecules than the chip holds at once, or a
+// neighbourhood that overflows the staging area or the lists (the launch
+// then stops without writing anything back, so the step launches start from
+// the same state).  b->last_layer_fallback says which.
+constexpr int kLpMaxStaged = 768;   // neighbour atoms staged per molecule
+static_assert(kLpMaxStaged <= 1024 && kLpMaxInst <= 64, "a list entry packs a staged atom in 10 bits, its instance in 6");
+constexpr int kLpMaxList = 256;     // list entries per atom at most (fewer when LDS is short)
+constexpr int kLpMinList = 48;      // ... and at least
+constexpr size_t kLdsPerCu = 160 * 1024;
+
+int layer_relax_persistent(smr_batch *b, const std::vector<KernelArgs> &args, bool &done) {
+    done = false;
+    b->last_layer_persistent = 0;
+    b->last_layer_fallback = 0;
+    b->last_layer_rebuilds = 0;
+    if (!b->layer_persistent) return SMR_OK;
+    smr_ctx *c = b->ctx;
+    hipStream_t st = c->stream;
+    if (b->plans.size() != 1 || b->plans[0].shape.block != 64 || b->plans[0].shape.apt != 1 ||
+        !b->plans[0].shape.fint_lds || b->plans[0].count != b->nsys) {
+        b->last_layer_fallback = 6;
+        return SMR_OK;
+    }
+    const ClassPlan &p = b->plans[0];
+    const size_t off = (p.shape.lds_bytes + 15) & ~(size_t)15;
+    // the lists take what LDS is left when ceil(nsys / CUs) molecules share a CU
+    const int lds_atoms = args[0].lds_atoms;
+    const int need_per_cu = (b->nsys + c->num_cu - 1) / c->num_cu;
+    const size_t cap = std::min(c->max_lds, kLdsPerCu / (size_t)need_per_cu);
+    const size_t fixed = off + lp_lds_bytes(kLpMaxStaged, lds_atoms, 0);
+    const long long room = (long long)cap - (long long)fixed - 16;
+    const int maxl = room > 0 ? (int)std::min<long long>(kLpMaxList, room / (2LL * lds_atoms)) : 0;
+    if (maxl < kLpMinList) {
+        b->last_layer_fallback = 5;
+        return SMR_OK;
+    }
+    const size_t lds = off + lp_lds_bytes(kLpMaxStaged, lds_atoms, maxl);
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, relax_layer_persistent_ptr(), 64, lds);
+    if (e != hipSuccess) return hip_fail(e, "occupancy of the one-launch layer kernel");
+    if (lds > c->max_lds || (long long)per_cu * c->num_cu < b->nsys) {
+        b->last_layer_fallback = 5;
+        return SMR_OK;
+    }
+    const size_t na = b->total_atoms, ns = b->nsys;
+    const size_t nbar = kLpBarWords;
+    if (b->lp_xbuf.bytes < sizeof(float4) * 6 * na && (e = b->lp_xbuf.alloc(sizeof(float4) * 6 * na)) != hipSuccess)
+        return hip_fail(e, "layer exchange buffer");
+    if (b->lp_xrec.bytes < sizeof(double) * 2 * kLpRec * ns &&
+        (e = b->lp_xrec.alloc(sizeof(double) * 2 * kLpRec * ns)) != hipSuccess)
+        return hip_fail(e, "layer record buffer");
+    if (b->lp_bar.bytes < sizeof(unsigned) * nbar && (e = b->lp_bar.alloc(sizeof(unsigned) * nbar)) != hipSuccess)
+        return hip_fail(e, "layer barrier words");
+    KernelArgs A = args[0];
+    A.mode = KMODE_LAYER_P;
+    A.lp.xbuf = (float4 *)b->lp_xbuf.ptr;
+    A.lp.xrec = (double *)b->lp_xrec.ptr;
+    A.lp.bar = (unsigned *)b->lp_bar.ptr;
+    A.lp.maxl = maxl;
+    A.lp.maxs = kLpMaxStaged;
+    A.lp.lds_off = (int)off;
+    A.lp.natoms = b->total_atoms;
+    A.lp.skin = b->layer_skin;
+    // barrier counters and the error word start at zero (a kernel, not a memset)
+    if ((e = launch_zero_i32((int *)b->lp_bar.ptr, (int)nbar, st)) != hipSuccess) return hip_fail(e, "layer barrier reset");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    if (e == hipSuccess) e = launch_relax_layer_persistent(A, b->nsys, lds, st);
+    if (e == hipErrorCooperativeLaunchTooLarge) {
+        (void)hipGetLastError();
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        b->last_layer_fallback = 5;
+        return SMR_OK;
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&b->last_kernel_ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(e, "one-launch layer relaxation");
+    unsigned err = 0;
+    HIPCHK(hipMemcpy(&err, (unsigned *)b->lp_bar.ptr + (kLpGroups + 1) * 32, sizeof(err), hipMemcpyDeviceToHost));
+    if (err != 0u) {
+        b->last_layer_fallback = (int)err;
+        return SMR_OK;
+    }
+    LayerState hs;
+    HIPCHK(hipMemcpy(&hs, b->layer_state.ptr, sizeof(hs), hipMemcpyDeviceToHost));
+    b->last_layer_rebuilds = hs.rebuilds;
+    b->last_layer_persistent = 1;
+    done = true;
+    return SMR_OK;
+}
+
+// FIRE on a coupled layer: one cooperative launch where the layer fits
+// (layer_relax_persistent), else steps of [force launches (per size class)
+// -> FIRE decision -> move + cell-list rebuild] captured S at a time into a
+// HIP graph, which the host replays until the shared FIRE state leaves
+// RUNNING.
 int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double fconv, smr_relax_result *res) {
     smr_ctx *c = b->ctx;
     hipStream_t st = c->stream;
@@ -1162,6 +1280,24 @@ int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double
     std::vector<KernelArgs> args;
     rc = prepare_plans(b->ls, b->plans, A, args);
     if (rc) return rc;
+    {
+        KernelArgs P = A;
+        P.steps_per_launch = max_steps;
+        std::vector<KernelArgs> pargs;
+        rc = prepare_plans(b->ls, b->plans, P, pargs);
+        if (rc) return rc;
+        bool one_launch = false;
+        rc = layer_relax_persistent(b, pargs, one_launch);
+        if (rc) return rc;
+        if (one_launch) {
+            LayerState hs;
+            HIPCHK(hipMemcpy(&hs, b->layer_state.ptr, sizeof(hs), hipMemcpyDeviceToHost));
+            // the cell list of the final positions, for the FP64 energies
+            rc = layer_cells(b);
+            if (rc) return rc;
+            return layer_results(b, hs, res);
+        }
+    }
     LayerArgs LA = layer_args(b);
     LA.partials = A.partials;
     LA.fire = A.fire;
@@ -1216,17 +1352,7 @@ int layer_relax(smr_batch *b, const smr_fire_params *fire, int max_steps, double
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return hip_fail(e, "layer relaxation");
-    rc = eval_energies64(b);
-    if (rc) return rc;
-    rc = read_results(b, res);
-    if (rc) return rc;
-    const int status = hs.status == SMR_SYS_RUNNING ? SMR_SYS_MAXSTEPS : hs.status;
-    for (int s = 0; s < b->nsys; s++) {
-        res[s].steps = hs.steps;
-        res[s].status = status;
-        res[s].fmax = hs.fmax;
-    }
-    return SMR_OK;
+    return layer_results(b, hs, res);
 }
 
 }  // namespace smr
@@ -1563,6 +1689,24 @@ extern "C" int smr_batch_set_order(smr_batch *b, int energy_order) {
         HIPCHK(hipSetDevice(b->ctx->device));
         HIPCHK(hipMemcpy(b->work.ptr, b->work_host.data(), sizeof(int) * b->work_host.size(), hipMemcpyHostToDevice));
     }
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_set_layer_persistent(smr_batch *b, int on, double skin_A) {
+    if (!b) return fail(SMR_ERR_INVALID, "smr_batch_set_layer_persistent: NULL argument");
+    if (!(skin_A <= 0.0) && !(skin_A < 10.0))
+        return fail(SMR_ERR_INVALID, "smr_batch_set_layer_persistent: skin_A = " + std::to_string(skin_A) +
+                                         " (need < 10 A; <= 0 keeps the current skin)");
+    b->layer_persistent = on != 0;
+    if (skin_A > 0.0) b->layer_skin = (float)skin_A;
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_layer_path(const smr_batch *b, int *persistent, int *fallback, int *rebuilds) {
+    if (!b) return fail(SMR_ERR_INVALID, "smr_batch_layer_path: NULL argument");
+    if (persistent) *persistent = b->last_layer_persistent;
+    if (fallback) *fallback = b->last_layer_fallback;
+    if (rebuilds) *rebuilds = b->last_layer_rebuilds;
     return SMR_OK;
 }
 

@@ -17,6 +17,7 @@
 // Per-term math is in smr_device.h and shared by both paths.
 #include "smr_device.h"
 #include "smr_kernels.h"
+#include "layer_persist.h"
 
 namespace smr {
 
@@ -99,8 +100,12 @@ __device__ __forceinline__ void refresh_stencils(const DevTables &T, bool need, 
 // spill).  LAYER: the launch carries a coupled layer's cell list (its
 // intermolecular partial forces and, in KMODE_LAYER, the shared FIRE
 // state); the code for it is compiled into these instances only.
-template <int BLOCK, int APT, bool FINT_LDS, bool LAYER>
-__global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A) {
+// PERSIST: a coupled layer in one cooperative launch (KMODE_LAYER_P, one
+// workgroup per molecule for all steps; layer_persist.h).  Its workgroups
+// are one per molecule and at most a few per CU, so it is built for one wave
+// per SIMD (all registers).
+template <int BLOCK, int APT, bool FINT_LDS, bool LAYER, bool PERSIST = false>
+__global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax(KernelArgs A) {
     constexpr int NW = BLOCK / 64;
     // one atom per thread: each atom's surface stencil lives in registers
     constexpr bool RSTEN = (APT == 1);
@@ -137,13 +142,13 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
     float *fint_x, *fint_y, *fint_z;
 
     // a coupled layer that has converged (or stopped) does no more work
-    if (LAYER && A.mode == KMODE_LAYER && A.layer->status != SMR_SYS_RUNNING) return;
+    if (LAYER && (A.mode == KMODE_LAYER || A.mode == KMODE_LAYER_P) && A.layer->status != SMR_SYS_RUNNING) return;
     // a launch of one step (coupled-layer steps, force evaluations) with a
     // workgroup for every system: each takes its own, without a contended
     // counter; otherwise systems are fetched through the atomic counter in
     // the order the workgroups start (concurrent launches share the chip,
     // so not every workgroup starts at once)
-    const bool one_each = A.steps_per_launch == 1 && A.nwork <= (int)gridDim.x;
+    const bool one_each = PERSIST || (A.steps_per_launch == 1 && A.nwork <= (int)gridDim.x);
     int pass = 0;
     while (true) {
         // ---- fetch the next system from the work list ----
@@ -163,7 +168,8 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
         const TopoDesc td = T.topo[sd.topo];
         SysState st = A.state[s];
         // a finished system is not stepped again (forces are always evaluated)
-        if (A.mode != KMODE_EVAL && st.status != SMR_SYS_RUNNING) continue;
+        // (a persistent layer's workgroups all take part in every step)
+        if (!PERSIST && A.mode != KMODE_EVAL && st.status != SMR_SYS_RUNNING) continue;
         const int N = td.natoms;
         const int abase = sd.atom_base;
         if (FINT_LDS) {
@@ -260,7 +266,238 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
         const int nsteps = A.steps_per_launch;
         double fm2_cur = st.fmax * st.fmax;   // max |F|^2 of the last step
         int step_in_launch = 0;
+
+        // ---- coupled layer in one cooperative launch (KMODE_LAYER_P) ----
+        // The shared FIRE state, the staging area of the neighbour atoms and
+        // the own atom's Verlet list (length lcnt, built at pref) persist
+        // over the launch's steps.
+        [[maybe_unused]] lp::Fire fs{};
+        [[maybe_unused]] int lcnt = 0, kstep = 0, nrebuild = 0;
+        [[maybe_unused]] unsigned gen = 0;
+        [[maybe_unused]] bool lp_failed = false;
+        [[maybe_unused]] float3 pref = make_float3(0.f, 0.f, 0.f);
+        [[maybe_unused]] float4 msph = make_float4(0.f, 0.f, 0.f, 0.f);
+        [[maybe_unused]] float lp_rc = 0.f;
+        if constexpr (PERSIST) lp_rc = sqrtf(A.cells.rc2);
+        const LayerPersist &LP = A.lp;
+        const lp::Stage stg = lp::stage_of(smem, LP, A.lds_atoms);
+        const unsigned nat_all = (unsigned)LP.natoms;
+        const unsigned nmol = gridDim.x;
+        const auto rx = lp::rsrc(LP.xbuf, 96u * nat_all);
+        const auto rrec = lp::rsrc(LP.xrec, 2u * nmol * kLpRec * 8u);
+        // bounding sphere of the own atoms (origin-grid frame)
+        auto own_sphere = [&]() -> float4 {
+            const bool own = tid < N;
+            float3 g = make_float3(0.f, 0.f, 0.f);
+            if (own) {
+                const float4 p = s_pos[tid];
+                g = make_float3(p.x + (float)sd.gx * T.ostep_x, p.y + (float)sd.gy * T.ostep_y, p.z + (float)sd.gz * T.ostep_z);
+            }
+            float lo[3] = {own ? g.x : INFINITY, own ? g.y : INFINITY, own ? g.z : INFINITY};
+            float hi[3] = {own ? g.x : -INFINITY, own ? g.y : -INFINITY, own ? g.z : -INFINITY};
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    lo[c] = fminf(lo[c], __shfl_xor(lo[c], o, 64));
+                    hi[c] = fmaxf(hi[c], __shfl_xor(hi[c], o, 64));
+                }
+            const float3 cc = make_float3(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]));
+            const float3 d = g - cc;
+            float r2 = own ? dot3(d, d) : 0.f;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) r2 = fmaxf(r2, __shfl_xor(r2, o, 64));
+            return make_float4(cc.x, cc.y, cc.z, sqrtf(r2));
+        };
+        // publish this step's own atoms (position before the move, velocity
+        // after v += F dt, force) and the molecule's record
+        auto publish = [&](int par, float3 v, float3 f, double r0, double r1, double r2, double r3, double r4, double r5,
+                           float4 sph) {
+            if (tid < N) {
+                const unsigned a = (unsigned)(abase + tid);
+                lp::stc4(rx, ((par * 3u + 0u) * nat_all + a) * 16u, s_pos[tid]);
+                lp::stc4(rx, ((par * 3u + 1u) * nat_all + a) * 16u, make_float4(v.x, v.y, v.z, 0.f));
+                lp::stc4(rx, ((par * 3u + 2u) * nat_all + a) * 16u, make_float4(f.x, f.y, f.z, 0.f));
+            }
+            if (tid < 4) {
+                const float4 w = tid == 0 ? lp::d2f4(r0, r1) : tid == 1 ? lp::d2f4(r2, r3) : tid == 2 ? lp::d2f4(r4, r5) : sph;
+                lp::stc4(rrec, (((unsigned)par * nmol + blockIdx.x) * kLpRec) * 8u + (unsigned)tid * 16u, w);
+            }
+        };
+        // staged neighbour atoms moved under this step's decision (the same
+        // operations as their own workgroup's move)
+        auto restage = [&](int par) {
+            const int total = stg.misc[1];
+            for (int j0 = 0; j0 < total; j0 += 4 * 64) {
+                float4 P4[4], V4[4], F4[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int j = j0 + u * 64 + lane;
+                    if (j < total) {
+                        const unsigned g = (unsigned)stg.gid[j];
+                        P4[u] = lp::ldc4(rx, ((par * 3u + 0u) * nat_all + g) * 16u);
+                        V4[u] = lp::ldc4(rx, ((par * 3u + 1u) * nat_all + g) * 16u);
+                        F4[u] = lp::ldc4(rx, ((par * 3u + 2u) * nat_all + g) * 16u);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int j = j0 + u * 64 + lane;
+                    if (j < total) {
+                        const float3 v = lp::mixed_velocity(f3(V4[u]), f3(F4[u]), fs.a, fs.b, fs.reset);
+                        const float3 p = lp::moved(f3(P4[u]), v, fs.dt_move);
+                        stg.pos[j] = make_float4(p.x, p.y, p.z, stg.pos[j].w);
+                    }
+                }
+            }
+            block_sync<NW>();
+        };
+        // neighbour instances (molecule, box image) within reach of the own
+        // sphere, their atoms staged, and the own atom's list of staged atoms
+        // within cutoff + skin; delta bounds this step's move of any atom
+        auto build = [&](int par, float delta) -> bool {
+            const CellGrid &C = A.cells;
+            const float reach = lp_rc + LP.skin + 2.f * delta + 0.05f;
+            int ninst = 0;
+            for (int q0 = 0; q0 < (int)nmol; q0 += 64) {
+                const int q = q0 + lane;
+                unsigned mask = 0u;
+                float4 ct = make_float4(0.f, 0.f, 0.f, 0.f);
+                int n0x = 0, n0y = 0;
+                if (q < (int)nmol) {
+                    ct = lp::ldc4(rrec, (((unsigned)par * nmol + q) * kLpRec) * 8u + 48u);
+                    if (C.pbc_x) n0x = (int)rintf((msph.x - ct.x) * C.inv_Lx);
+                    if (C.pbc_y) n0y = (int)rintf((msph.y - ct.y) * C.inv_Ly);
+                    const float lim = msph.w + ct.w + reach;
+#pragma unroll
+                    for (int ix = 0; ix < 3; ix++)
+#pragma unroll
+                        for (int iy = 0; iy < 3; iy++) {
+                            if ((!C.pbc_x && ix != 1) || (!C.pbc_y && iy != 1)) continue;
+                            const int bx = C.pbc_x ? n0x + ix - 1 : 0, by = C.pbc_y ? n0y + iy - 1 : 0;
+                            if (q == (int)blockIdx.x && bx == 0 && by == 0) continue;   // the molecule itself
+                            const float dx = ct.x + (float)bx * C.Lx - msph.x, dy = ct.y + (float)by * C.Ly - msph.y,
+                                        dz = ct.z - msph.z;
+                            if (dx * dx + dy * dy + dz * dz < lim * lim) mask |= 1u << (ix * 3 + iy);
+                        }
+                }
+                const int c = __builtin_popcount(mask);
+                const int off = lp::wave_excl_scan(c, lane);
+                const int tot = lp::wave_sum_i(c);
+                if (ninst + tot > kLpMaxInst) {
+                    if (lane == 0) lp::set_error(LP, LP_ERR_INST);
+                    return false;
+                }
+                int k = ninst + off;
+                while (mask) {
+                    const int bit = __builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    const int ix = bit / 3, iy = bit % 3;
+                    const int bx = C.pbc_x ? n0x + ix - 1 : 0, by = C.pbc_y ? n0y + iy - 1 : 0;
+                    const int t = A.work_list[q];
+                    const SysDesc dt_ = T.sys[t];
+                    const float3 dg = make_float3((float)((dt_.gx - sd.gx) + bx * C.nLx) * T.ostep_x,
+                                                  (float)((dt_.gy - sd.gy) + by * C.nLy) * T.ostep_y,
+                                                  (float)(dt_.gz - sd.gz) * T.ostep_z);
+                    stg.off[k] = make_float4(dg.x, dg.y, dg.z, 0.f);
+                    stg.sph[k] = make_float4(ct.x + (float)bx * C.Lx - (float)sd.gx * T.ostep_x,
+                                             ct.y + (float)by * C.Ly - (float)sd.gy * T.ostep_y,
+                                             ct.z - (float)sd.gz * T.ostep_z, ct.w);
+                    stg.nat[k] = T.topo[dt_.topo].natoms;
+                    stg.mol[k] = t;
+                    k++;
+                }
+                ninst += tot;
+            }
+            block_sync<NW>();
+            // first staged index of each instance
+            const int na_k = lane < ninst ? stg.nat[lane] : 0;
+            const int first = lp::wave_excl_scan(na_k, lane);
+            const int total = lp::wave_sum_i(na_k);
+            if (total > LP.maxs) {
+                if (lane == 0) lp::set_error(LP, LP_ERR_STAGE);
+                return false;
+            }
+            if (lane < ninst) stg.off[lane].w = __int_as_float(first);
+            if (lane == 0) {
+                stg.misc[0] = ninst;
+                stg.misc[1] = total;
+            }
+            block_sync<NW>();
+            for (int k = 0; k < ninst; k++) {
+                const int t = stg.mol[k], n = stg.nat[k], f0 = __float_as_int(stg.off[k].w);
+                const SysDesc dt_ = T.sys[t];
+                const int aoff = T.topo[dt_.topo].atom_off;
+                for (int a = lane; a < n; a += 64) {
+                    stg.gid[f0 + a] = dt_.atom_base + a;
+                    stg.req[f0 + a] = T.req[aoff + a];
+                    stg.pos[f0 + a] = make_float4(0.f, 0.f, 0.f, __int_as_float(k));
+                }
+            }
+            block_sync<NW>();
+            restage(par);
+            // the own atom's list
+            int cnt = 0;
+            if (tid < N) {
+                const float3 pi = f3(s_pos[tid]);
+                const float rl = lp_rc + LP.skin;
+                const float rl2 = rl * rl;
+                unsigned short *nl = stg.nl + tid;
+                for (int k = 0; k < ninst; k++) {
+                    const float4 sp = stg.sph[k];
+                    const float3 d = pi - f3(sp);
+                    const float lim = sp.w + rl + delta + 0.05f;
+                    if (dot3(d, d) >= lim * lim) continue;
+                    const float4 o = stg.off[k];
+                    const int f0 = __float_as_int(o.w), n = stg.nat[k];
+                    for (int a = 0; a < n; a++) {
+                        const float3 dp = (f3(stg.pos[f0 + a]) - pi) + f3(o);
+                        if (dot3(dp, dp) < rl2) {
+                            if (cnt < LP.maxl) nl[cnt * A.lds_atoms] = (unsigned short)((f0 + a) | (k << 10));
+                            cnt++;
+                        }
+                    }
+                }
+                pref = pi;
+            }
+            lcnt = cnt;
+            const bool over = __builtin_amdgcn_ballot_w64(cnt > LP.maxl) != 0ull;
+            if (over) {
+                if (lane == 0) lp::set_error(LP, LP_ERR_LIST);
+                return false;
+            }
+            // chunks of kLpChunk entries: the wave's lanes take them in turn
+            const int nch = (cnt + kLpChunk - 1) / kLpChunk;
+            const int co = lp::wave_excl_scan(nch, lane);
+            stg.choff[lane] = co;
+            stg.lcnt[lane] = cnt;
+            for (int c = 0; c < nch; c++) stg.iatom[co + c] = (unsigned char)lane;
+            const int nitems = lp::wave_sum_i(nch);
+            if (lane == 0) stg.misc[2] = nitems;
+            block_sync<NW>();
+            nrebuild++;
+            return true;
+        };
+        if constexpr (PERSIST) {
+            // the launch's first exchange: positions as they are (a reset
+            // decision moves nothing), lists built from them
+            const LayerState L0 = *A.layer;
+            fs.dt = L0.dt;
+            fs.alpha = L0.alpha;
+            fs.fmax = L0.fmax;
+            fs.npos = L0.npos;
+            fs.steps = L0.steps;
+            fs.status = L0.status;
+            fs.a = fs.b = 0.f;
+            fs.reset = 1;
+            fs.dt_move = 0.f;
+            msph = own_sphere();
+            const float3 z3 = make_float3(0.f, 0.f, 0.f);
+            publish(1, z3, z3, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, msph);
+            lp_failed = !lp::barrier(LP, ++gen, lane) || !build(1, 0.f);
+        }
         for (;;) {
+            if (PERSIST && lp_failed) break;
             // parameters of each thread's first angle, dihedral and inversion
             // (read through the caches), issued before the bonds so that their
             // latency overlaps phase A; each term loop prefetches the next
@@ -513,6 +750,61 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                     if (own) fe_r = grid_eval_stencil([&](int e) { return rs[e]; }, gc);
                 }
             }
+            if constexpr (PERSIST) {
+                // coupled layer, one launch: the list entries of the own
+                // atoms in chunks of kLpChunk, lane by lane (the wave's lanes
+                // stay busy whatever the molecule's size); each chunk's
+                // partial force goes to LDS.  Branch-free: a block's four
+                // entries are read first (the chunk's last entry stands in
+                // past its end, masked), then their rows, then the pairs; an
+                // entry holds the staged atom (bits 0-9) and its instance
+                // (bits 10-15).  The switch factor is exactly 1 below its
+                // start (x clamped to 0); pairs beyond the cutoff (the skin)
+                // contribute nothing.
+                const CellGrid &C = A.cells;
+                const int nitems = stg.misc[2];
+                for (int t = lane; t < nitems; t += 64) {
+                    const int a = stg.iatom[t];
+                    const int e0 = (t - stg.choff[a]) * kLpChunk;
+                    const int n = min(kLpChunk, stg.lcnt[a] - e0);
+                    const float3 pa = f3(s_pos[a]);
+                    const float4 ra = s_req[a];
+                    const unsigned short *nl = stg.nl + a;
+                    KahanF3 pk;
+                    pk.init(make_float3(0.f, 0.f, 0.f));
+                    for (int b0 = 0; b0 < n; b0 += 4) {
+                        int ent[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) ent[u] = nl[(e0 + min(b0 + u, n - 1)) * A.lds_atoms];
+                        float4 pj[4], rj[4], dg[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            pj[u] = stg.pos[ent[u] & 1023];
+                            rj[u] = stg.req[ent[u] & 1023];
+                            dg[u] = stg.off[ent[u] >> 10];
+                        }
+                        float3 bs = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const float3 dp = (f3(pj[u]) - pa) + f3(dg[u]);
+                            const float r2 = dot3(dp, dp);
+                            float3 f;
+                            const float e = ljq_pair(dp, ra, rj[u], T.r2damp, f);
+                            const float ir = frsq(r2), r = r2 * ir, x = fmaxf((r - C.ron) * C.inv_w, 0.f);
+                            const float S = 1.f - x * x * (3.f - 2.f * x), dS_r = 6.f * x * (x - 1.f) * C.inv_w * ir;
+                            f = f * S + dp * (e * dS_r);
+                            const bool use = b0 + u < n && r2 < C.rc2;
+                            bs.x += use ? f.x : 0.f;
+                            bs.y += use ? f.y : 0.f;
+                            bs.z += use ? f.z : 0.f;
+                        }
+                        pk.add(bs);
+                    }
+                    const float3 v = pk.value();
+                    stg.ipart[t] = make_float4(v.x, v.y, v.z, 0.f);
+                }
+                block_sync<NW>();
+            }
             float3 F[APT];
 #pragma unroll
             for (int k = 0; k < APT; k++) {
@@ -564,7 +856,14 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                             acc.add(f);
                         }
                     }
-                    if (LAYER && A.cells.spos) {
+                    if constexpr (PERSIST) {
+                        // coupled layer, one launch: the partners of the
+                        // atom's list (staged neighbour atoms), summed by
+                        // chunks above
+                        // its chunks' partial forces, in chunk order
+                        const int c0 = stg.choff[i], nch = (lcnt + kLpChunk - 1) / kLpChunk;
+                        for (int c = 0; c < nch; c++) acc.add(f3(stg.ipart[c0 + c]));
+                    } else if (LAYER && A.cells.spos) {
                         // coupled layer: intermolecular forces of the
                         // neighbour cells (k_layer_inter), in cell order
                         const CellGrid &C = A.cells;
@@ -639,8 +938,8 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
             }
             // per-atom partial sums (FP64)
             double p_vf = 0.0, p_vv = 0.0, p_ff = 0.0, p_fm = 0.0;
-            if (A.mode == KMODE_FIRE || A.mode == KMODE_LAYER) {
-                const float dtf = (float)(LAYER && A.mode == KMODE_LAYER ? A.layer->dt : st.dt);
+            if (A.mode == KMODE_FIRE || A.mode == KMODE_LAYER || A.mode == KMODE_LAYER_P) {
+                const float dtf = (float)(PERSIST ? fs.dt : (LAYER && A.mode == KMODE_LAYER ? A.layer->dt : st.dt));
                 const double flim = A.fire.flim;
 #pragma unroll
                 for (int k = 0; k < APT; k++) {
@@ -682,6 +981,100 @@ __global__ __launch_bounds__(BLOCK, APT == 1 ? 2 : 1) void k_relax(KernelArgs A)
                         my_v[k] = my_v[k] * cd + fc * dtf;
                     }
                 }
+            }
+            if constexpr (PERSIST) {
+                // ---- coupled layer, one launch: end of the step ----
+                const bool own = tid < N;
+                const float3 pcur = own ? f3(s_pos[tid]) : make_float3(0.f, 0.f, 0.f);
+                const float3 dpr = pcur - pref;
+                const double vm2 = own ? (double)dot3(my_v[0], my_v[0]) : 0.0;
+                const double dm2 = own ? (double)dot3(dpr, dpr) : 0.0;
+                msph = own_sphere();
+                const double r0 = wave_sum(p_vf), r1 = wave_sum(p_vv), r2 = wave_sum(p_ff), r3 = wave_max(p_fm),
+                             r4 = wave_max(vm2), r5 = wave_max(dm2);
+                const int par = kstep & 1;
+                publish(par, my_v[0], F[0], r0, r1, r2, r3, r4, r5, msph);
+                if (!lp::barrier(LP, ++gen, lane)) {
+                    lp_failed = true;
+                    break;
+                }
+                // every workgroup: the records of all molecules in the same
+                // order (k_layer_decide's), the same decision
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0;
+                for (int q0 = lane; q0 < (int)nmol; q0 += 8 * 64) {
+                    float4 x0[8], x1[8], x2[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int q = q0 + 64 * u;
+                        if (q < (int)nmol) {
+                            const unsigned ro = (((unsigned)par * nmol + q) * kLpRec) * 8u;
+                            x0[u] = lp::ldc4(rrec, ro);
+                            x1[u] = lp::ldc4(rrec, ro + 16u);
+                            x2[u] = lp::ldc4(rrec, ro + 32u);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        if (q0 + 64 * u < (int)nmol) {
+                            const double2 d01 = lp::f42d(x0[u]), d23 = lp::f42d(x1[u]), d45 = lp::f42d(x2[u]);
+                            a0 += d01.x;
+                            a1 += d01.y;
+                            a2 += d23.x;
+                            a3 = fmax(a3, d23.y);
+                            a4 = fmax(a4, d45.x);
+                            a5 = fmax(a5, d45.y);
+                        }
+                    }
+                }
+                const double vf = wave_sum(a0), vv = wave_sum(a1), ff = wave_sum(a2), fm = wave_max(a3),
+                             vm = wave_max(a4), dm = wave_max(a5);
+                lp::decide(fs, A.fire, vf, vv, ff, fm);
+                // the move of the own atoms
+                if (own) {
+                    const float3 v2 = lp::mixed_velocity(my_v[0], F[0], fs.a, fs.b, fs.reset);
+                    const float3 p2 = lp::moved(pcur, v2, fs.dt_move);
+                    my_v[0] = v2;
+                    s_pos[tid] = make_float4(p2.x, p2.y, p2.z, s_pos[tid].w);
+                }
+                kstep++;
+                step_in_launch++;
+                if (fs.status != SMR_SYS_RUNNING || step_in_launch >= nsteps) {
+                    if (own) {
+                        persist_stencil();
+                        if (A.stencil_key) A.stencil_key[abase + tid] = skey[0];
+                        A.pos[abase + tid] = s_pos[tid];
+                        A.vel[abase + tid] = make_float4(my_v[0].x, my_v[0].y, my_v[0].z, 0.f);
+                    }
+                    if (blockIdx.x == 0 && tid == 0) {
+                        LayerState S = *A.layer;
+                        S.dt = fs.dt;
+                        S.alpha = fs.alpha;
+                        S.npos = fs.npos;
+                        S.steps = fs.steps;
+                        S.status = fs.status;
+                        S.fmax = fs.fmax;
+                        S.pending_move = 0;
+                        S.cells_dirty = 1;
+                        S.rebuilds += nrebuild;
+                        *A.layer = S;
+                    }
+                    break;
+                }
+                // the lists hold while no atom has moved more than skin/2
+                // since they were built; this step moves an atom by at most
+                // dt (a |v| + b |F|)
+                const float delta =
+                    fs.reset ? 0.f : fs.dt_move * (fs.a * (float)sqrt(vm) + fs.b * (float)sqrt(fm));
+                if ((float)sqrt(dm) + delta > 0.5f * LP.skin) {
+                    if (!build(par, delta)) {
+                        lp_failed = true;
+                        break;
+                    }
+                } else {
+                    restage(par);
+                }
+                block_sync<NW>();
+                continue;
             }
             // block reduction of (vf, vv, ff, fmax^2)
             if (LAYER && A.mode == KMODE_LAYER) {
@@ -1001,6 +1394,14 @@ static hipError_t launch_one(const KernelArgs &A, int grid, size_t lds, hipStrea
         hipLaunchKernelGGL((k_relax<BLOCK, APT, FL, false>), dim3(grid), dim3(BLOCK), lds, stream, A);
     return hipGetLastError();
 }
+
+hipError_t launch_relax_layer_persistent(const KernelArgs &A, int grid, size_t lds, hipStream_t stream) {
+    void *args[] = {const_cast<KernelArgs *>(&A)};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_relax<64, 1, true, true, true>), dim3(grid),
+                                      dim3(64), args, (unsigned)lds, stream);
+}
+
+const void *relax_layer_persistent_ptr() { return reinterpret_cast<const void *>(&k_relax<64, 1, true, true, true>); }
 
 // (occupancy of a shape: the layer instances use the same launch bounds)
 template <int BLOCK, int APT, bool FL>

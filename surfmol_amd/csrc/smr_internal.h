@@ -184,6 +184,11 @@ struct smr_batch {
     smr::DevBuf cell_part, slot_of;
     smr::DevBuf partials, layer_state;
     smr::CellGrid cells;
+    // coupled layer in one cooperative launch (allocated on first use)
+    smr::DevBuf lp_xbuf, lp_xrec, lp_bar;
+    int layer_persistent = 1;
+    float layer_skin = 1.f;
+    int last_layer_persistent = 0, last_layer_fallback = 0, last_layer_rebuilds = 0;
     smr::DevBuf plain_hneigh, plain_fint, plain_e, plain_cvf;
     std::vector<smr::ClassPlan> class_plans;   // one per size class (shapes chosen per class)
     std::vector<smr::ClassPlan> plans;         // launches: adjacent classes of one shape merged

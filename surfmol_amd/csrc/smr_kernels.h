@@ -8,7 +8,7 @@
 
 namespace smr {
 
-enum { KMODE_FIRE = 0, KMODE_EVAL = 1, KMODE_MD = 2, KMODE_LAYER = 3 };
+enum { KMODE_FIRE = 0, KMODE_EVAL = 1, KMODE_MD = 2, KMODE_LAYER = 3, KMODE_LAYER_P = 4 };
 
 // Cell list of a coupled layer (atoms sorted by cell, by atom index inside a
 // cell), rebuilt every step.  Cells are at least as large as the cutoff, so
@@ -48,7 +48,40 @@ struct LayerState {
     int reset;
     int pending_move;        // the decision of this step still has to be applied
     int cells_dirty;         // an atom changed cell in this step's move: the cell list is rebuilt
+    int rebuilds;            // neighbour-list builds of the one-launch path (a statistic)
 };
+
+// A coupled layer in one cooperative launch (KMODE_LAYER_P): every molecule
+// keeps one workgroup for all steps of the launch; the workgroups exchange
+// per step, through device-coherent (sc1) stores and loads, each atom's
+// position, velocity and force and each molecule's FIRE record, separated by
+// a grid barrier; intermolecular partners come from per-atom Verlet lists
+// over neighbour atoms staged in LDS.
+constexpr int kLpMaxInst = 64;     // neighbour (molecule, box image) instances per molecule
+constexpr int kLpGroups = 8;       // arrival counters of the grid barrier (128 B apart)
+constexpr int kLpBarWords = (kLpGroups + 2) * 32;   // counters, top counter, error word
+constexpr int kLpRec = 8;          // doubles per molecule record
+enum { LP_OK = 0, LP_ERR_BARRIER = 1, LP_ERR_INST = 2, LP_ERR_STAGE = 3, LP_ERR_LIST = 4 };
+struct LayerPersist {
+    float4 *xbuf;            // [2][3][natoms] position (before the move), velocity (after v += F dt), force; by step parity
+    double *xrec;            // [2][nsys][kLpRec] v.F, v.v, F.F, max F^2, max v^2, max displacement^2, sphere (4 floats)
+    unsigned *bar;           // barrier words: 8 group counters (128 B apart), the top counter, the error word
+    int maxl, maxs;          // list entries per atom (lists in LDS, entry-major), staged atoms per molecule
+    int lds_off;             // byte offset of the staging area in dynamic LDS
+    int natoms;
+    float skin;              // list radius = cutoff + skin; lists are rebuilt when an atom may have moved skin/2
+};
+// the lists are evaluated in chunks of kLpChunk entries spread over the wave
+constexpr int kLpChunk = 16;
+__host__ __device__ inline int lp_chunks(int maxl) { return (maxl + kLpChunk - 1) / kLpChunk; }
+// dynamic LDS of the staging area and the lists (lds_atoms atoms of maxl
+// entries, their chunk partial sums and chunk owners)
+inline size_t lp_lds_bytes(int maxs, int lds_atoms, int maxl) {
+    const size_t items = (size_t)lds_atoms * lp_chunks(maxl);
+    const size_t lists = ((size_t)lds_atoms * maxl * 2 + 15) & ~(size_t)15;
+    return (size_t)maxs * (16 + 16 + 4) + (size_t)kLpMaxInst * (16 + 16 + 4 + 4) + 16 + 64 * 8 + items * 16 +
+           ((items + 15) & ~(size_t)15) + lists;
+}
 
 struct KernelArgs {
     DevTables tab;
@@ -72,6 +105,7 @@ struct KernelArgs {
     CellGrid cells;          // coupled layer: intermolecular partners (spos == NULL: none)
     LayerState *layer;       // coupled layer: shared FIRE state (KMODE_LAYER)
     double4 *partials;       // coupled layer: per-system (v.F, v.v, F.F, max F^2)
+    LayerPersist lp;         // coupled layer in one cooperative launch (KMODE_LAYER_P)
 };
 
 struct LaunchShape {
@@ -122,6 +156,11 @@ struct PlainArgs {
 };
 
 hipError_t launch_relax(const KernelArgs &A, const LaunchShape &L, hipStream_t stream);
+// the coupled layer in one cooperative launch (64 threads per molecule, one
+// atom per thread); hipErrorCooperativeLaunchTooLarge when the molecules do
+// not fit the chip at once
+hipError_t launch_relax_layer_persistent(const KernelArgs &A, int grid, size_t lds, hipStream_t stream);
+const void *relax_layer_persistent_ptr();
 const void *relax_kernel_ptr(int block, int apt, bool fint_lds);
 hipError_t launch_zero_i32(int *p, int n, hipStream_t st);
 // device-to-device copies described by a list (serving loop)

@@ -8,6 +8,12 @@ the cutoff could fall on the other side), energies within 1e-6 eV/atom; the
 relaxed layer is an FP64 minimum; repeated relaxations are bit-identical.
 Both the reference's hard cutoff and the smooth switch (energy times
 S = 1 - 3x^2 + 2x^3 over the last w below the cutoff) are checked.
+
+Relaxations run on both of the layer's paths: one cooperative launch
+(Verlet lists, grid barriers; the default where the layer fits the chip) and
+the step launches (force launch, decision, cell list).  Along a trajectory
+both follow the FP64 oracle's FIRE steps (positions within 5e-5 A after 1,
+7 and 37 steps).
 """
 import numpy as np
 import pytest
@@ -95,13 +101,21 @@ def test_layer_without_neighbours_is_independent(ctx, surf, oracle, params):
     assert np.abs(El - Ei).max() < 1e-12
 
 
-def test_layer_relax_is_fp64_minimum(ctx, surf, oracle, params):
+POS_ATOL = 5e-5
+
+
+@pytest.mark.parametrize("persistent", [True, False])
+def test_layer_relax_is_fp64_minimum(ctx, surf, oracle, params, persistent):
     import surfmol_amd as S
     # the smooth switch: with a hard cutoff, pairs crossing it make the
     # forces jump by ~4e-3 eV/A and FIRE may stall above Fmax = 1e-3
     mol, X, b, osys = _layer(ctx, surf, oracle, params, 3, 3, 12.0, w=2.0)
+    b.set_layer_persistent(persistent)
     fp = fire_params_internal()
     res = b.relax_batched(S.FireParams(**fp), 3000, 1e-3)
+    path = b.layer_path()
+    print("path", path)
+    assert path["persistent"] == persistent and path["fallback"] is None
     assert all(r["status"] == S.SMR_SYS_CONVERGED for r in res), res[0]
     assert len({r["steps"] for r in res}) == 1          # one FIRE state
     Xf = b.positions()
@@ -117,10 +131,19 @@ def test_layer_relax_is_fp64_minimum(ctx, surf, oracle, params):
 
 def test_layer_relax_is_reproducible(ctx, surf, oracle, params):
     """Repeated relaxations, replayed from the captured graph or issued
-    directly, are bit-identical."""
+    directly, are bit-identical; so are repeated one-launch relaxations."""
     import surfmol_amd as S
     mol, X, b, osys = _layer(ctx, surf, oracle, params, 4, 4, 12.0, w=2.0)
     fp = S.FireParams(**fire_params_internal())
+    p1 = b.relax_batched(fp, 600, 1e-3)
+    xp = b.positions()
+    assert b.layer_path()["persistent"]
+    b.reset()
+    p2 = b.relax_batched(fp, 600, 1e-3)
+    assert np.array_equal(b.positions(), xp)
+    assert [r["energy"] for r in p1] == [r["energy"] for r in p2] and p1[0]["steps"] == p2[0]["steps"]
+    b.set_layer_persistent(False)
+    b.reset()
     r1 = b.relax_batched(fp, 600, 1e-3)
     x1 = b.positions()
     for graph in (True, False, True):
@@ -174,11 +197,14 @@ def test_pbc_layer_eval_forces(ctx, surf, oracle, params, rc, w, shift):
     assert np.abs(Fo - F).max() > 1e-3
 
 
-def test_pbc_layer_relax_is_fp64_minimum(ctx, surf, oracle, params):
+@pytest.mark.parametrize("persistent", [True, False])
+def test_pbc_layer_relax_is_fp64_minimum(ctx, surf, oracle, params, persistent):
     import surfmol_amd as S
     mol, X, box, b, osys = _pbc_layer(ctx, surf, oracle, params, 3, 3, 16.0, 12.0, shift=(-4.0, 3.0), w=2.0)
+    b.set_layer_persistent(persistent)
     fp = fire_params_internal()
     res = b.relax_batched(S.FireParams(**fp), 4000, 1e-3)
+    assert b.layer_path()["persistent"] == persistent
     assert all(r["status"] == S.SMR_SYS_CONVERGED for r in res), res[0]
     Xf = b.positions()
     fo, eo = oracle.eval_forces(osys, Xf)
@@ -201,3 +227,68 @@ def test_pbc_box_checks(ctx, surf):
                     ((48.0, 48.0), 0.0)):      # a box without a coupled layer
         with pytest.raises(S.SmrError):
             S.Batch(*args, grid=surf["grid"], rdamp=RDAMP, inter_cutoff=rc, pbc=box)
+
+
+# ---------------------------------------------------------------------------
+# the two paths of a layer relaxation along a trajectory
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("persistent", [True, False])
+@pytest.mark.parametrize("k", [1, 7, 37])
+def test_layer_trajectory_matches_oracle(ctx, surf, oracle, params, persistent, k):
+    """k FIRE steps of the layer (one shared state; the n_min transition at
+    step 6) from the same FP32 start as the oracle's FP64 FIRE."""
+    import surfmol_amd as S
+    mol, X, b, osys = _layer(ctx, surf, oracle, params, 4, 3, 12.0, w=2.0)
+    b.set_layer_persistent(persistent)
+    fp = fire_params_internal()
+    x0 = b.positions()
+    res = b.relax_batched(S.FireParams(**fp), k, 1e-9)
+    assert b.layer_path()["persistent"] == persistent
+    assert res[0]["steps"] == k
+    r = oracle.relax_fire(osys, oracle.FireParams(**fp), k, 1e-9, apos=x0)
+    d = np.abs(b.positions() - r["apos"]).max()
+    print("persistent", persistent, "k", k, "max position deviation", d)
+    assert d < POS_ATOL
+
+
+@pytest.mark.parametrize("skin", [0.05, 2.0])
+def test_one_launch_list_rebuilds(ctx, surf, oracle, params, skin):
+    """A thin skin rebuilds the Verlet lists on many steps, a thick one on
+    few: both reach the FP64 minimum of the periodic layer."""
+    import surfmol_amd as S
+    mol, X, box, b, osys = _pbc_layer(ctx, surf, oracle, params, 3, 3, 16.0, 12.0, shift=(-4.0, 3.0), w=2.0)
+    b.set_layer_persistent(True, skin)
+    fp = fire_params_internal()
+    res = b.relax_batched(S.FireParams(**fp), 4000, 1e-3)
+    path = b.layer_path()
+    print("skin", skin, "steps", res[0]["steps"], path)
+    assert path["persistent"] and all(r["status"] == S.SMR_SYS_CONVERGED for r in res)
+    if skin < 0.1:
+        assert path["rebuilds"] > 20
+    Xf = b.positions()
+    fo, eo = oracle.eval_forces(osys, Xf)
+    Etot = sum(r["energy"] for r in res)
+    n = len(Xf)
+    assert abs(eo.sum() - Etot) / n < 1e-6
+    rf = oracle.relax_fire(osys, oracle.FireParams(**fp), 4000, 1e-3, apos=Xf)
+    assert rf["converged"] and abs(rf["E"] - Etot) / n < 1e-6, (rf["E"], Etot)
+
+
+def test_one_launch_overflow_falls_back(ctx, surf, oracle, params):
+    """A 9 A skin stages more neighbour atoms than a molecule's staging area
+    holds: the launch stops without writing anything back and the step
+    launches relax the layer from the same start (the same minimum)."""
+    import surfmol_amd as S
+    mol, X, box, b, osys = _pbc_layer(ctx, surf, oracle, params, 4, 4, 12.0, 12.0, w=2.0)
+    b.set_layer_persistent(True, 9.0)
+    fp = fire_params_internal()
+    res = b.relax_batched(S.FireParams(**fp), 200, 1e-3)
+    path = b.layer_path()
+    print(path)
+    assert not path["persistent"] and path["fallback"] is not None and "overflow" in path["fallback"]
+    x1 = b.positions()
+    b.set_layer_persistent(False)
+    b.reset()
+    res2 = b.relax_batched(S.FireParams(**fp), 200, 1e-3)
+    assert np.array_equal(b.positions(), x1)
+    assert [r["energy"] for r in res] == [r["energy"] for r in res2]
