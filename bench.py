@@ -397,6 +397,7 @@ def run_workload(args, S, W, workload, steps, warmup, cpu_seconds, want_plain, w
         cfg["energy_eV_per_atom"] = round(float(sum(r["energy"] for r in results)) / len(X), 9)
         if path["persistent"]:
             cfg["neighbour_list_builds"] = path["rebuilds"]
+        if path["persistent"] and not args.no_extras:
             # the other path on the same layer, one relaxation
             batch.set_layer_persistent(False)
             batch.reset()
