@@ -117,12 +117,13 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
     const int lane = tid & 63, wid = tid >> 6;
     const DevTables &T = A.tab;
 
-    // LDS carve-up (sizes fixed per launch by the host)
-    float4 *s_pos = reinterpret_cast<float4 *>(smem);
-    float4 *s_req = s_pos + A.lds_atoms;
-    float4 *s_h = s_req + A.lds_atoms;                       // [4*natoms] hneigh
-    float4 *s_fp = s_h + 4 * A.lds_atoms;                    // [NWP][natoms] pair reaction forces
-    const int NWP = A.pair_mode ? NW : 1;                    // one reaction buffer per wave
+    // LDS carve-up (sizes fixed per launch by the host, relax_lds_bytes)
+    const int rows = relax_lds_rows(BLOCK, APT, A.lds_atoms);
+    float4 *s_pos = reinterpret_cast<float4 *>(smem);        // [rows] (ROT: rows 0..N/2 again after row N-1)
+    float4 *s_req = s_pos + rows;                            // [rows]
+    float4 *s_h = s_req + rows;                              // [4*natoms] hneigh
+    float4 *s_fp = s_h + 4 * A.lds_atoms;                    // [NWP][natoms] pair reaction forces (not ROT)
+    const int NWP = ROT ? 0 : (A.pair_mode ? NW : 1);        // one reaction buffer per wave
     double *s_red = reinterpret_cast<double *>(s_fp + NWP * A.lds_atoms);  // [NW*8]
     int *s_next = reinterpret_cast<int *>(s_red + NW * 8);
     // interaction slots and piece destinations: staged in LDS, except for
@@ -179,6 +180,13 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
         }
         fint_y = fint_x + A.nf_max;
         fint_z = fint_y + A.nf_max;
+        // ROT with pieces in LDS: the gather masks by weight (0 or 1), so
+        // every float it may read must be finite: the slots this system does
+        // not write ([nf, nf_max) of each array) and the tail are zeroed
+        if constexpr (ROT && FINT_LDS) {
+            for (int q = td.nf + tid; q < A.nf_max; q += BLOCK) fint_x[q] = fint_y[q] = fint_z[q] = 0.f;
+            if (tid < kPieceTail) fint_z[A.nf_max + tid] = 0.f;
+        }
 
         // ---- stage the system ----
         int4 my_ng[APT];
@@ -205,6 +213,43 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                 my_a2f[k] = make_int2(0, 0);
                 my_v[k] = make_float3(0.f, 0.f, 0.f);
             }
+        }
+        // ROT: rows 0..N/2 again after row N-1 (partner of sweep d = row i + d)
+        if (ROT && tid < N && tid <= (N >> 1)) {
+            s_pos[N + tid] = s_pos[tid];
+            s_req[N + tid] = s_req[tid];
+        }
+        // ROT: the gather of each step follows a plan fixed per system.  An
+        // atom's force pieces are summed in rounds of eight; the wave runs
+        // as many rounds as its heaviest atom needs.  Lanes without an atom
+        // take the tail of the heaviest atoms' ranges (ranked by piece count,
+        // then index): the atom sums its first 8 ceil(r/2) pieces (r = its
+        // rounds), the helper lane the rest, and the atom adds the helper's
+        // sum after its own.  (g_start, g_cnt): the lane's range; g_help: the
+        // helper lane of the lane's atom, or -1.
+        [[maybe_unused]] int g_start = 0, g_cnt = 0, g_help = -1;
+        if constexpr (ROT) {
+            const bool own = tid < N;
+            const int c = own ? my_a2f[0].y : 0;
+            const int r = (c + 7) >> 3;
+            int rank = 0;
+            for (int j = 0; j < N; j++) {
+                const int cj = __shfl(c, j, 64);
+                rank += (cj > c || (cj == c && j < tid)) ? 1 : 0;
+            }
+            const int nhelp = 64 - N;
+            const bool helped = own && r >= 2 && rank < nhelp;
+            const int a = helped ? 8 * ((r + 1) >> 1) : c;
+            g_help = helped ? N + rank : -1;
+            int hatom = -1;
+            for (int k = 0; k < nhelp; k++) {
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(helped && rank == k);
+                if (m != 0ull && lane == N + k) hatom = __builtin_ctzll(m);
+            }
+            const int src = hatom >= 0 ? hatom : lane;
+            const int s_x = __shfl(my_a2f[0].x, src, 64), s_a = __shfl(a, src, 64), s_c = __shfl(c, src, 64);
+            g_start = own ? my_a2f[0].x : (hatom >= 0 ? s_x + s_a : 0);
+            g_cnt = own ? a : (hatom >= 0 ? s_c - s_a : 0);
         }
         // interaction slots and piece destinations stay in LDS for all steps
         if constexpr (TABLES_LDS) {
@@ -617,35 +662,50 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                 }
                 KahanF3 pacc;
                 pacc.init(make_float3(0.f, 0.f, 0.f));
-                // byte offset of the partner's row in s_pos / s_req, wrapping at N
-                const unsigned jend = 16u * (unsigned)N;
-                unsigned ja = own ? 16u * (unsigned)tid : 0u;
-                ja = (ja + 16u == jend) ? 0u : ja + 16u;
-                const char *pb = reinterpret_cast<const char *>(s_pos);
-                const char *rb = reinterpret_cast<const char *>(s_req);
-                float4 pj = *reinterpret_cast<const float4 *>(pb + ja), rj = *reinterpret_cast<const float4 *>(rb + ja);
-                // (the prefetch of the next row is unconditional: the offset
-                // wraps inside the N rows, so the last sweep's read is in bounds)
-                // The own-atom forces of four sweeps are added in FP32 and
-                // their sum goes into the compensated sum.
+                // partner of sweep d: row (own ? tid : 0) + d, d <= N/2 (the
+                // rows after N-1 repeat rows 0..N/2: no wrap)
+                const float4 *prow = s_pos + (own ? tid : 0);
+                const float4 *rrow = s_req + (own ? tid : 0);
+                // one sweep: the pair's radial factor (zero where skipped)
+                // scales dp into the own sum and, negated, into the rotating
+                // reaction sum
+                auto pair = [&](int d, float4 pc, float4 rc, float3 &bs) {
+                    const float3 dp = f3(pc) - f3(pi4);
+                    float fr = ljq_pair_fr(dp, ri, rc, T.r2damp);
+                    if ((skip >> (d - 1)) & 1u) fr = 0.f;
+                    bs.x = __builtin_fmaf(dp.x, fr, bs.x);
+                    bs.y = __builtin_fmaf(dp.y, fr, bs.y);
+                    bs.z = __builtin_fmaf(dp.z, fr, bs.z);
+                    rr.x = __builtin_fmaf(-dp.x, fr, rr.x);
+                    rr.y = __builtin_fmaf(-dp.y, fr, rr.y);
+                    rr.z = __builtin_fmaf(-dp.z, fr, rr.z);
+                };
+                // Blocks of four sweeps: the four partner rows are read first,
+                // then the pairs; each block's own-atom sum goes into the
+                // compensated sum.  The last 1-3 sweeps form a block of their own.
                 auto sweep = [&](auto rotate) {
-                    for (int d0 = 1; d0 <= nd; d0 += 4) {
+                    int d0 = 1;
+                    for (; d0 + 3 <= nd; d0 += 4) {
+                        float4 pc[4], rc[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            pc[u] = prow[d0 + u];
+                            rc[u] = rrow[d0 + u];
+                        }
                         float3 bs = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
-                            const int d = d0 + u;
-                            if (d > nd) break;
-                            const float4 pc = pj, rc = rj;
-                            ja = (ja + 16u == jend) ? 0u : ja + 16u;
-                            pj = *reinterpret_cast<const float4 *>(pb + ja);
-                            rj = *reinterpret_cast<const float4 *>(rb + ja);
-                            if (d > 1) rotate();
-                            const float3 dp = f3(pc) - f3(pi4);
-                            float fr = ljq_pair_fr(dp, ri, rc, T.r2damp);
-                            if ((skip >> (d - 1)) & 1u) fr = 0.f;
-                            const float3 f = dp * fr;
-                            bs = bs + f;
-                            rr.x -= f.x; rr.y -= f.y; rr.z -= f.z;
+                            if (d0 + u > 1) rotate();
+                            pair(d0 + u, pc[u], rc[u], bs);
+                        }
+                        pacc.add(bs);
+                    }
+                    if (d0 <= nd) {
+                        float3 bs = make_float3(0.f, 0.f, 0.f);
+                        for (; d0 <= nd; d0++) {
+                            const float4 pc = prow[d0], rc = rrow[d0];
+                            if (d0 > 1) rotate();
+                            pair(d0, pc, rc, bs);
                         }
                         pacc.add(bs);
                     }
@@ -673,7 +733,7 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     rr.z = __shfl(rr.z, src, 64);
                 }
                 pf[0] = pacc.value();
-            } else if (A.pair_mode && T.use_pairs) {
+            } else if (!ROT && A.pair_mode && T.use_pairs) {
                 float4 *fpw = s_fp + wid * A.lds_atoms;
                 const int nd = N >> 1;
                 const bool even = (N & 1) == 0;
@@ -805,6 +865,40 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                 }
                 block_sync<NW>();
             }
+            // ROT: the gather plan's ranges, all 64 lanes (see the plan above);
+            // the own range's rounds and then the helper's sum go into the
+            // atom's compensated sum
+            [[maybe_unused]] float3 g_own = make_float3(0.f, 0.f, 0.f), g_tail = make_float3(0.f, 0.f, 0.f);
+            if constexpr (ROT) {
+                KahanF3 gk;
+                gk.init(make_float3(0.f, 0.f, 0.f));
+                for (int q = 0; q < g_cnt; q += 8) {
+                    const float *px = fint_x + g_start + q, *py = fint_y + g_start + q, *pz = fint_z + g_start + q;
+                    float3 pv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) pv[u] = make_float3(px[u], py[u], pz[u]);
+                    float3 rs8 = pv[0];
+                    if constexpr (FINT_LDS) {
+                        // entries past the range (the next atom's pieces,
+                        // zeroed slots) weighted by 0
+#pragma unroll
+                        for (int u = 1; u < 8; u++) {
+                            const float w = q + u < g_cnt ? 1.f : 0.f;
+                            rs8.x = __builtin_fmaf(pv[u].x, w, rs8.x);
+                            rs8.y = __builtin_fmaf(pv[u].y, w, rs8.y);
+                            rs8.z = __builtin_fmaf(pv[u].z, w, rs8.z);
+                        }
+                    } else {
+#pragma unroll
+                        for (int u = 1; u < 8; u++)
+                            if (q + u < g_cnt) rs8 = rs8 + pv[u];
+                    }
+                    gk.add(rs8);
+                }
+                g_own = gk.value();
+                const int hs = g_help >= 0 ? g_help : lane;
+                g_tail = make_float3(__shfl(g_own.x, hs, 64), __shfl(g_own.y, hs, 64), __shfl(g_own.z, hs, 64));
+            }
             float3 F[APT];
 #pragma unroll
             for (int k = 0; k < APT; k++) {
@@ -831,17 +925,22 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     // to 7 entries past the atom's last piece: they lie in the
                     // next atom's pieces, the next array or the 8-float tail
                     // that every force-piece area carries (index < 3*nf_max + 8)
-                    const int pb = my_a2f[k].x, cnt = my_a2f[k].y;
-                    for (int q = 0; q < cnt; q += 8) {
-                        const float *px = fint_x + pb + q, *py = fint_y + pb + q, *pz = fint_z + pb + q;
-                        float3 pv[8];
+                    if constexpr (ROT) {
+                        acc.add(g_own);
+                        if (g_help >= 0) acc.add(g_tail);
+                    } else {
+                        const int pb = my_a2f[k].x, cnt = my_a2f[k].y;
+                        for (int q = 0; q < cnt; q += 8) {
+                            const float *px = fint_x + pb + q, *py = fint_y + pb + q, *pz = fint_z + pb + q;
+                            float3 pv[8];
 #pragma unroll
-                        for (int u = 0; u < 8; u++) pv[u] = make_float3(px[u], py[u], pz[u]);
-                        float3 rs8 = pv[0];
+                            for (int u = 0; u < 8; u++) pv[u] = make_float3(px[u], py[u], pz[u]);
+                            float3 rs8 = pv[0];
 #pragma unroll
-                        for (int u = 1; u < 8; u++)
-                            if (q + u < cnt) rs8 = rs8 + pv[u];
-                        acc.add(rs8);
+                            for (int u = 1; u < 8; u++)
+                                if (q + u < cnt) rs8 = rs8 + pv[u];
+                            acc.add(rs8);
+                        }
                     }
                     const float4 pi4 = s_pos[i];
                     const float3 pi = f3(pi4);
@@ -1034,7 +1133,9 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     const float3 v2 = lp::mixed_velocity(my_v[0], F[0], fs.a, fs.b, fs.reset);
                     const float3 p2 = lp::moved(pcur, v2, fs.dt_move);
                     my_v[0] = v2;
-                    s_pos[tid] = make_float4(p2.x, p2.y, p2.z, s_pos[tid].w);
+                    const float4 p4 = make_float4(p2.x, p2.y, p2.z, s_pos[tid].w);
+                    s_pos[tid] = p4;
+                    if (tid <= (N >> 1)) s_pos[N + tid] = p4;
                 }
                 kstep++;
                 step_in_launch++;
@@ -1168,6 +1269,7 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     p.y += my_v[k].y * dt_move;
                     p.z += my_v[k].z * dt_move;
                     s_pos[i] = p;
+                    if (ROT && i <= (N >> 1)) s_pos[N + i] = p;
                 }
             }
             if (A.mode == KMODE_MD && p_vf < 0.0) {

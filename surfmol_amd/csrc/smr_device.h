@@ -457,13 +457,46 @@ __device__ __forceinline__ float4 grid_eval_cached(const float4 *__restrict__ sc
 // reductions (deterministic: xor butterflies give bitwise-identical results
 // in every lane; cross-wave partials are summed in wave order)
 // ---------------------------------------------------------------------------
+// Each step combines a lane's value with one partner's (a + b in one lane,
+// b + a in the other: equal bits), so after six steps every lane holds the
+// same result.  Partners: xor 1 and xor 2 in a quad (DPP quad_perm), the
+// other quad of the half-row and the other half of the row (DPP
+// row_half_mirror, row_mirror: the quads and halves already hold equal
+// values), the other row of a row pair and the other half of the wave
+// (v_permlane16_swap, v_permlane32_swap).  Registers only, no LDS.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)u, CTRL, 0xf, 0xf, false);
+    const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <bool ROW16>
+__device__ __forceinline__ void swap_halves(double v, double &a, double &b) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+    const auto l = ROW16 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                         : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = ROW16 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                         : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    a = __builtin_bit_cast(double, ((unsigned long long)h[0] << 32) | l[0]);
+    b = __builtin_bit_cast(double, ((unsigned long long)h[1] << 32) | l[1]);
+}
+template <typename Op>
+__device__ __forceinline__ double wave_reduce(double v, Op op) {
+    v = op(v, dpp_f64<0xB1>(v));    // quad_perm [1,0,3,2]
+    v = op(v, dpp_f64<0x4E>(v));    // quad_perm [2,3,0,1]
+    v = op(v, dpp_f64<0x141>(v));   // row_half_mirror
+    v = op(v, dpp_f64<0x140>(v));   // row_mirror
+    double a, b;
+    swap_halves<true>(v, a, b);     // rows (0,1), (2,3): a = the even row's, b = the odd row's
+    v = op(a, b);
+    swap_halves<false>(v, a, b);    // lanes 0-31 and 32-63
+    return op(a, b);
+}
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    return wave_reduce(v, [](double x, double y) { return x + y; });
 }
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
+    return wave_reduce(v, [](double x, double y) { return fmax(x, y); });
 }

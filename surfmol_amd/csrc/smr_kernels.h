@@ -129,10 +129,22 @@ inline int relax_lds_atoms(int block, int max_atoms) { return max_atoms > 16 * (
 // floats after a force-piece area that the gather may read (not use)
 constexpr int kPieceTail = 8;
 
+// Rows of the position and REQ arrays: one wave with one atom per thread
+// (the register-rotation pair sweep) keeps a copy of rows 0..lds_atoms/2
+// after the last one, so that the partner of sweep d is row i + d without a
+// wrap; its reaction forces stay in registers, so it has no per-wave
+// reaction buffer.
+__host__ __device__ inline bool relax_rot_shape(int block, int apt) { return block == 64 && apt == 1; }
+__host__ __device__ inline int relax_lds_rows(int block, int apt, int lds_atoms) {
+    return relax_rot_shape(block, apt) ? lds_atoms + lds_atoms / 2 + 1 : lds_atoms;
+}
+
 inline size_t relax_lds_bytes(int block, int apt, int lds_atoms, int nf_max, int na, int nd, int ni, bool fint_lds,
                               bool pair_mode) {
     const int nw = block / 64;
-    size_t b = (size_t)lds_atoms * 16 * (6 + (pair_mode ? nw : 1)) + (size_t)nw * 8 * 8 + 16;
+    const int rows = relax_lds_rows(block, apt, lds_atoms);
+    const int fp = relax_rot_shape(block, apt) ? 0 : (pair_mode ? nw : 1);
+    size_t b = (size_t)16 * (2 * rows + (size_t)lds_atoms * (4 + fp)) + (size_t)nw * 8 * 8 + 16;
     const bool tables = apt < 4;
     // slots (8, 8, 4 B) and piece destinations (8 B) per dihedral, inversion, angle
     if (tables) b += (size_t)nd * 16 + (size_t)ni * 16 + (size_t)na * 12;

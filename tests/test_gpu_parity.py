@@ -92,10 +92,13 @@ def test_relax_batched_matches_fp64(ctx, surf, oracle, params):
         assert r["status"] == S.SMR_SYS_CONVERGED, r
         sl = slice(b.offsets[s], b.offsets[s + 1])
         osys = oracle_system(oracle, params, mol, X[s], Cg, spec)
-        # FP64 evaluation at the GPU's final positions (after the last move)
+        # FP64 evaluation at the GPU's final positions (after the last move:
+        # convergence is tested on the forces before it, the reference's
+        # order, and the move can raise max|F| above fconv = 1e-3; 2.3e-3 has
+        # been observed)
         fo, eo = oracle.eval_forces(osys, Xf[sl])
         fmax64 = np.linalg.norm(fo, axis=1).max()
-        assert fmax64 < 2e-3, (s, fmax64)
+        assert fmax64 < 3e-3, (s, fmax64)
         # the reported energy is the FP64 energy of the returned positions
         assert abs(eo.sum() - r["energy"]) / mol.natoms < 1e-6, (s, eo.sum(), r["energy"])
         # the device's minimum is an FP64 minimum: FP64 FIRE started there
