@@ -114,7 +114,7 @@ def workload_uneven(args, S, W):
     return mols, np.arange(nsys, dtype=np.int32), X, 48.0, desc
 
 
-def raff_bench(args, S, W, nsys, nsteps=100, warmup=10, cpu_seconds=0.0):
+def raff_bench(args, S, W, nsys, nsteps=100, warmup=10, cpu_seconds=0.0, serve=True):
     """The reference's RAFF benchmark (crates/libs/oclff/src/bin/
     raff_ocl_benchmark.rs): xylitol (22 atoms, 10 nodes) packed into 32-atom
     groups, port stiffness 200, its LCG perturbations, StepConfig dt 0.1,
@@ -156,33 +156,34 @@ def raff_bench(args, S, W, nsys, nsteps=100, warmup=10, cpu_seconds=0.0):
                                    "replicas", "source": "notes/reports/2026-08-29_gpu_raff_benchmark.md:24"}}
     # serving: Poisson arrivals of single-replica relax requests (tol 2e-4,
     # <= 3000 steps) into a 4096-slot server, 64-step chunks
-    from surfmol_amd.raff import RaffServer
-    nreq, rate = min(nsys, 4000), 20000.0
-    tmpl = RaffReplicas.from_molecule(ctx, m, X[:1], 1)
-    srv = RaffServer(tmpl, capacity=4096, chunk_steps=64, max_steps=3000, tol=2e-4, cfg=cfg)
-    arr = W.poisson_arrivals(rate, nreq, W.SEED_BASE + 77)
-    ids = []
-    t0 = time.perf_counter()
-    for k in range(nreq):
-        wait = arr[k] - (time.perf_counter() - t0)
-        if wait > 0:
-            time.sleep(wait)
-        ids.append(srv.submit(X[k]))
-    srv.drain()
-    res = [srv.poll(i) for i in ids]
-    sst = srv.stats()
-    srv.close()
-    tmpl.close()
-    lat = np.array([r["latency"] for r in res])
-    span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
-    out["serve"] = {"rate_req_s": rate, "requests": nreq,
-                    "converged": int(sum(1 for r in res if r["status"] == 1)),
-                    "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
-                    "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
-                    "served_per_s": round(nreq / span, 1),
-                    "steps_mean": round(float(np.mean([r["steps"] for r in res])), 1),
-                    "chunks": int(sst["chunks"]), "peak_resident": int(sst["peak_resident"]),
-                    "config": "4096 slots, 64-step chunks, tol 2e-4 on the step's largest correction, <= 3000 steps"}
+    if serve:
+        from surfmol_amd.raff import RaffServer
+        nreq, rate = min(nsys, 4000), 20000.0
+        tmpl = RaffReplicas.from_molecule(ctx, m, X[:1], 1)
+        srv = RaffServer(tmpl, capacity=4096, chunk_steps=64, max_steps=3000, tol=2e-4, cfg=cfg)
+        arr = W.poisson_arrivals(rate, nreq, W.SEED_BASE + 77)
+        ids = []
+        t0 = time.perf_counter()
+        for k in range(nreq):
+            wait = arr[k] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            ids.append(srv.submit(X[k]))
+        srv.drain()
+        res = [srv.poll(i) for i in ids]
+        sst = srv.stats()
+        srv.close()
+        tmpl.close()
+        lat = np.array([r["latency"] for r in res])
+        span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+        out["serve"] = {"rate_req_s": rate, "requests": nreq,
+                        "converged": int(sum(1 for r in res if r["status"] == 1)),
+                        "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+                        "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+                        "served_per_s": round(nreq / span, 1),
+                        "steps_mean": round(float(np.mean([r["steps"] for r in res])), 1),
+                        "chunks": int(sst["chunks"]), "peak_resident": int(sst["peak_resident"]),
+                        "config": "4096 slots, 64-step chunks, tol 2e-4 on the step's largest correction, <= 3000 steps"}
     if cpu_seconds > 0:
         from oracle import oracle as O
         O.build()
@@ -597,6 +598,7 @@ def main():
         return
     if args.workload == "raff":
         print(json.dumps(raff_bench(args, S, W, args.nsys or 5000, nsteps=max(args.steps, 100), warmup=10,
+                                    serve=not args.no_extras,
                                     cpu_seconds=0.0 if args.no_cpu else args.cpu_seconds)))
         return
     out = run_workload(args, S, W, args.workload, args.steps, args.warmup, args.cpu_seconds, not args.no_plain,
