@@ -95,11 +95,22 @@ __global__ __launch_bounds__(SEG < 64 ? 64 : SEG) void k_raff(RaffArgs A) {
     constexpr int BLK = SEG < 64 ? 64 : SEG;
     constexpr int R = BLK / SEG;            // replicas per workgroup
     constexpr int NW = BLK / 64;
+    // one replica per wave at most, and no more ports than lanes: the ports
+    // are spread over the lanes (one lane per port) instead of looping over
+    // each node's four slots
+    constexpr bool SPREAD = SEG <= 64;
+    constexpr int PB = SPREAD ? BLK : 1;
     __shared__ float4 l_pos[BLK];
     __shared__ float4 l_slot[BLK * 4];      // port recoils: [replica][node*4 + k]
     __shared__ float l_rad[BLK];
     __shared__ float l_red[R * NW];
     __shared__ int l_run[R];
+    __shared__ float4 l_quat[PB];           // node orientations of the step
+    __shared__ float4 l_pdp[PB], l_pdt[PB]; // per port: node translation, rotation contributions
+    __shared__ int l_port[PB];              // per replica: its ports (node*4 + k | neighbour << 16), node-major
+    __shared__ float4 l_parm[PB];           // ... their arms (xyz) and stiffnesses (w)
+    __shared__ int l_pfirst[PB];            // per node: first port | port count << 16
+    __shared__ int l_nport[R];
 
     const int tid = threadIdx.x;
     const int r = tid / SEG, lid = tid % SEG;
@@ -128,15 +139,7 @@ __global__ __launch_bounds__(SEG < 64 ? 64 : SEG) void k_raff(RaffArgs A) {
     const float invM = p4.w;
     const bool active = inrange && invM > 1e-12f;
     const bool isnode = inrange && lid < nnode;
-    // this node's port arms and stiffnesses (shared topology)
-    float4 arm[4];
-    float kst[4];
     const int ngs[4] = {ng.x, ng.y, ng.z, ng.w};
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        arm[k] = isnode ? A.port_local[lid * 4 + k] : make_float4(0.f, 0.f, 0.f, 0.f);
-        kst[k] = isnode ? A.kflat[lid * 4 + k] : 0.f;
-    }
     const float mi = active ? 1.0f / invM : 0.f;
     const float invI = A.rot_mass_scale / (0.4f * mi + 1e-12f);
     const float dt2 = A.dt * A.dt + 1e-16f;
@@ -155,6 +158,46 @@ __global__ __launch_bounds__(SEG < 64 ? 64 : SEG) void k_raff(RaffArgs A) {
     bool ran = false;
     float4 *slot = l_slot + r * SEG * 4;
 
+    // SPREAD: the replica's ports (slots k of node i with a neighbour j < n
+    // and a positive stiffness, node-major) go to the lanes of its segment,
+    // one each (a table in LDS, made once per launch); a node adds its ports'
+    // contributions in slot order, as the loop over its slots does.  Port
+    // slots without a port are never written: they stay zero.  With more
+    // ports than lanes the node loop runs instead.
+    bool spread = false;
+    int pfirst = 0, pcnt = 0;
+    const int *ptab = l_port + r * SEG;
+    const float4 *parm = l_parm + r * SEG;
+    if constexpr (SPREAD) {
+        if (lid == 0) {
+            int c = 0;
+            for (int i = 0; i < nnode && i < n; i++) {
+                const int4 g = A.neighs[i];
+                const int gg[4] = {g.x, g.y, g.z, g.w};
+                const int c0 = c;
+                for (int k = 0; k < 4; k++)
+                    if (gg[k] >= 0 && gg[k] < n && A.kflat[i * 4 + k] > 0.f) {
+                        if (c < SEG) {
+                            const float4 a = A.port_local[i * 4 + k];
+                            l_parm[r * SEG + c] = make_float4(a.x, a.y, a.z, A.kflat[i * 4 + k]);
+                            l_port[r * SEG + c] = (i * 4 + k) | (gg[k] << 16);
+                        }
+                        c++;
+                    }
+                l_pfirst[r * SEG + i] = c0 | ((c - c0) << 16);
+            }
+            l_nport[r] = c;
+        }
+        for (int e = lid; e < SEG * 4; e += SEG) slot[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        spread = l_nport[r] <= SEG;
+        if (spread && isnode) {
+            const int pf = l_pfirst[r * SEG + lid];
+            pfirst = pf & 0xffff;
+            pcnt = pf >> 16;
+        }
+    }
+
     for (int it = 0; it < A.nsteps; it++) {
         // does any replica of this workgroup still run (relax mode)?
         if (A.relax) {
@@ -168,6 +211,7 @@ __global__ __launch_bounds__(SEG < 64 ? 64 : SEG) void k_raff(RaffArgs A) {
         }
         l_pos[tid] = p4;
         l_rad[tid] = (active && rad > 0.f) ? rad : 0.f;
+        if constexpr (SPREAD) l_quat[tid] = q;
         __syncthreads();
         const bool work = running && active;
         // ---- collision (compute_collision_multi) ----
@@ -195,17 +239,73 @@ __global__ __launch_bounds__(SEG < 64 ? 64 : SEG) void k_raff(RaffArgs A) {
         }
         // ---- ports (compute_ports_current_multi) ----
         float3 sdp = make_float3(0.f, 0.f, 0.f), sdt = make_float3(0.f, 0.f, 0.f);
-        if (isnode) {
+        if (SPREAD && spread) {
+            // one port per lane: the node's state from LDS (its invI and
+            // activity recomputed with the node lane's operations), the
+            // contributions into the port's entry, the recoil into its slot
+            const float4 *rp = l_pos + r * SEG;
+            const int pp = lid;
+            if (pp < l_nport[r]) {
+                const int code = ptab[pp];
+                const int i = (code & 0xffff) >> 2, k = code & 3, j = code >> 16;
+                const float4 arm4 = parm[pp];
+                const float4 xi4 = rp[i];
+                const float invMi = xi4.w;
+                const bool wi = running && invMi > 1e-12f;
+                float4 rec = make_float4(0.f, 0.f, 0.f, 0.f), cdp = rec, cdt = rec;
+                if (wi) {
+                    const float mii = 1.0f / invMi;
+                    const float invIi = A.rot_mass_scale / (0.4f * mii + 1e-12f);
+                    const float4 pj4 = rp[j];
+                    if (pj4.w > 1e-12f) {
+                        const float3 ra = qrot(l_quat[r * SEG + i], v3(arm4));
+                        const float3 diff = vsub(v3(pj4), vadd(v3(xi4), ra));
+                        const float d2 = vdot(diff, diff);
+                        if (d2 >= 1e-16f) {
+                            const float dist = sqrtf(d2);
+                            const float3 nn = vmul(diff, 1.0f / dist);
+                            const float3 rxn = vcross(ra, nn);
+                            const float wang = vdot(rxn, rxn) * invIi;
+                            const float alpha = 1.0f / (arm4.w * dt2);
+                            const bool jnode = j < nnode;
+                            float wtot = invMi + pj4.w + wang + alpha + 1e-12f;
+                            if (jnode) wtot += wang;
+                            float imp = dist / wtot;
+                            if (jnode) imp *= 0.5f;
+                            const float3 P = vmul(nn, imp);
+                            const float3 a = vmul(P, invMi), b = vmul(vcross(ra, P), invIi);
+                            cdp = make_float4(a.x, a.y, a.z, 0.f);
+                            cdt = make_float4(b.x, b.y, b.z, 0.f);
+                            const float3 rj = vmul(P, -pj4.w);
+                            rec = make_float4(rj.x, rj.y, rj.z, 0.f);
+                        }
+                    }
+                }
+                if (running) {
+                    l_pdp[r * SEG + pp] = cdp;
+                    l_pdt[r * SEG + pp] = cdt;
+                    slot[i * 4 + k] = rec;
+                }
+            }
+            __syncthreads();
+            for (int t = 0; t < pcnt; t++) {
+                sdp = vadd(sdp, v3(l_pdp[r * SEG + pfirst + t]));
+                sdt = vadd(sdt, v3(l_pdt[r * SEG + pfirst + t]));
+            }
+        } else if (isnode) {
             const float3 xi = v3(p4);
             const float4 *rp = l_pos + r * SEG;
+            // this node's port arms and stiffnesses (shared topology, read
+            // through the caches)
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
                 const int j = ngs[k];
-                if (work && j >= 0 && j < n && kst[k] > 0.f) {
+                const float kstk = A.kflat[lid * 4 + k];
+                if (work && j >= 0 && j < n && kstk > 0.f) {
                     const float4 pj4 = rp[j];
                     if (pj4.w > 1e-12f) {
-                        const float3 ra = qrot(q, v3(arm[k]));
+                        const float3 ra = qrot(q, v3(A.port_local[lid * 4 + k]));
                         const float3 diff = vsub(v3(pj4), vadd(xi, ra));
                         const float d2 = vdot(diff, diff);
                         if (d2 >= 1e-16f) {
@@ -213,7 +313,7 @@ __global__ __launch_bounds__(SEG < 64 ? 64 : SEG) void k_raff(RaffArgs A) {
                             const float3 nn = vmul(diff, 1.0f / dist);
                             const float3 rxn = vcross(ra, nn);
                             const float wang = vdot(rxn, rxn) * invI;
-                            const float alpha = 1.0f / (kst[k] * dt2);
+                            const float alpha = 1.0f / (kstk * dt2);
                             const bool jnode = j < nnode;
                             float wtot = invM + pj4.w + wang + alpha + 1e-12f;
                             if (jnode) wtot += wang;
