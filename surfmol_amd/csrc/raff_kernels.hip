@@ -91,7 +91,7 @@ __device__ __forceinline__ float seg_max(float v) {
 
 // One replica per segment of SEG lanes; BLK = max(SEG, 64) threads.
 template <int SEG>
-__global__ __launch_bounds__(SEG < 64 ? 64 : SEG) void k_raff(RaffArgs A) {
+__global__ __launch_bounds__(SEG < 64 ? 64 : SEG) __attribute__((amdgpu_waves_per_eu(4))) void k_raff(RaffArgs A) {
     constexpr int BLK = SEG < 64 ? 64 : SEG;
     constexpr int R = BLK / SEG;            // replicas per workgroup
     constexpr int NW = BLK / 64;
@@ -169,25 +169,40 @@ __global__ __launch_bounds__(SEG < 64 ? 64 : SEG) void k_raff(RaffArgs A) {
     const int *ptab = l_port + r * SEG;
     const float4 *parm = l_parm + r * SEG;
     if constexpr (SPREAD) {
-        if (lid == 0) {
-            int c = 0;
-            for (int i = 0; i < nnode && i < n; i++) {
-                const int4 g = A.neighs[i];
-                const int gg[4] = {g.x, g.y, g.z, g.w};
-                const int c0 = c;
-                for (int k = 0; k < 4; k++)
-                    if (gg[k] >= 0 && gg[k] < n && A.kflat[i * 4 + k] > 0.f) {
-                        if (c < SEG) {
-                            const float4 a = A.port_local[i * 4 + k];
-                            l_parm[r * SEG + c] = make_float4(a.x, a.y, a.z, A.kflat[i * 4 + k]);
-                            l_port[r * SEG + c] = (i * 4 + k) | (gg[k] << 16);
-                        }
-                        c++;
-                    }
-                l_pfirst[r * SEG + i] = c0 | ((c - c0) << 16);
+        // each node lane lists its own ports; an exclusive prefix over the
+        // segment places them (node-major, slot order)
+        float kk[4] = {0.f, 0.f, 0.f, 0.f};
+        int cnt = 0;
+        if (lid < nnode && lid < n) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                kk[k] = A.kflat[lid * 4 + k];
+                cnt += (ngs[k] >= 0 && ngs[k] < n && kk[k] > 0.f) ? 1 : 0;
             }
-            l_nport[r] = c;
         }
+        int first = cnt;
+#pragma unroll
+        for (int o = 1; o < SEG; o <<= 1) {
+            const int y = __shfl_up(first, o, SEG);
+            if (lid >= o) first += y;
+        }
+        const int total = __shfl(first, SEG - 1, SEG);
+        first -= cnt;
+        if (lid < nnode && lid < n) {
+            int c = first;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (ngs[k] >= 0 && ngs[k] < n && kk[k] > 0.f) {
+                    if (c < SEG) {
+                        const float4 a = A.port_local[lid * 4 + k];
+                        l_parm[r * SEG + c] = make_float4(a.x, a.y, a.z, kk[k]);
+                        l_port[r * SEG + c] = (lid * 4 + k) | (ngs[k] << 16);
+                    }
+                    c++;
+                }
+            l_pfirst[r * SEG + lid] = first | (cnt << 16);
+        }
+        if (lid == 0) l_nport[r] = total;
         for (int e = lid; e < SEG * 4; e += SEG) slot[e] = make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
         spread = l_nport[r] <= SEG;
