@@ -35,6 +35,17 @@ __device__ __forceinline__ void block_sync() {
     }
 }
 
+// The launch's arguments in the kernel-argument segment (constant address
+// space: scalar loads).  fresh_args() hides the pointer from the optimiser,
+// so that fields read through it are loaded where they are used instead of
+// being held in scalar registers for the whole kernel (which spills them).
+typedef const __attribute__((address_space(4))) KernelArgs ConstArgs;
+__device__ __forceinline__ ConstArgs *fresh_args() {
+    ConstArgs *p = (ConstArgs *)(__builtin_amdgcn_kernarg_segment_ptr());
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 // Value of lane (lane+1) mod 64 (DPP wave_rol:1).
 __device__ __forceinline__ float wave_rol1(float x) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x134, 0xf, 0xf, true));
@@ -58,7 +69,8 @@ __device__ __forceinline__ float wave_rot1(float x, int last, int lane) {
 // atom's own lane picks up its 16 values.  A wave then waits for one round of
 // grid loads per nslot changed atoms instead of one per x plane of every
 // changed atom; the values equal grid_contract_stencil's.
-__device__ __forceinline__ void refresh_stencils(const DevTables &T, bool need, const GridCell &gc, int key,
+__device__ __forceinline__ void refresh_stencils(const float4 *grid, int gnx, int gny, int gnz, bool need,
+                                                 const GridCell &gc, int key,
                                                  float4 plq, float4 (&rs)[16], int &rkey, float4 *stage, int nslot,
                                                  int lane) {
     unsigned long long pend = __builtin_amdgcn_ballot_w64(need);
@@ -72,9 +84,9 @@ __device__ __forceinline__ void refresh_stencils(const DevTables &T, bool need, 
         const int oix = __shfl(gc.ix, owner, 64), oiy = __shfl(gc.iy, owner, 64), oiz = __shfl(gc.iz, owner, 64);
         const float4 op = make_float4(__shfl(plq.x, owner, 64), __shfl(plq.y, owner, 64), __shfl(plq.z, owner, 64), 0.f);
         if (serve) {
-            int xi = oix + (col >> 2); if (xi >= T.gnx) xi -= T.gnx;
-            int yi = oiy + (col & 3); if (yi >= T.gny) yi -= T.gny;
-            const float4 *c = T.grid + ((size_t)(xi * T.gny + yi) * T.gnz + oiz);
+            int xi = oix + (col >> 2); if (xi >= gnx) xi -= gnx;
+            int yi = oiy + (col & 3); if (yi >= gny) yi -= gny;
+            const float4 *c = grid + ((size_t)(xi * gny + yi) * gnz + oiz);
             stage[q * 16 + col] = make_float4(dotplq(op, c[0]), dotplq(op, c[1]), dotplq(op, c[2]), dotplq(op, c[3]));
         }
         block_sync<1>();
@@ -153,21 +165,25 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
     int pass = 0;
     while (true) {
         // ---- fetch the next system from the work list ----
+        // (tables and state arrays read through fresh_args(): these reads
+        // happen once per system, and the step loop keeps its scalar
+        // registers for its own constants)
+        ConstArgs *fa = fresh_args();
         int wi;
         if (one_each) {
             if (pass++) break;
             wi = blockIdx.x;
         } else {
-            if (tid == 0) s_next[0] = atomicAdd(A.work_counter, 1);
+            if (tid == 0) s_next[0] = atomicAdd(fa->work_counter, 1);
             block_sync<NW>();
             wi = s_next[0];
             block_sync<NW>();
         }
-        if (wi >= A.nwork) break;
-        const int s = A.work_list[wi];
-        const SysDesc sd = T.sys[s];
-        const TopoDesc td = T.topo[sd.topo];
-        SysState st = A.state[s];
+        if (wi >= fa->nwork) break;
+        const int s = fa->work_list[wi];
+        const SysDesc sd = fa->tab.sys[s];
+        const TopoDesc td = fa->tab.topo[sd.topo];
+        SysState st = fa->state[s];
         // a finished system is not stepped again (forces are always evaluated)
         // (a persistent layer's workgroups all take part in every step)
         if (!PERSIST && A.mode != KMODE_EVAL && st.status != SMR_SYS_RUNNING) continue;
@@ -176,16 +192,16 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
         if (FINT_LDS) {
             fint_x = s_fx;
         } else {
-            fint_x = A.fint_scratch + (size_t)blockIdx.x * 3 * A.nf_max;
+            fint_x = fa->fint_scratch + (size_t)blockIdx.x * 3 * fa->nf_max;
         }
-        fint_y = fint_x + A.nf_max;
-        fint_z = fint_y + A.nf_max;
+        fint_y = fint_x + fa->nf_max;
+        fint_z = fint_y + fa->nf_max;
         // ROT with pieces in LDS: the gather masks by weight (0 or 1), so
         // every float it may read must be finite: the slots this system does
         // not write ([nf, nf_max) of each array) and the tail are zeroed
         if constexpr (ROT && FINT_LDS) {
-            for (int q = td.nf + tid; q < A.nf_max; q += BLOCK) fint_x[q] = fint_y[q] = fint_z[q] = 0.f;
-            if (tid < kPieceTail) fint_z[A.nf_max + tid] = 0.f;
+            for (int q = td.nf + tid; q < fa->nf_max; q += BLOCK) fint_x[q] = fint_y[q] = fint_z[q] = 0.f;
+            if (tid < kPieceTail) fint_z[fa->nf_max + tid] = 0.f;
         }
 
         // ---- stage the system ----
@@ -198,14 +214,14 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
             const int i = tid + k * BLOCK;
             if (i < N) {
                 const int ta = td.atom_off + i;
-                s_pos[i] = A.pos[abase + i];
-                s_req[i] = T.req[ta];
-                my_ng[k] = T.ng[ta];
-                my_bk[k] = T.bk[ta];
-                my_bl[k] = T.bl[ta];
-                my_plq[k] = T.plq[ta];
-                my_a2f[k] = T.a2f[ta];
-                float4 v = A.vel[abase + i];
+                s_pos[i] = fa->pos[abase + i];
+                s_req[i] = fa->tab.req[ta];
+                my_ng[k] = fa->tab.ng[ta];
+                my_bk[k] = fa->tab.bk[ta];
+                my_bl[k] = fa->tab.bl[ta];
+                my_plq[k] = fa->tab.plq[ta];
+                my_a2f[k] = fa->tab.a2f[ta];
+                float4 v = fa->vel[abase + i];
                 my_v[k] = make_float3(v.x, v.y, v.z);
             } else {
                 my_ng[k] = make_int4(-1, -1, -1, -1);
@@ -254,32 +270,32 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
         // interaction slots and piece destinations stay in LDS for all steps
         if constexpr (TABLES_LDS) {
             for (int q = tid; q < td.nangles; q += BLOCK) {
-                s_ang[q] = T.ang_ng[td.ang_off + q];
-                s_angd[q] = T.ang_dst[td.ang_off + q];
+                s_ang[q] = fa->tab.ang_ng[td.ang_off + q];
+                s_angd[q] = fa->tab.ang_dst[td.ang_off + q];
             }
             for (int q = tid; q < td.ndih; q += BLOCK) {
-                s_dih[q] = T.dih_ng[td.dih_off + q];
-                s_dihd[q] = T.dih_dst[td.dih_off + q];
+                s_dih[q] = fa->tab.dih_ng[td.dih_off + q];
+                s_dihd[q] = fa->tab.dih_dst[td.dih_off + q];
             }
             for (int q = tid; q < td.ninv; q += BLOCK) {
-                s_inv[q] = T.inv_ng[td.inv_off + q];
-                s_invd[q] = T.inv_dst[td.inv_off + q];
+                s_inv[q] = fa->tab.inv_ng[td.inv_off + q];
+                s_invd[q] = fa->tab.inv_dst[td.inv_off + q];
             }
             t_ang = s_ang; t_dih = s_dih; t_inv = s_inv; t_angd = s_angd; t_dihd = s_dihd; t_invd = s_invd;
         } else {
-            t_ang = T.ang_ng + td.ang_off; t_dih = T.dih_ng + td.dih_off; t_inv = T.inv_ng + td.inv_off;
-            t_angd = T.ang_dst + td.ang_off; t_dihd = T.dih_dst + td.dih_off; t_invd = T.inv_dst + td.inv_off;
+            t_ang = fa->tab.ang_ng + td.ang_off; t_dih = fa->tab.dih_ng + td.dih_off; t_inv = fa->tab.inv_ng + td.inv_off;
+            t_angd = fa->tab.ang_dst + td.ang_off; t_dihd = fa->tab.dih_dst + td.dih_off; t_invd = fa->tab.inv_dst + td.inv_off;
         }
         block_sync<NW>();
 
-        // cell of each own atom's stored surface stencil (A.stencil): the
+        // cell of each own atom's stored surface stencil (fa->stencil): the
         // PLQH-contracted 4x4x4 stencil is re-read from the grid only when
         // the atom enters another cell
         int skey[APT];
 #pragma unroll
         for (int k = 0; k < APT; k++) {
             const int i = tid + k * BLOCK;
-            skey[k] = (A.stencil_key && i < N) ? A.stencil_key[abase + i] : -1;
+            skey[k] = (fa->stencil_key && i < N) ? fa->stencil_key[abase + i] : -1;
         }
         // one atom per thread: the contracted stencil is held in registers
         // for the whole launch (rkey = its cell, -1: none); the stored copy
@@ -287,10 +303,10 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
         float4 rs[RSTEN ? 16 : 1];
         int rkey = -1;
         if constexpr (RSTEN) {
-            if (A.stencil && tid < N && skey[0] != -1) {
+            if (fa->stencil && tid < N && skey[0] != -1) {
                 rkey = skey[0];
                 if (rkey >= 0) {
-                    const float4 *sc = A.stencil + (size_t)abase * 16 + tid;
+                    const float4 *sc = fa->stencil + (size_t)abase * 16 + tid;
 #pragma unroll
                     for (int e = 0; e < 16; e++) rs[e] = sc[e * N];
                 }
@@ -298,9 +314,9 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
         }
         auto persist_stencil = [&]() {
             if constexpr (RSTEN) {
-                if (A.stencil && tid < N && rkey != skey[0]) {
+                if (fa->stencil && tid < N && rkey != skey[0]) {
                     if (rkey >= 0) {
-                        float4 *sc = A.stencil + (size_t)abase * 16 + tid;
+                        float4 *sc = fa->stencil + (size_t)abase * 16 + tid;
 #pragma unroll
                         for (int e = 0; e < 16; e++) sc[e * N] = rs[e];
                     }
@@ -309,7 +325,6 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
             }
         };
         const int nsteps = A.steps_per_launch;
-        double fm2_cur = st.fmax * st.fmax;   // max |F|^2 of the last step
         int step_in_launch = 0;
 
         // ---- coupled layer in one cooperative launch (KMODE_LAYER_P) ----
@@ -806,7 +821,7 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     const bool need = own && key != rkey;
                     // staging: 4 slots per wave in the hneigh area (the host
                     // reserves >= 16 atoms per wave), idle until the next bonds
-                    refresh_stencils(T, need, gc, key, my_plq[0], rs, rkey, s_h + wid * 64, 4, lane);
+                    refresh_stencils(T.grid, T.gnx, T.gny, T.gnz, need, gc, key, my_plq[0], rs, rkey, s_h + wid * 64, 4, lane);
                     if (own) fe_r = grid_eval_stencil([&](int e) { return rs[e]; }, gc);
                 }
             }
@@ -1209,26 +1224,41 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                 block_sync<NW>();
                 break;
             }
+            // Convergence (max |F|^2 < fconv^2) is tested as "no atom has
+            // |F|^2 >= fconv^2" (a ballot); max |F|^2 itself is reduced only
+            // when the system leaves the launch.
+            bool any_big;
             {
-                double r0 = wave_sum(p_vf), r1 = wave_sum(p_vv), r2 = wave_sum(p_ff), r3 = wave_max(p_fm);
+                const double fc2 = A.fire.fconv * A.fire.fconv;
+                any_big = __builtin_amdgcn_ballot_w64(p_fm >= fc2) != 0ull;
+                double r0, r1, r2;
+                wave_sum3(p_vf, p_vv, p_ff, r0, r1, r2);
                 if (NW > 1) {
                     if (lane == 0) {
                         s_red[wid * 8 + 0] = r0; s_red[wid * 8 + 1] = r1;
-                        s_red[wid * 8 + 2] = r2; s_red[wid * 8 + 3] = r3;
+                        s_red[wid * 8 + 2] = r2; s_red[wid * 8 + 3] = any_big ? 1.0 : 0.0;
                     }
                     block_sync<NW>();
-                    r0 = r1 = r2 = r3 = 0.0;
+                    r0 = r1 = r2 = 0.0;
+                    any_big = false;
                     for (int w = 0; w < NW; w++) {
                         r0 += s_red[w * 8 + 0]; r1 += s_red[w * 8 + 1];
-                        r2 += s_red[w * 8 + 2]; r3 = fmax(r3, s_red[w * 8 + 3]);
+                        r2 += s_red[w * 8 + 2]; any_big = any_big || s_red[w * 8 + 3] != 0.0;
                     }
                     block_sync<NW>();
                 }
-                p_vf = r0; p_vv = r1; p_ff = r2; p_fm = r3;
+                p_vf = r0; p_vv = r1; p_ff = r2;
             }
             const double dt_old = st.dt;
             float dt_move = (float)dt_old;
             bool nonfinite = !(isfinite(p_vf) && isfinite(p_vv) && isfinite(p_ff));
+            FireConst FC;
+            {
+                ConstArgs *fa = fresh_args();
+                FC.dt_max = fa->fire.dt_max; FC.alpha0 = fa->fire.alpha0; FC.f_inc = fa->fire.f_inc;
+                FC.f_dec = fa->fire.f_dec; FC.f_alpha = fa->fire.f_alpha; FC.fconv = fa->fire.fconv;
+                FC.n_min = fa->fire.n_min; FC.max_steps = fa->fire.max_steps; FC.md_dt = fa->fire.md_dt;
+            }
             if (A.mode == KMODE_FIRE) {
                 // FIRE decision (relax_pentacene_uff.rs:162-181)
                 if (p_vf > 0.0 && p_ff > 1e-30 && p_vv > 1e-30) {
@@ -1240,19 +1270,19 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
 #pragma unroll
                     for (int k = 0; k < APT; k++) my_v[k] = my_v[k] * af + F[k] * bf;
                     st.npos++;
-                    if (st.npos > A.fire.n_min) {
-                        st.dt = fmin(st.dt * A.fire.f_inc, A.fire.dt_max);
-                        st.alpha *= A.fire.f_alpha;
+                    if (st.npos > FC.n_min) {
+                        st.dt = fmin(st.dt * FC.f_inc, FC.dt_max);
+                        st.alpha *= FC.f_alpha;
                     }
                 } else {
                     st.npos = 0;
-                    st.dt *= A.fire.f_dec;
-                    st.alpha = A.fire.alpha0;
+                    st.dt *= FC.f_dec;
+                    st.alpha = FC.alpha0;
 #pragma unroll
                     for (int k = 0; k < APT; k++) my_v[k] = make_float3(0.f, 0.f, 0.f);
                 }
             } else {
-                dt_move = (float)A.fire.md_dt;
+                dt_move = (float)FC.md_dt;
                 if (p_vf < 0.0) {
                     // v is zeroed after the move (molecular.rs:156-158)
                 }
@@ -1278,35 +1308,41 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
             }
             st.steps++;
             step_in_launch++;
-            fm2_cur = p_fm;
             bool done = false;
             if (nonfinite) {
                 st.status = SMR_SYS_NONFINITE;
                 done = true;
-            } else if (A.mode == KMODE_FIRE ? (fm2_cur < A.fire.fconv * A.fire.fconv)
-                                            : (p_ff < A.fire.fconv * A.fire.fconv)) {
+            } else if (A.mode == KMODE_FIRE ? !any_big : (p_ff < FC.fconv * FC.fconv)) {
                 st.status = SMR_SYS_CONVERGED;
                 done = true;
-            } else if (st.steps >= A.fire.max_steps) {
+            } else if (st.steps >= FC.max_steps) {
                 st.status = SMR_SYS_MAXSTEPS;
                 done = true;
             }
             const bool leave = done || step_in_launch >= nsteps;
             if (leave) {
+                // max |F|^2 of this step's forces
+                double fm2 = wave_max(p_fm);
+                if (NW > 1) {
+                    if (lane == 0) s_red[wid * 8 + 4] = fm2;
+                    block_sync<NW>();
+                    fm2 = 0.0;
+                    for (int w = 0; w < NW; w++) fm2 = fmax(fm2, s_red[w * 8 + 4]);
+                }
                 // the energies of the final positions are evaluated in FP64
                 // afterwards (k_energy64)
                 if (tid == 0) {
-                    st.fmax = sqrt(fm2_cur);
-                    A.state[s] = st;
+                    st.fmax = sqrt(fm2);
+                    fa->state[s] = st;
                 }
 #pragma unroll
                 for (int k = 0; k < APT; k++) {
                     const int i = tid + k * BLOCK;
                     if (i < N) {
                         if (k == 0) persist_stencil();
-                        if (A.stencil_key) A.stencil_key[abase + i] = skey[k];
-                        A.pos[abase + i] = s_pos[i];
-                        A.vel[abase + i] = make_float4(my_v[k].x, my_v[k].y, my_v[k].z, 0.f);
+                        if (fa->stencil_key) fa->stencil_key[abase + i] = skey[k];
+                        fa->pos[abase + i] = s_pos[i];
+                        fa->vel[abase + i] = make_float4(my_v[k].x, my_v[k].y, my_v[k].z, 0.f);
                     }
                 }
                 block_sync<NW>();

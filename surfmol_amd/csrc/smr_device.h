@@ -467,8 +467,10 @@ __device__ __forceinline__ float4 grid_eval_cached(const float4 *__restrict__ sc
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
     const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)u, CTRL, 0xf, 0xf, false);
-    const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+    // every lane's source is valid for the patterns used (quad_perm, row
+    // mirrors), so bound_ctrl only spares the copy of the unused old value
+    const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)u, CTRL, 0xf, 0xf, true);
+    const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(u >> 32), CTRL, 0xf, 0xf, true);
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 template <bool ROW16>
@@ -496,6 +498,48 @@ __device__ __forceinline__ double wave_reduce(double v, Op op) {
 }
 __device__ __forceinline__ double wave_sum(double v) {
     return wave_reduce(v, [](double x, double y) { return x + y; });
+}
+
+// Lanes 0-31 of x and 32-63 of y exchanged (ROW16: the odd rows of x and the
+// even rows of y), then added: with ROW16 = false, lanes 0-31 hold
+// x[l] + x[l+32] and lanes 32-63 y[l-32] + y[l]; with ROW16 = true, rows
+// 0 and 2 hold x[r] + x[r+16] and rows 1 and 3 y[r-16] + y[r].
+template <bool ROW16>
+__device__ __forceinline__ double swap_add(double x, double y) {
+    const unsigned long long ux = __builtin_bit_cast(unsigned long long, x);
+    const unsigned long long uy = __builtin_bit_cast(unsigned long long, y);
+    const auto l = ROW16 ? __builtin_amdgcn_permlane16_swap((unsigned)ux, (unsigned)uy, false, false)
+                         : __builtin_amdgcn_permlane32_swap((unsigned)ux, (unsigned)uy, false, false);
+    const auto h = ROW16 ? __builtin_amdgcn_permlane16_swap((unsigned)(ux >> 32), (unsigned)(uy >> 32), false, false)
+                         : __builtin_amdgcn_permlane32_swap((unsigned)(ux >> 32), (unsigned)(uy >> 32), false, false);
+    const double a = __builtin_bit_cast(double, ((unsigned long long)h[0] << 32) | l[0]);
+    const double b = __builtin_bit_cast(double, ((unsigned long long)h[1] << 32) | l[1]);
+    return a + b;
+}
+
+// Wave sums of three values, wave-uniform (scalar) results.  Reduce-scatter:
+// the half-wave exchange leaves a's half sums in lanes 0-31 and b's in lanes
+// 32-63 (c is summed over the halves in every lane), the row exchange leaves
+// a in row 0, c in rows 1 and 3 and b in row 2, and four DPP steps sum each
+// row; the results are read from lanes 0, 32 and 16.  Every sum runs in a
+// fixed order, the same for every wave.
+__device__ __forceinline__ void wave_sum3(double a, double b, double c, double &ra, double &rb, double &rc) {
+    const double s = swap_add<false>(a, b);
+    const double c1 = swap_add<false>(c, c);
+    double t = swap_add<true>(s, c1);
+    t += dpp_f64<0xB1>(t);    // quad_perm [1,0,3,2]
+    t += dpp_f64<0x4E>(t);    // quad_perm [2,3,0,1]
+    t += dpp_f64<0x141>(t);   // row_half_mirror
+    t += dpp_f64<0x140>(t);   // row_mirror
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, t);
+    auto rd = [&](int lane) {
+        const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, lane);
+        const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), lane);
+        return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+    };
+    ra = rd(0);
+    rb = rd(32);
+    rc = rd(16);
 }
 __device__ __forceinline__ double wave_max(double v) {
     return wave_reduce(v, [](double x, double y) { return fmax(x, y); });
