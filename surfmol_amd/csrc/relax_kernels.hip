@@ -662,7 +662,10 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                 const int nd = N >> 1;
                 const bool even = (N & 1) == 0;
                 const float4 pi4 = own ? s_pos[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const float4 ri = own ? s_req[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+                // the own REQ row with the pair force's constant factors
+                // folded in: E0 = -12 sqrtEi sqrtEj and kQ = -qi qj
+                const float4 ri0 = own ? s_req[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 ri = make_float4(ri0.x, -12.f * ri0.y, -ri0.z, 0.f);
                 // bit d-1: sweep d is skipped (nd <= 32)
                 unsigned skip = own ? 0u : ~0u;
                 {
@@ -686,7 +689,7 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                 // reaction sum
                 auto pair = [&](int d, float4 pc, float4 rc, float3 &bs) {
                     const float3 dp = f3(pc) - f3(pi4);
-                    float fr = ljq_pair_fr(dp, ri, rc, T.r2damp);
+                    float fr = ljq_pair_fr_folded(dp, ri, rc, T.r2damp);
                     if ((skip >> (d - 1)) & 1u) fr = 0.f;
                     bs.x = __builtin_fmaf(dp.x, fr, bs.x);
                     bs.y = __builtin_fmaf(dp.y, fr, bs.y);

@@ -275,6 +275,21 @@ __device__ __forceinline__ float ljq_pair_fr(float3 dp, float4 ri, float4 rj, fl
     return -12.0f * (u6 - 1.0f) * vdW * ir2 - (kQ * ir_) * (ir_ * ir_);
 }
 
+// The same radial factor from a REQ row ri whose sqrtE and q carry the
+// force's constant factors (ri.y = -12 sqrtEi, ri.z = -qi): one multiply
+// fewer per pair.
+__device__ __forceinline__ float ljq_pair_fr_folded(float3 dp, float4 ri, float4 rj, float r2damp) {
+    const float R0 = ri.x + rj.x;
+    const float E0 = ri.y * rj.y;
+    const float kQ = ri.z * rj.z;
+    const float r2 = dot3(dp, dp);
+    const float ir_ = frsq(r2 + r2damp);
+    const float ir2 = frcp(r2);
+    const float u2 = R0 * R0 * ir2;
+    const float u6 = u2 * u2 * u2;
+    return (u6 - 1.0f) * (u6 * E0) * ir2 + (kQ * ir_) * (ir_ * ir_);
+}
+
 // ---------------------------------------------------------------------------
 // GridFF tricubic B-spline sample: fe3d_pbc_comb (gridff_eval.cl:234-286)
 // ---------------------------------------------------------------------------
