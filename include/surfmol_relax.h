@@ -326,6 +326,18 @@ int smr_batch_layer_path(const smr_batch *b, int *persistent, int *fallback, int
  * reference. */
 int smr_batch_set_order(smr_batch *b, int energy_order);
 
+/* Re-ranking of a relaxation's schedule (with the energy order on, and where
+ * a size class has more systems than resident workgroups): smr_relax_batched
+ * runs two launches.  Every system first takes up to `steps` FIRE steps; the
+ * unfinished ones are then ranked again by the FP64 energy per atom of their
+ * current positions and continue from their persisted state (positions,
+ * velocities, FIRE state, stored stencils).  The energy after a few hundred
+ * steps ranks the remaining step counts better than the starting energy, so
+ * long relaxations start earlier in the continuation.  -1 (default): after
+ * max_steps / 5 steps (one launch when that is below 50); 0: one launch.
+ * Results do not depend on it.  No counterpart in the reference. */
+int smr_batch_set_rerank(smr_batch *b, int steps);
+
 /* Damped MD of MolWorld::run_md (crates/libs/surfmol/src/mol_world.rs:141-164,
  * moltopo/src/molecular.rs:118-137) for every system: clamp |F| to flim,
  * v = (1-damping) v + F dt, x += v dt, v = 0 when sum v.F < 0, stop when

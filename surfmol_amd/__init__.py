@@ -441,6 +441,13 @@ class Batch:
         atom first in each size class; False: batch order."""
         check(lib().smr_batch_set_order(self._h, 1 if energy_order else 0))
 
+    def set_rerank(self, steps: int):
+        """Re-rank the unfinished systems by the energy of their current
+        positions after `steps` FIRE steps and continue in that order (two
+        launches); -1 (default) = after max_steps / 5; 0 = one launch.
+        Results do not depend on it."""
+        check(lib().smr_batch_set_rerank(self._h, int(steps)))
+
     def set_graph(self, use_graph: bool):
         check(lib().smr_batch_set_graph(self._h, 1 if use_graph else 0))
 

@@ -123,6 +123,7 @@ _SIGNATURES = [
     ("smr_batch_set_layer_persistent", c_int, [c_void_p, c_int, c_double]),
     ("smr_batch_layer_path", c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),
     ("smr_batch_set_order", c_int, [c_void_p, c_int]),
+    ("smr_batch_set_rerank", c_int, [c_void_p, c_int]),
     ("smr_server_create", c_int, [c_void_p, c_void_p, P(SmrServerConfig), P(c_void_p)]),
     ("smr_server_submit", c_int, [c_void_p, P(SmrTopology), P(c_double), P(ctypes.c_longlong)]),
     ("smr_server_poll", c_int, [c_void_p, ctypes.c_longlong, P(c_int), P(SmrRelaxResult), P(c_double),

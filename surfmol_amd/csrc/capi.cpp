@@ -998,7 +998,7 @@ extern "C" int smr_batch_get_positions(const smr_batch *b, double *apos) {
 namespace smr {
 
 int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch, bool reset_state,
-              double dt0, double alpha0, float *kernel_ms) {
+              double dt0, double alpha0, float *kernel_ms, LaunchSet *ls = nullptr) {
     smr_ctx *c = b->ctx;
     hipStream_t st = c->stream;
     if (reset_state) {
@@ -1025,7 +1025,9 @@ int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch,
     if (b->layer) A.cells = b->cells;
     A.steps_per_launch = steps_per_launch;
     A.mode = mode;
-    return launch_plans(c->ss, b->ls, b->plans, A, kernel_ms);
+    if (!ls) ls = &b->ls;
+    ls->use_graph = b->ls.use_graph;
+    return launch_plans(c->ss, *ls, b->plans, A, kernel_ms);
 }
 
 // FP64 energies of every system at its current positions (into state[].E)
@@ -1035,11 +1037,17 @@ int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch,
 // furthest from a minimum, which need the most FIRE steps), so that long
 // relaxations do not start in the tail of the persistent launch.  The order
 // affects only the schedule: every system's trajectory is its own.
+// (only where a size class has more systems than resident workgroups:
+// otherwise every system starts at once anyway)
+bool reorder_pays(const smr_batch *b) {
+    if (!b->energy_order || b->layer || b->nsys < 2) return false;
+    for (const auto &p : b->plans)
+        if (p.count > p.shape.grid) return true;
+    return false;
+}
+
 int order_by_energy(smr_batch *b) {
-    if (!b->energy_order || b->layer || b->nsys < 2) return SMR_OK;
-    bool any = false;
-    for (const auto &p : b->plans) any = any || p.count > p.shape.grid;
-    if (!any) return SMR_OK;   // every system starts at once anyway
+    if (!reorder_pays(b)) return SMR_OK;
     hipStream_t st = b->ctx->stream;
     hipError_t e = launch_energy64(b->tab, (const float4 *)b->pos.ptr, (SysState *)b->state.ptr,
                                    (const int *)b->work.ptr, b->nsys, st);
@@ -1397,8 +1405,24 @@ extern "C" int smr_relax_batched(smr_batch *b, const smr_fire_params *fire, int 
     rc = order_by_energy(b);
     if (rc) return rc;
     FireConst fc = make_fire_const(fire, max_steps, fconv);
-    rc = run_fused(b, KMODE_FIRE, fc, max_steps, true, fire->dt0, fire->alpha0, &b->last_kernel_ms);
-    if (rc) return rc;
+    const int rerank = b->rerank_steps < 0 ? max_steps / 5 : b->rerank_steps;
+    if (rerank >= 50 && rerank < max_steps && reorder_pays(b)) {
+        // two launches: every system takes up to `rerank` steps (the
+        // state, velocities and stored stencils persist as in the server's
+        // chunks), then the unfinished ones are ranked again by the energy of
+        // their current positions and continue; trajectories are unchanged
+        float k1 = 0.f, k2 = 0.f;
+        rc = run_fused(b, KMODE_FIRE, fc, rerank, true, fire->dt0, fire->alpha0, &k1);
+        if (rc) return rc;
+        rc = order_by_energy(b);
+        if (rc) return rc;
+        rc = run_fused(b, KMODE_FIRE, fc, max_steps, false, fire->dt0, fire->alpha0, &k2, &b->ls_rerank);
+        if (rc) return rc;
+        b->last_kernel_ms = k1 + k2;
+    } else {
+        rc = run_fused(b, KMODE_FIRE, fc, max_steps, true, fire->dt0, fire->alpha0, &b->last_kernel_ms);
+        if (rc) return rc;
+    }
     rc = eval_energies64(b);
     if (rc) return rc;
     return read_results(b, results);
@@ -1689,6 +1713,13 @@ extern "C" int smr_batch_set_order(smr_batch *b, int energy_order) {
         HIPCHK(hipSetDevice(b->ctx->device));
         HIPCHK(hipMemcpy(b->work.ptr, b->work_host.data(), sizeof(int) * b->work_host.size(), hipMemcpyHostToDevice));
     }
+    return SMR_OK;
+}
+
+extern "C" int smr_batch_set_rerank(smr_batch *b, int steps) {
+    if (!b) return fail(SMR_ERR_INVALID, "smr_batch_set_rerank: NULL argument");
+    if (steps < -1) return fail(SMR_ERR_INVALID, "smr_batch_set_rerank: steps must be >= -1");
+    b->rerank_steps = steps;
     return SMR_OK;
 }
 

@@ -193,10 +193,13 @@ struct smr_batch {
     std::vector<smr::ClassPlan> class_plans;   // one per size class (shapes chosen per class)
     std::vector<smr::ClassPlan> plans;         // launches: adjacent classes of one shape merged
     smr::LaunchSet ls;
+    smr::LaunchSet ls_rerank;        // the continuation launch of a re-ranked relaxation
     int tune_block = 0;              // 0: automatic
     int tune_cache = -1;             // -1: automatic
     int tune_pairs = -1;             // -1: automatic
     int energy_order = 1;            // 1: start systems of higher initial energy per atom first
+    int rerank_steps = -1;           // > 0: re-rank the unfinished systems after this many steps;
+                                     // -1: after max_steps / 5; 0: never
     std::vector<int> work_host;      // the work list as uploaded (class order, atom count inside a class)
     float last_kernel_ms = 0.f;
 };

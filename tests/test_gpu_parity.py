@@ -202,18 +202,21 @@ def test_gridff_cache(ctx, surf, tmp_path):
 
 def test_energy_order_changes_no_result(ctx, surf):
     """More systems than resident workgroups (2100 > 8 per CU x 256 CUs): the
-    energy-ordered start (default) and the batch order give bit-identical
-    per-system results; only the schedule differs."""
+    energy-ordered start (default: two launches, re-ranked after max_steps/5
+    steps), the same with one launch or another re-rank point, and the batch
+    order give bit-identical per-system results; only the schedule differs."""
     import surfmol_amd as S
     spec = surf["spec"]
     mol, X = ptcda_replicas(2100, spec["L"])
     fp = S.FireParams(**fire_params_internal())
     out = []
-    for order in (True, False):
+    for order, rerank in ((True, -1), (False, -1), (True, 0), (True, 150)):
         b = _batch(ctx, surf, mol, X)
         b.set_order(order)
+        b.set_rerank(rerank)
         res = b.relax_batched(fp, 400, 1e-3)
         out.append(([(r["steps"], r["status"], r["energy"]) for r in res], b.positions()))
         b.close()
-    assert out[0][0] == out[1][0]
-    assert np.array_equal(out[0][1], out[1][1])
+    for o in out[1:]:
+        assert o[0] == out[0][0]
+        assert np.array_equal(o[1], out[0][1])
