@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsurfmol_relax.so")
-SOURCES = ["relax_kernels.hip", "energy_kernels.hip", "gridff_kernels.hip", "server_kernels.hip", "layer_kernels.hip", "raff_kernels.hip", "plain_noalloc.hip", "raff.cpp", "raff_server.cpp", "host_raff.cpp", "capi.cpp", "launch_plan.cpp", "host_topology.cpp", "server.cpp"]
+SOURCES = ["relax_kernels.hip", "energy_kernels.hip", "order_kernels.hip", "gridff_kernels.hip", "server_kernels.hip", "layer_kernels.hip", "raff_kernels.hip", "plain_noalloc.hip", "raff.cpp", "raff_server.cpp", "host_raff.cpp", "capi.cpp", "launch_plan.cpp", "host_topology.cpp", "server.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SMR_OFFLOAD_ARCH", "gfx950")
 

@@ -927,6 +927,9 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
     refresh_shapes(b);
     b->work_host = wl;
     e = hipMemcpyAsync(b->work.ptr, wl.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = b->work_base.alloc(sizeof(int) * nsys);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->work_base.ptr, wl.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = b->hstate.alloc(sizeof(SysState) * nsys);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) { delete b; ctx_release(c); return hip_fail(e, "smr_batch_create: upload"); }
     if (opts->inter_cutoff > 0.0) {
@@ -935,6 +938,11 @@ extern "C" int smr_batch_create(smr_ctx *c, const smr_gridff *grid, int ntopo, c
     }
     *out = b;
     return SMR_OK;
+}
+
+smr_batch::~smr_batch() {
+    for (hipEvent_t &e : ev)
+        if (e) (void)hipEventDestroy(e);
 }
 
 extern "C" int smr_batch_destroy(smr_batch *b) {
@@ -997,20 +1005,7 @@ extern "C" int smr_batch_get_positions(const smr_batch *b, double *apos) {
 // ============================================================ kernels drivers
 namespace smr {
 
-int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch, bool reset_state,
-              double dt0, double alpha0, float *kernel_ms, LaunchSet *ls = nullptr) {
-    smr_ctx *c = b->ctx;
-    hipStream_t st = c->stream;
-    if (reset_state) {
-        std::vector<SysState> h(b->nsys);
-        for (auto &s : h) {
-            memset(&s, 0, sizeof(s));
-            s.dt = dt0;
-            s.alpha = alpha0;
-            s.status = SMR_SYS_RUNNING;
-        }
-        HIPCHK(hipMemcpyAsync(b->state.ptr, h.data(), sizeof(SysState) * h.size(), hipMemcpyHostToDevice, st));
-    }
+KernelArgs fused_args(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch) {
     KernelArgs A;
     memset(&A, 0, sizeof(A));
     A.tab = b->tab;
@@ -1025,9 +1020,14 @@ int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch,
     if (b->layer) A.cells = b->cells;
     A.steps_per_launch = steps_per_launch;
     A.mode = mode;
-    if (!ls) ls = &b->ls;
-    ls->use_graph = b->ls.use_graph;
-    return launch_plans(c->ss, *ls, b->plans, A, kernel_ms);
+    return A;
+}
+
+int run_fused(smr_batch *b, int mode, const FireConst &fc, int steps_per_launch, bool reset_state,
+              double dt0, double alpha0, float *kernel_ms) {
+    smr_ctx *c = b->ctx;
+    if (reset_state) HIPCHK(launch_reset_state((SysState *)b->state.ptr, b->nsys, dt0, alpha0, c->stream));
+    return launch_plans(c->ss, b->ls, b->plans, fused_args(b, mode, fc, steps_per_launch), kernel_ms);
 }
 
 // FP64 energies of every system at its current positions (into state[].E)
@@ -1049,30 +1049,18 @@ bool reorder_pays(const smr_batch *b) {
 int order_by_energy(smr_batch *b) {
     if (!reorder_pays(b)) return SMR_OK;
     hipStream_t st = b->ctx->stream;
+    // asynchronous on the batch's stream: FP64 energies of the current
+    // positions, then the ranked work list (order_kernels.hip)
     hipError_t e = launch_energy64(b->tab, (const float4 *)b->pos.ptr, (SysState *)b->state.ptr,
-                                   (const int *)b->work.ptr, b->nsys, st);
-    std::vector<SysState> h(b->nsys);
-    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), b->state.ptr, sizeof(SysState) * h.size(), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+                                   (const int *)b->work_base.ptr, b->nsys, st);
+    RankSlices rs;
+    rs.n = (int)b->plans.size();
+    for (int q = 0; q < rs.n; q++) rs.first[q] = b->plans[q].first;
+    rs.first[rs.n] = b->nsys;
+    if (e == hipSuccess)
+        e = rank_work_list(b->tab, (const SysState *)b->state.ptr, (const int *)b->work_base.ptr, b->nsys, rs, b->rank,
+                           (int *)b->work.ptr, st);
     if (e != hipSuccess) return hip_fail(e, "order_by_energy");
-    std::vector<double> key(b->nsys);
-    for (int s = 0; s < b->nsys; s++) {
-        double E = 0.0;
-        for (int q = 0; q < 6; q++) E += h[s].E[q];
-        const int n = b->atom_off[s + 1] - b->atom_off[s];
-        key[s] = std::isfinite(E) ? E / n : 1e300;
-    }
-    // larger systems first (the class's own order), then higher energy per atom
-    std::vector<int> w = b->work_host;
-    auto natoms = [&](int s) { return b->atom_off[s + 1] - b->atom_off[s]; };
-    for (const auto &p : b->plans)
-        std::stable_sort(w.begin() + p.first, w.begin() + p.first + p.count, [&](int x, int y) {
-            if (natoms(x) != natoms(y)) return natoms(x) > natoms(y);
-            return key[x] > key[y];
-        });
-    e = hipMemcpyAsync(b->work.ptr, w.data(), sizeof(int) * w.size(), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(e, "order_by_energy: work list");
     return SMR_OK;
 }
 
@@ -1087,10 +1075,8 @@ int eval_energies64(smr_batch *b) {
     return SMR_OK;
 }
 
-int read_results(smr_batch *b, smr_relax_result *res) {
-    std::vector<SysState> h(b->nsys);
-    HIPCHK(hipMemcpy(h.data(), b->state.ptr, sizeof(SysState) * h.size(), hipMemcpyDeviceToHost));
-    for (int s = 0; s < b->nsys; s++) {
+void results_from_states(const SysState *h, int nsys, smr_relax_result *res) {
+    for (int s = 0; s < nsys; s++) {
         res[s].steps = h[s].steps;
         res[s].status = h[s].status;
         res[s].fmax = h[s].fmax;
@@ -1098,6 +1084,12 @@ int read_results(smr_batch *b, smr_relax_result *res) {
         for (int q = 0; q < 6; q++) { res[s].terms[q] = h[s].E[q]; E += h[s].E[q]; }
         res[s].energy = E;
     }
+}
+
+int read_results(smr_batch *b, smr_relax_result *res) {
+    std::vector<SysState> h(b->nsys);
+    HIPCHK(hipMemcpy(h.data(), b->state.ptr, sizeof(SysState) * h.size(), hipMemcpyDeviceToHost));
+    results_from_states(h.data(), b->nsys, res);
     return SMR_OK;
 }
 
@@ -1401,31 +1393,49 @@ extern "C" int smr_relax_batched(smr_batch *b, const smr_fire_params *fire, int 
     if (max_steps <= 0 || !(fconv > 0.0)) return fail(SMR_ERR_INVALID, "smr_relax_batched: need max_steps > 0, fconv > 0");
     HIPCHK(hipSetDevice(b->ctx->device));
     if (b->layer) return layer_relax(b, fire, max_steps, fconv, results);
-    HIPCHK(hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, b->ctx->stream));
+    // Everything up to the results' readback is issued without waiting:
+    // the start order, the launches and the final FP64 energies run back to
+    // back on the batch's stream.
+    hipStream_t st = b->ctx->stream;
+    HIPCHK(hipMemsetAsync(b->vel.ptr, 0, sizeof(float4) * b->total_atoms, st));
+    HIPCHK(launch_reset_state((SysState *)b->state.ptr, b->nsys, fire->dt0, fire->alpha0, st));
     rc = order_by_energy(b);
     if (rc) return rc;
+    for (hipEvent_t &e : b->ev)
+        if (!e) HIPCHK(hipEventCreate(&e));
     FireConst fc = make_fire_const(fire, max_steps, fconv);
     const int rerank = b->rerank_steps < 0 ? max_steps / 5 : b->rerank_steps;
-    if (rerank >= 50 && rerank < max_steps && reorder_pays(b)) {
-        // two launches: every system takes up to `rerank` steps (the
-        // state, velocities and stored stencils persist as in the server's
-        // chunks), then the unfinished ones are ranked again by the energy of
-        // their current positions and continue; trajectories are unchanged
-        float k1 = 0.f, k2 = 0.f;
-        rc = run_fused(b, KMODE_FIRE, fc, rerank, true, fire->dt0, fire->alpha0, &k1);
+    const bool two = rerank >= 50 && rerank < max_steps && reorder_pays(b);
+    b->ls_rerank.use_graph = b->ls.use_graph;
+    if (!b->plans.empty()) {
+        // two launches when re-ranking: every system takes up to `rerank`
+        // steps (the state, velocities and stored stencils persist as in the
+        // server's chunks), then the unfinished ones are ranked again by the
+        // energy of their current positions and continue; trajectories are
+        // unchanged
+        rc = launch_plans_async(b->ctx->ss, b->ls, b->plans, fused_args(b, KMODE_FIRE, fc, two ? rerank : max_steps),
+                                b->ev[0], b->ev[1]);
         if (rc) return rc;
-        rc = order_by_energy(b);
-        if (rc) return rc;
-        rc = run_fused(b, KMODE_FIRE, fc, max_steps, false, fire->dt0, fire->alpha0, &k2, &b->ls_rerank);
-        if (rc) return rc;
-        b->last_kernel_ms = k1 + k2;
-    } else {
-        rc = run_fused(b, KMODE_FIRE, fc, max_steps, true, fire->dt0, fire->alpha0, &b->last_kernel_ms);
-        if (rc) return rc;
+        if (two) {
+            rc = order_by_energy(b);
+            if (rc) return rc;
+            rc = launch_plans_async(b->ctx->ss, b->ls_rerank, b->plans, fused_args(b, KMODE_FIRE, fc, max_steps),
+                                    b->ev[2], b->ev[3]);
+            if (rc) return rc;
+        }
     }
-    rc = eval_energies64(b);
-    if (rc) return rc;
-    return read_results(b, results);
+    HIPCHK(launch_energy64(b->tab, (const float4 *)b->pos.ptr, (SysState *)b->state.ptr, (const int *)b->work.ptr,
+                           b->nsys, st));
+    HIPCHK(hipMemcpyAsync(b->hstate.ptr, b->state.ptr, sizeof(SysState) * b->nsys, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float k1 = 0.f, k2 = 0.f;
+    if (!b->plans.empty()) {
+        HIPCHK(hipEventElapsedTime(&k1, b->ev[0], b->ev[1]));
+        if (two) HIPCHK(hipEventElapsedTime(&k2, b->ev[2], b->ev[3]));
+    }
+    b->last_kernel_ms = k1 + k2;
+    results_from_states((const SysState *)b->hstate.ptr, b->nsys, results);
+    return SMR_OK;
 }
 
 extern "C" int smr_run_md(smr_batch *b, int niter, double dt, double fconv, double flim, double damping, int *steps) {

@@ -242,12 +242,8 @@ hipError_t issue_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<Cla
     return hipSuccess;
 }
 
-int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
-                 float *kernel_ms) {
-    if (plans.empty()) {
-        if (kernel_ms) *kernel_ms = 0.f;
-        return SMR_OK;
-    }
+int launch_plans_async(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
+                       hipEvent_t ev_begin, hipEvent_t ev_end) {
     hipError_t e;
     std::vector<KernelArgs> args;
     int rc = prepare_plans(ls, plans, base, args);
@@ -277,20 +273,32 @@ int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan
             ls.key = key;
         }
     }
+    e = ev_begin ? hipEventRecord(ev_begin, ss.main) : hipSuccess;
+    if (e == hipSuccess) e = ls.use_graph ? hipGraphLaunch(ls.gexec, ss.main) : issue();
+    if (e == hipSuccess && ev_end) e = hipEventRecord(ev_end, ss.main);
+    if (e != hipSuccess) return hip_fail(e, "k_relax launches");
+    return SMR_OK;
+}
+
+int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
+                 float *kernel_ms) {
+    if (plans.empty()) {
+        if (kernel_ms) *kernel_ms = 0.f;
+        return SMR_OK;
+    }
+    hipError_t e;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (kernel_ms) {
         if ((e = hipEventCreate(&e0)) != hipSuccess) return hip_fail(e, "hipEventCreate");
         if ((e = hipEventCreate(&e1)) != hipSuccess) { (void)hipEventDestroy(e0); return hip_fail(e, "hipEventCreate"); }
-        e = hipEventRecord(e0, ss.main);
-    } else {
-        e = hipSuccess;
     }
-    if (e == hipSuccess) e = ls.use_graph ? hipGraphLaunch(ls.gexec, ss.main) : issue();
-    if (e == hipSuccess && kernel_ms) e = hipEventRecord(e1, ss.main);
-    if (e == hipSuccess) e = hipStreamSynchronize(ss.main);
-    if (e == hipSuccess && kernel_ms) e = hipEventElapsedTime(kernel_ms, e0, e1);
+    int rc = launch_plans_async(ss, ls, plans, base, e0, e1);
+    e = hipSuccess;
+    if (rc == SMR_OK) e = hipStreamSynchronize(ss.main);
+    if (rc == SMR_OK && e == hipSuccess && kernel_ms) e = hipEventElapsedTime(kernel_ms, e0, e1);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
     if (e != hipSuccess) return hip_fail(e, "k_relax launches");
     return SMR_OK;
 }

@@ -126,12 +126,49 @@ struct LaunchSet {
 };
 int launch_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
                  float *kernel_ms);
+// the same without waiting: ev_begin / ev_end (may be null) are recorded on
+// ss.main around the launches
+int launch_plans_async(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
+                       hipEvent_t ev_begin, hipEvent_t ev_end);
 // the two halves of launch_plans, for callers that build their own sequence
 int prepare_plans(LaunchSet &ls, const std::vector<ClassPlan> &plans, const KernelArgs &base,
                   std::vector<KernelArgs> &args);
 hipError_t issue_plans(const StreamSet &ss, LaunchSet &ls, const std::vector<ClassPlan> &plans,
                        const std::vector<KernelArgs> &args,
                        bool zero_counters = true);
+
+// ---- device-side start order (order_kernels.hip) ----
+// launch slices of the work list: slice k covers positions [first[k], first[k+1])
+struct RankSlices {
+    int n = 0;
+    int first[NCLASS + 1] = {0};
+};
+struct RankBuffers {
+    DevBuf keys, vals, tmp;
+    size_t tmp_bytes = 0;
+    int n = 0;
+};
+hipError_t launch_reset_state(SysState *state, int n, double dt0, double alpha0, hipStream_t st);
+// work[0..n) = work_base ranked by (slice, atom count desc, FP64 energy per
+// atom of state[].E desc), stable; asynchronous on st
+hipError_t rank_work_list(const DevTables &T, const SysState *state, const int *work_base, int n, const RankSlices &rs,
+                          RankBuffers &rb, int *work, hipStream_t st);
+
+// page-locked host memory (released with the owner)
+struct HostPinned {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    HostPinned() = default;
+    HostPinned(const HostPinned &) = delete;
+    HostPinned &operator=(const HostPinned &) = delete;
+    ~HostPinned() { if (ptr) (void)hipHostFree(ptr); }
+    hipError_t alloc(size_t nbytes) {
+        if (ptr) { (void)hipHostFree(ptr); ptr = nullptr; bytes = 0; }
+        hipError_t e = hipHostMalloc(&ptr, nbytes, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = nbytes; else ptr = nullptr;
+        return e;
+    }
+};
 
 int pack_topology(const smr_topology &t, int ti, double alpha, bool have_grid, HostPools &P, TopoDesc &td);
 hipError_t upload_pools(HostPools &P, DevPools &D, hipStream_t st);
@@ -194,6 +231,11 @@ struct smr_batch {
     std::vector<smr::ClassPlan> plans;         // launches: adjacent classes of one shape merged
     smr::LaunchSet ls;
     smr::LaunchSet ls_rerank;        // the continuation launch of a re-ranked relaxation
+    smr::DevBuf work_base;           // work_host on the device: the ranking's input
+    smr::RankBuffers rank;           // keys, values and sort scratch of the ranking
+    smr::HostPinned hstate;          // SysState [nsys]: results read back here
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // launch timing (created on first use)
+    ~smr_batch();
     int tune_block = 0;              // 0: automatic
     int tune_cache = -1;             // -1: automatic
     int tune_pairs = -1;             // -1: automatic
