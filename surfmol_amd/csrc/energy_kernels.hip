@@ -129,25 +129,29 @@ __global__ __launch_bounds__(256) void k_energy64(DevTables T, const float4 *__r
         const double c2x = c * c - sn * sn;
         e[3] += p.x * (p.y + p.z * c + p.w * c2x);
     }
-    // pairs: every unordered non-bonded pair once, counted at both ends
+    // pairs: every unordered non-bonded pair once, counted at both ends.
+    // Pair p of the block is atom i with i + d (mod N) on sweep d = p / N + 1
+    // (d = 1..N/2; the last sweep of an even N takes i < N/2 only), so the
+    // block's threads share the N(N-1)/2 pairs evenly.
     if (T.use_pairs) {
-        for (int i = tid; i < N; i += blockDim.x) {
+        const int nd = N >> 1;
+        double acc = 0.0;
+        for (int p = tid; p < N * nd; p += blockDim.x) {
+            const int d = p / N + 1, i = p - (d - 1) * N;
+            if ((N & 1) == 0 && d == nd && i >= nd) continue;
+            int j = i + d;
+            if (j >= N) j -= N;
             const int4 n = NG[i];
-            const d3 pi = ld(P, i);
-            const double4 ri = T.req64[td.atom_off + i];
-            double acc = 0.0;
-            for (int j = i + 1; j < N; j++) {
-                if (j == n.x || j == n.y || j == n.z || j == n.w) continue;
-                const double4 rj = T.req64[td.atom_off + j];
-                const d3 dp = sub(ld(P, j), pi);
-                const double r2 = dot(dp, dp);
-                const double R0 = ri.x + rj.x, E0 = ri.y * rj.y, kQ = ri.z * rj.z;
-                const double u2 = R0 * R0 / r2;
-                const double u6 = u2 * u2 * u2;
-                acc += (u6 - 2.0) * u6 * E0 + kQ / sqrt(r2 + T.r2damp64);
-            }
-            e[4] += 2.0 * acc;
+            if (j == n.x || j == n.y || j == n.z || j == n.w) continue;
+            const double4 ri = T.req64[td.atom_off + i], rj = T.req64[td.atom_off + j];
+            const d3 dp = sub(ld(P, j), ld(P, i));
+            const double r2 = dot(dp, dp);
+            const double R0 = ri.x + rj.x, E0 = ri.y * rj.y, kQ = ri.z * rj.z;
+            const double u2 = R0 * R0 / r2;
+            const double u6 = u2 * u2 * u2;
+            acc += (u6 - 2.0) * u6 * E0 + kQ / sqrt(r2 + T.r2damp64);
         }
+        e[4] += 2.0 * acc;
     }
     // surface: E = sum_stencil Bx By Bz (PLQH . C)
     if (T.grid) {
