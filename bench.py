@@ -55,9 +55,9 @@ def falg_per_system_step(nb, na, nd, ni, npairs, n):
 
 
 def measured_traffic(args):
-    """HBM bytes per k_relax launch from the committed rocprofv3 counter runs
-    (FETCH_SIZE and WRITE_SIZE, copy-calibrated), for the workload they were
-    measured on; (None, None) otherwise.  Counters need their own profiler
+    """HBM bytes of the k_relax launches of one relaxation from the committed
+    rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, copy-calibrated), for
+    the workload they were measured on; (None, None) otherwise.  Counters need their own profiler
     run, so this figure cannot be taken inside the timed loop."""
     if args.workload != "uniform" or args.nsys != 4096 or args.max_steps != 2000:
         return None, None
@@ -66,7 +66,7 @@ def measured_traffic(args):
         return None, None
     with open(path) as f:
         rec = json.load(f)
-    return float(rec["traffic_bytes_per_launch"]), rec["source"]
+    return float(rec["traffic_bytes_per_relaxation"]), rec["source"]
 
 
 def grid_for(ctx, S, W, L, zlen=20.0, step=0.1, alpha=1.5, rdamp=1.0, npbc=2):
@@ -373,6 +373,7 @@ def run_workload(args, S, W, workload, steps, warmup, cpu_seconds, want_plain, w
             "traffic": traffic,
             "traffic_source": traffic_src,
             "kernel": "k_relax",
+            "kernel_ms_note": "HIP-event time of the relaxation's k_relax launches (two when the schedule is re-ranked)",
             "flop_per_relaxation": flop_launch,
             "kernel_ms_avg": round(1e3 * kavg_s, 3),
             "grid_lookup_GBs": round(bytes_launch / kavg_s / 1e9, 1),
