@@ -15,6 +15,8 @@
 //               opencl/UFF.cl:141-1011 driven by a host loop).
 //
 // Per-term math is in smr_device.h and shared by both paths.
+// this translation unit holds the pair sweep's schedule table (kSched)
+#define SMR_SWEEP_SCHEDULE_TABLE
 #include "smr_device.h"
 #include "smr_kernels.h"
 #include "layer_persist.h"
@@ -146,8 +148,7 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
     uint2 *s_dih = reinterpret_cast<uint2 *>(s_next + 4);                // packed slots
     uint2 *s_inv = s_dih + tn_d;
     uint2 *s_dihd = s_inv + tn_i;                                       // piece destinations
-    uint2 *s_invd = s_dihd + tn_d;
-    uint2 *s_angd = s_invd + tn_i;
+    uint2 *s_angd = s_dihd + tn_d;
     unsigned *s_ang = reinterpret_cast<unsigned *>(s_angd + tn_a);
     float *s_fx = reinterpret_cast<float *>(s_ang + tn_a);
     const uint2 *t_dih, *t_inv, *t_dihd, *t_invd, *t_angd;
@@ -230,11 +231,59 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                 my_v[k] = make_float3(0.f, 0.f, 0.f);
             }
         }
-        // ROT: rows 0..N/2 again after row N-1 (partner of sweep d = row i + d)
-        if (ROT && tid < N && tid <= (N >> 1)) {
-            s_pos[N + tid] = s_pos[tid];
-            s_req[N + tid] = s_req[tid];
+        // ROT: the pair sweep's lane schedule (sweep_schedules.h) for up to
+        // kSchedMaxAtoms atoms: the lane takes atom s_av (-1: none) and meets
+        // rows lane+1 .. lane+s_T; bit t-1 of s_skip drops round t's pair (not
+        // the meeting that sums it, or a 1-2 neighbour)
+        // s_src: the atom's further pair-force sums (one byte each), s_cp: the
+        // atoms copied into rows N + lane and N + lane + 64 (bits 0-7, 8-15;
+        // bit 16: the second row exists)
+        [[maybe_unused]] int s_av = -1, s_T = 0;
+        [[maybe_unused]] unsigned s_skip = ~0u, s_src = 0xffffffffu, s_cp = 0u;
+        if constexpr (ROT) {
+            if (N <= kSchedMaxAtoms) {
+                const SweepSchedule &SS = kSched[N];
+                s_T = SS.T;
+                s_av = SS.a[lane];
+                s_skip = ~SS.keep[lane];
+                if (tid < N) s_src = SS.src[lane];
+                const int x1 = N + lane, x2 = x1 + 64;
+                const int j1 = SS.g[x1 & 63], j2 = SS.g[x2 & 63];
+                s_cp = (unsigned)(j1 >= 0 ? j1 : 0) | ((unsigned)(j2 >= 0 ? j2 : 0) << 8) | (x2 < 64 + s_T ? 1u << 16 : 0u);
+                if (s_av >= 0) {
+                    const int4 ng = s_av == tid ? my_ng[0] : fa->tab.ng[td.atom_off + s_av];
+                    for (int t = 1; t <= s_T; t++) {
+                        const int j = SS.g[(lane + t) & 63];
+                        if (j >= 0 && (j == ng.x || j == ng.y || j == ng.z || j == ng.w)) s_skip |= 1u << (t - 1);
+                    }
+                }
+            }
         }
+        // ROT: copies of atoms in the rows after row N-1, so that the partner
+        // of a lane's round t is row lane + t: the schedule's rows (an empty
+        // slot holds atom 0, whose meetings there are all dropped), or rows
+        // 0..N/2 again for more atoms.  Called after the atom rows changed
+        // (staging: with the REQ rows; every move: positions).
+        auto copy_rows = [&](bool with_req) {
+            if constexpr (ROT) {
+                if (N <= kSchedMaxAtoms) {
+                    block_sync<1>();
+                    const int x1 = N + lane, j1 = (int)(s_cp & 0xffu), j2 = (int)((s_cp >> 8) & 0xffu);
+                    if (x1 < 64 + s_T) {
+                        s_pos[x1] = s_pos[j1];
+                        if (with_req) s_req[x1] = s_req[j1];
+                    }
+                    if (s_cp >> 16) {
+                        s_pos[x1 + 64] = s_pos[j2];
+                        if (with_req) s_req[x1 + 64] = s_req[j2];
+                    }
+                } else if (tid <= (N >> 1)) {
+                    s_pos[N + tid] = s_pos[tid];
+                    if (with_req) s_req[N + tid] = s_req[tid];
+                }
+            }
+        };
+        copy_rows(true);
         // ROT: the gather of each step follows a plan fixed per system.  An
         // atom's force pieces are summed in rounds of eight; the wave runs
         // as many rounds as its heaviest atom needs.  Lanes without an atom
@@ -279,9 +328,9 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
             }
             for (int q = tid; q < td.ninv; q += BLOCK) {
                 s_inv[q] = fa->tab.inv_ng[td.inv_off + q];
-                s_invd[q] = fa->tab.inv_dst[td.inv_off + q];
             }
-            t_ang = s_ang; t_dih = s_dih; t_inv = s_inv; t_angd = s_angd; t_dihd = s_dihd; t_invd = s_invd;
+            t_ang = s_ang; t_dih = s_dih; t_inv = s_inv; t_angd = s_angd; t_dihd = s_dihd;
+            t_invd = fa->tab.inv_dst + td.inv_off;   // read through the caches
         } else {
             t_ang = fa->tab.ang_ng + td.ang_off; t_dih = fa->tab.dih_ng + td.dih_off; t_inv = fa->tab.inv_ng + td.inv_off;
             t_angd = fa->tab.ang_dst + td.ang_off; t_dihd = fa->tab.dih_dst + td.dih_off; t_invd = fa->tab.inv_dst + td.inv_off;
@@ -565,6 +614,7 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
             float4 apar = tid < td.nangles ? T.ang_par[td.ang_off + tid] : zero4;
             float4 dpar = tid < td.ndih ? T.dih_par[td.dih_off + tid] : zero4;
             float4 ipar = tid < td.ninv ? T.inv_par[td.inv_off + tid] : zero4;
+            uint2 idst = tid < td.ninv ? t_invd[tid] : make_uint2(0u, 0u);
             // ---------------- phase A: bonds + hneigh ----------------
             if (!ROT && A.pair_mode)
                 for (int w = 0; w < NWP; w++)
@@ -618,9 +668,12 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                 }
                 for (int v = tid; v < td.ninv; v += BLOCK) {
                     const uint2 g = t_inv[v];
-                    const uint2 dd = t_invd[v];
+                    const uint2 dd = idst;
                     const float4 par = ipar;
-                    if (v + BLOCK < td.ninv) ipar = T.inv_par[td.inv_off + v + BLOCK];
+                    if (v + BLOCK < td.ninv) {
+                        ipar = T.inv_par[td.inv_off + v + BLOCK];
+                        idst = t_invd[v + BLOCK];
+                    }
                     float3 f1, f2, f3v, f4;
                     (void)inversion_term(s_h[g.x & 0xffffu], s_h[g.x >> 16], s_h[g.y], par, f1, f2, f3v, f4);
                     put(dd.x & 0xffffu, f1); put(dd.x >> 16, f2); put(dd.y & 0xffffu, f3v); put(dd.y >> 16, f4);
@@ -639,36 +692,42 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
             for (int k = 0; k < APT; k++) pf[k] = make_float3(0.f, 0.f, 0.f);
             float3 rr = make_float3(0.f, 0.f, 0.f);   // ROT: reaction forces on the own atom
             if (ROT && A.pair_mode && T.use_pairs) {
-                // The same sweep with the reaction sum of atom j = (i+d) mod N
-                // held by lane i: on sweep d+1 atom (i+d+1) mod N is the
-                // target of lane i+1 on sweep d, so the accumulators rotate
-                // by one lane per sweep.
-                //  - N + N/2 <= 64: the accumulators rotate over all 64 lanes
-                //    (DPP wave_rol:1, no fix-up).  The accumulator that starts
-                //    in lane a collects the reactions of "virtual atom" a+1;
-                //    own lanes reach virtual atoms i+d <= N + N/2 - 1 < 64, so
-                //    nothing wraps, and atom k's reaction is the sum of its
-                //    virtual atoms k (pairs i < k) and k+N (pairs i > k, only
-                //    for k < N/2).  After the N/2 - 1 rotations accumulator a
-                //    sits in lane (a - N/2 + 1) mod 64.
-                //  - larger N: the rotation wraps at N (wave_rot1), and lane
-                //    (i - N/2) mod N ends with atom i's sum.
+                // The accumulators of the reaction sums rotate by one lane per
+                // round (DPP): on round t lane v meets row v + t, and the
+                // accumulator of row x, which starts in lane x - 1, is then in
+                // lane v = x - t.
+                //  - up to kSchedMaxAtoms atoms: the lane schedule
+                //    (sweep_schedules.h).  Lanes and rows below N are the
+                //    atoms, the rest copies, placed so that every pair of
+                //    atoms meets within T rounds (PTCDA, 38 atoms: 13 rounds
+                //    over all 64 lanes where rows 0..N/2 repeated after the
+                //    atoms need 19 with 26 lanes idle).  After the T-1
+                //    rotations row x's accumulator is in lane x - T; an atom's
+                //    pair force is its lane's own sum plus its rows'
+                //    accumulators and its copies' own sums (the schedule's
+                //    source list, in a fixed order).
+                //  - more atoms: the rotation wraps at N (wave_rot1) over
+                //    rounds d = 1..N/2, and lane (i - N/2) mod N ends with atom
+                //    i's sum.
                 // The loop body is branch-free: the pair force is computed in
-                // every lane and zeroed where the lane has no atom, the pair
-                // is bonded (bit d-1 of `skip`: the 1-2 exclusions of the own
-                // atom's sweep) or, on the last sweep of an even N, the pair
-                // belongs to the lower half.
-                const bool own = tid < N;
+                // every lane and zeroed where bit t-1 of `skip` is set (no
+                // atom, another meeting sums the pair, a 1-2 neighbour, or the
+                // upper half of an even N's last round).
+                const bool sch = N <= kSchedMaxAtoms;
                 const int nd = N >> 1;
-                const bool even = (N & 1) == 0;
-                const float4 pi4 = own ? s_pos[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
-                // the own REQ row with the pair force's constant factors
+                const int nr = sch ? s_T : nd;
+                const bool own = tid < N;
+                const int av = sch ? s_av : (own ? tid : -1);
+                const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 pi4 = av >= 0 ? s_pos[av] : zero;
+                // the lane's REQ row with the pair force's constant factors
                 // folded in: E0 = -12 sqrtEi sqrtEj and kQ = -qi qj
-                const float4 ri0 = own ? s_req[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 ri0 = av >= 0 ? s_req[av] : zero;
                 const float4 ri = make_float4(ri0.x, -12.f * ri0.y, -ri0.z, 0.f);
-                // bit d-1: sweep d is skipped (nd <= 32)
-                unsigned skip = own ? 0u : ~0u;
-                {
+                unsigned skip = s_skip;
+                if (!sch) {
+                    const bool even = (N & 1) == 0;
+                    skip = own ? 0u : ~0u;
                     const int ngs[4] = {my_ng[0].x, my_ng[0].y, my_ng[0].z, my_ng[0].w};
 #pragma unroll
                     for (int sl = 0; sl < 4; sl++) {
@@ -680,11 +739,12 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                 }
                 KahanF3 pacc;
                 pacc.init(make_float3(0.f, 0.f, 0.f));
-                // partner of sweep d: row (own ? tid : 0) + d, d <= N/2 (the
-                // rows after N-1 repeat rows 0..N/2: no wrap)
-                const float4 *prow = s_pos + (own ? tid : 0);
-                const float4 *rrow = s_req + (own ? tid : 0);
-                // one sweep: the pair's radial factor (zero where skipped)
+                // partner of round d: row base + d (the rows after N-1 hold
+                // copies: no wrap)
+                const int base = sch ? lane : (own ? tid : 0);
+                const float4 *prow = s_pos + base;
+                const float4 *rrow = s_req + base;
+                // one round: the pair's radial factor (zero where skipped)
                 // scales dp into the own sum and, negated, into the rotating
                 // reaction sum
                 auto pair = [&](int d, float4 pc, float4 rc, float3 &bs) {
@@ -698,12 +758,12 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     rr.y = __builtin_fmaf(-dp.y, fr, rr.y);
                     rr.z = __builtin_fmaf(-dp.z, fr, rr.z);
                 };
-                // Blocks of four sweeps: the four partner rows are read first,
+                // Blocks of four rounds: the four partner rows are read first,
                 // then the pairs; each block's own-atom sum goes into the
-                // compensated sum.  The last 1-3 sweeps form a block of their own.
+                // compensated sum.  The last 1-3 rounds form a block of their own.
                 auto sweep = [&](auto rotate) {
                     int d0 = 1;
-                    for (; d0 + 3 <= nd; d0 += 4) {
+                    for (; d0 + 3 <= nr; d0 += 4) {
                         float4 pc[4], rc[4];
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
@@ -718,9 +778,9 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                         }
                         pacc.add(bs);
                     }
-                    if (d0 <= nd) {
+                    if (d0 <= nr) {
                         float3 bs = make_float3(0.f, 0.f, 0.f);
-                        for (; d0 <= nd; d0++) {
+                        for (; d0 <= nr; d0++) {
                             const float4 pc = prow[d0], rc = rrow[d0];
                             if (d0 > 1) rotate();
                             pair(d0, pc, rc, bs);
@@ -728,16 +788,30 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                         pacc.add(bs);
                     }
                 };
-                if (N + nd <= 64) {
+                if (sch) {
                     sweep([&]() {
                         rr.x = wave_rol1(rr.x);
                         rr.y = wave_rol1(rr.y);
                         rr.z = wave_rol1(rr.z);
                     });
-                    const int s1 = (tid - nd) & 63, s2 = (tid + N - nd) & 63;
-                    const float3 r1 = make_float3(__shfl(rr.x, s1, 64), __shfl(rr.y, s1, 64), __shfl(rr.z, s1, 64));
-                    const float3 r2 = make_float3(__shfl(rr.x, s2, 64), __shfl(rr.y, s2, 64), __shfl(rr.z, s2, 64));
-                    rr = tid < nd ? r1 + r2 : r1;
+                    // the atom's further sums: one byte each (lane << 1 | kind;
+                    // kind 1: that lane's accumulator, 0: its own sum), 0xff
+                    // ends the list
+                    const float3 pv = pacc.value();
+                    const unsigned sw = s_src;
+                    float3 ext = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const unsigned b = (sw >> (8 * k)) & 0xffu;
+                        if (__builtin_amdgcn_ballot_w64(b != 0xffu) == 0ull) break;
+                        const int sl = (int)(b >> 1) & 63;
+                        const float3 xo = make_float3(__shfl(pv.x, sl, 64), __shfl(pv.y, sl, 64), __shfl(pv.z, sl, 64));
+                        const float3 xa = make_float3(__shfl(rr.x, sl, 64), __shfl(rr.y, sl, 64), __shfl(rr.z, sl, 64));
+                        const float3 x = (b & 1u) ? xa : xo;
+                        if (b != 0xffu) ext = ext + x;
+                    }
+                    pf[0] = pv;
+                    rr = ext;
                 } else {
                     sweep([&]() {
                         rr.x = wave_rot1(rr.x, N - 1, lane);
@@ -749,8 +823,8 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     rr.x = __shfl(rr.x, src, 64);
                     rr.y = __shfl(rr.y, src, 64);
                     rr.z = __shfl(rr.z, src, 64);
+                    pf[0] = pacc.value();
                 }
-                pf[0] = pacc.value();
             } else if (!ROT && A.pair_mode && T.use_pairs) {
                 float4 *fpw = s_fp + wid * A.lds_atoms;
                 const int nd = N >> 1;
@@ -1153,8 +1227,8 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     my_v[0] = v2;
                     const float4 p4 = make_float4(p2.x, p2.y, p2.z, s_pos[tid].w);
                     s_pos[tid] = p4;
-                    if (tid <= (N >> 1)) s_pos[N + tid] = p4;
                 }
+                copy_rows(false);
                 kstep++;
                 step_in_launch++;
                 if (fs.status != SMR_SYS_RUNNING || step_in_launch >= nsteps) {
@@ -1302,9 +1376,9 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     p.y += my_v[k].y * dt_move;
                     p.z += my_v[k].z * dt_move;
                     s_pos[i] = p;
-                    if (ROT && i <= (N >> 1)) s_pos[N + i] = p;
                 }
             }
+            copy_rows(false);
             if (A.mode == KMODE_MD && p_vf < 0.0) {
 #pragma unroll
                 for (int k = 0; k < APT; k++) my_v[k] = make_float3(0.f, 0.f, 0.f);

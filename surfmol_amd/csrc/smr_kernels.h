@@ -5,6 +5,7 @@
 #include "../../include/surfmol_relax.h"
 #include "../../include/surfmol_raff.h"
 #include "smr_device.h"
+#include "sweep_schedules.h"
 
 namespace smr {
 
@@ -130,13 +131,17 @@ inline int relax_lds_atoms(int block, int max_atoms) { return max_atoms > 16 * (
 constexpr int kPieceTail = 8;
 
 // Rows of the position and REQ arrays: one wave with one atom per thread
-// (the register-rotation pair sweep) keeps a copy of rows 0..lds_atoms/2
-// after the last one, so that the partner of sweep d is row i + d without a
-// wrap; its reaction forces stay in registers, so it has no per-wave
-// reaction buffer.
+// (the register-rotation pair sweep) keeps copies of atoms after its own
+// rows, so that the partner row of round t of lane v is row v + t without a
+// wrap: for up to kSchedMaxAtoms atoms the rows 0..63+T of its schedule
+// (sweep_schedules.h), for more atoms rows 0..N/2 again after row N-1; its
+// reaction forces stay in registers, so it has no per-wave reaction buffer.
 __host__ __device__ inline bool relax_rot_shape(int block, int apt) { return block == 64 && apt == 1; }
 __host__ __device__ inline int relax_lds_rows(int block, int apt, int lds_atoms) {
-    return relax_rot_shape(block, apt) ? lds_atoms + lds_atoms / 2 + 1 : lds_atoms;
+    if (!relax_rot_shape(block, apt)) return lds_atoms;
+    const int sched = 64 + kSchedTmax[lds_atoms < kSchedMaxAtoms ? lds_atoms : kSchedMaxAtoms];
+    const int wrap = lds_atoms + lds_atoms / 2 + 1;
+    return sched > wrap ? sched : wrap;
 }
 
 inline size_t relax_lds_bytes(int block, int apt, int lds_atoms, int nf_max, int na, int nd, int ni, bool fint_lds,
@@ -146,8 +151,9 @@ inline size_t relax_lds_bytes(int block, int apt, int lds_atoms, int nf_max, int
     const int fp = relax_rot_shape(block, apt) ? 0 : (pair_mode ? nw : 1);
     size_t b = (size_t)16 * (2 * rows + (size_t)lds_atoms * (4 + fp)) + (size_t)nw * 8 * 8 + 16;
     const bool tables = apt < 4;
-    // slots (8, 8, 4 B) and piece destinations (8 B) per dihedral, inversion, angle
-    if (tables) b += (size_t)nd * 16 + (size_t)ni * 16 + (size_t)na * 12;
+    // slots (8, 8, 4 B) per dihedral, inversion, angle and piece destinations
+    // (8 B) per dihedral and angle (the inversions' are read through the caches)
+    if (tables) b += (size_t)nd * 16 + (size_t)ni * 8 + (size_t)na * 12;
     // force pieces (x, y, z arrays of nf_max) and the 8-float tail that the
     // gather's rounds of eight may read past the last piece
     if (fint_lds) b += (size_t)nf_max * 12 + kPieceTail * sizeof(float);

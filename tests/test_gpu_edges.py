@@ -132,6 +132,56 @@ def test_tiny_systems_relax(ctx, surf, oracle, params):
     b.close()
 
 
+def _every_size(nmin=2, nmax=64):
+    """One system of each size nmin..nmax: a fused-ring molecule (bonds,
+    so 1-2 exclusions) topped up to the exact count with unbonded C atoms
+    3.4 A above it."""
+    from surfmol_amd import workloads as W
+    rng = np.random.Generator(np.random.PCG64(0x5EED3005))
+    mols, X = [], []
+    for n in range(nmin, nmax + 1):
+        poly = None
+        for target in (int(0.75 * n), int(0.6 * n), int(0.45 * n)):
+            if target >= 12:
+                cand = W.random_polyhex(rng, target)
+                if cand.natoms <= n:
+                    poly = cand
+                    break
+        if poly is None:
+            # small sizes: bonded C-O pairs 3.3 A apart
+            k = n // 2
+            el = ["C", "O"] * k
+            pos = np.array([[3.3 * q, 0.0, 0.0] if e == "C" else [3.3 * q + 1.13, 0.0, 0.0]
+                            for q in range(k) for e in ("C", "O")]).reshape(-1, 3)
+            poly = types.SimpleNamespace(elems=el, charges=np.tile([0.1, -0.1], k), bonds=np.array(
+                [[2 * q, 2 * q + 1] for q in range(k)], dtype=np.int32).reshape(-1, 2), natoms=2 * k, apos=pos)
+            Xp = pos + np.array([4.0, 4.0, 3.3])
+        else:
+            ang, xy, h = W.uniform_batch_poses(1, L=12.0, hmin=3.3, hmax=3.3, seed=0x5EED3006 + n)
+            Xp = W.place_on_surface(poly, ang[0], xy[0], h[0])
+        npad = n - poly.natoms
+        c = Xp.mean(axis=0)
+        side = max(1, int(np.ceil(np.sqrt(npad))))
+        g = (np.arange(side) - (side - 1) / 2) * 3.3
+        pad = np.array([[c[0] + gx, c[1] + gy, Xp[:, 2].max() + 3.4] for gx in g for gy in g][:npad]).reshape(-1, 3)
+        mols.append(_mol(list(poly.elems) + ["C"] * npad, np.concatenate([poly.charges, np.zeros(npad)]), poly.bonds))
+        X.append(np.vstack([Xp, pad]))
+    return mols, X
+
+
+def test_sweep_schedule_every_size(ctx, surf, oracle, params):
+    """The one-wave pair sweep's lane schedules (one per size up to 43 atoms,
+    the wrapping rotation above): every size from 2 to 64 atoms in the
+    64-thread shape, forces against the FP64 oracle."""
+    mols, X = _every_size()
+    assert [m.natoms for m in mols] == list(range(2, 65))
+    b = _batch(ctx, surf, mols, X)
+    b.set_launch(64, -1, 1)
+    print(b.plan())
+    _check_eval(b, mols, oracle, params, surf)
+    b.close()
+
+
 @pytest.mark.parametrize("block", [0, 512])
 @pytest.mark.parametrize("pair_sweep", [-1, 0])
 def test_max_size_system_eval(ctx, surf, oracle, params, block, pair_sweep):
