@@ -148,7 +148,8 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
     uint2 *s_dih = reinterpret_cast<uint2 *>(s_next + 4);                // packed slots
     uint2 *s_inv = s_dih + tn_d;
     uint2 *s_dihd = s_inv + tn_i;                                       // piece destinations
-    uint2 *s_angd = s_dihd + tn_d;
+    uint2 *s_invd = s_dihd + tn_d;                                      // (not ROT)
+    uint2 *s_angd = s_invd + (ROT ? 0 : tn_i);
     unsigned *s_ang = reinterpret_cast<unsigned *>(s_angd + tn_a);
     float *s_fx = reinterpret_cast<float *>(s_ang + tn_a);
     const uint2 *t_dih, *t_inv, *t_dihd, *t_invd, *t_angd;
@@ -328,9 +329,11 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
             }
             for (int q = tid; q < td.ninv; q += BLOCK) {
                 s_inv[q] = fa->tab.inv_ng[td.inv_off + q];
+                if constexpr (!ROT) s_invd[q] = fa->tab.inv_dst[td.inv_off + q];
             }
             t_ang = s_ang; t_dih = s_dih; t_inv = s_inv; t_angd = s_angd; t_dihd = s_dihd;
-            t_invd = fa->tab.inv_dst + td.inv_off;   // read through the caches
+            // one wave, one atom per thread: read through the caches
+            t_invd = ROT ? fa->tab.inv_dst + td.inv_off : s_invd;
         } else {
             t_ang = fa->tab.ang_ng + td.ang_off; t_dih = fa->tab.dih_ng + td.dih_off; t_inv = fa->tab.inv_ng + td.inv_off;
             t_angd = fa->tab.ang_dst + td.ang_off; t_dihd = fa->tab.dih_dst + td.dih_off; t_invd = fa->tab.inv_dst + td.inv_off;

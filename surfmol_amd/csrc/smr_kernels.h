@@ -151,9 +151,10 @@ inline size_t relax_lds_bytes(int block, int apt, int lds_atoms, int nf_max, int
     const int fp = relax_rot_shape(block, apt) ? 0 : (pair_mode ? nw : 1);
     size_t b = (size_t)16 * (2 * rows + (size_t)lds_atoms * (4 + fp)) + (size_t)nw * 8 * 8 + 16;
     const bool tables = apt < 4;
-    // slots (8, 8, 4 B) per dihedral, inversion, angle and piece destinations
-    // (8 B) per dihedral and angle (the inversions' are read through the caches)
-    if (tables) b += (size_t)nd * 16 + (size_t)ni * 8 + (size_t)na * 12;
+    // slots (8, 8, 4 B) and piece destinations (8 B) per dihedral, inversion,
+    // angle (one wave with one atom per thread reads the inversions'
+    // destinations through the caches: its LDS holds the sweep's rows)
+    if (tables) b += (size_t)nd * 16 + (size_t)ni * (relax_rot_shape(block, apt) ? 8 : 16) + (size_t)na * 12;
     // force pieces (x, y, z arrays of nf_max) and the 8-float tail that the
     // gather's rounds of eight may read past the last piece
     if (fint_lds) b += (size_t)nf_max * 12 + kPieceTail * sizeof(float);
