@@ -829,17 +829,41 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     pf[0] = pacc.value();
                 }
             } else if (!ROT && A.pair_mode && T.use_pairs) {
+                // Several waves: chunk c holds atoms 64c..64c+63, one per
+                // lane, and on sweep d = 1..N/2 atom i meets (i+d) mod N.
+                // The (chunk, sweep) items, q = c*(N/2) + d-1, are shared
+                // evenly by the block's waves in contiguous runs (one or a
+                // few chunks each), so that no wave idles while another
+                // sweeps a second chunk.  Each wave sums into its own LDS
+                // buffer: the reaction -f_ij into slot j as it comes, and a
+                // run's own-atom forces (compensated) into slot i at the
+                // run's end; the gather adds the waves' buffers.  The
+                // neighbour lists of the bonded-pair test are staged in the
+                // hneigh area, idle until the next step.
+                int4 *s_ngl = reinterpret_cast<int4 *>(s_h);
+#pragma unroll
+                for (int k = 0; k < APT; k++) {
+                    const int i = tid + k * BLOCK;
+                    if (i < N) s_ngl[i] = my_ng[k];
+                }
+                block_sync<NW>();
                 float4 *fpw = s_fp + wid * A.lds_atoms;
                 const int nd = N >> 1;
                 const bool even = (N & 1) == 0;
-#pragma unroll
-                for (int k = 0; k < APT; k++) {
-                    if (k * BLOCK + wid * 64 >= N) break;   // wave-uniform: no atom of this wave left
-                    const int i = tid + k * BLOCK;
+                const int total = ((N + 63) >> 6) * nd;
+                const int share = (total + NW - 1) / NW;
+                int q = min(wid * share, total);
+                const int qend = min(q + share, total);
+                while (q < qend) {   // wave-uniform
+                    const int c = q / nd;
+                    const int d_lo = q - c * nd + 1;
+                    const int d_hi = min(nd, d_lo + (qend - q) - 1);
+                    q += d_hi - d_lo + 1;
+                    const int i = c * 64 + lane;
                     const bool own = i < N;
                     const float4 pi4 = own ? s_pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
                     const float4 ri = own ? s_req[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    const int4 ng = my_ng[k];
+                    const int4 ng = own ? s_ngl[i] : make_int4(-1, -1, -1, -1);
                     KahanF3 pacc;
                     pacc.init(make_float3(0.f, 0.f, 0.f));
                     // Positions and REQ rows do not change during the sweep:
@@ -853,16 +877,16 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     // the upper half of an even N's last sweep.  The own
                     // forces of four sweeps are added in FP32 before the
                     // compensated add.
-                    int j = own ? i : 0;
-                    int jn = (j + 1 == N) ? 0 : j + 1;
+                    int jn = (own ? i : 0) + d_lo;   // first partner (d_lo < N)
+                    if (jn >= N) jn -= N;
                     float4 pj = s_pos[jn], rj = s_req[jn];
-                    for (int d0 = 1; d0 <= nd; d0 += 4) {
+                    for (int d0 = d_lo; d0 <= d_hi; d0 += 4) {
                         float3 bs = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
                             const int d = d0 + u;
-                            if (d > nd) break;
-                            j = jn;
+                            if (d > d_hi) break;
+                            const int j = jn;
                             const float4 pc = pj, rc = rj;
                             jn = (j + 1 == N) ? 0 : j + 1;
                             pj = s_pos[jn];
@@ -882,7 +906,12 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                         }
                         pacc.add(bs);
                     }
-                    pf[k] = pacc.value();
+                    if (own) {
+                        const float3 v = pacc.value();
+                        float4 fi = fpw[i];
+                        fi.x += v.x; fi.y += v.y; fi.z += v.z;
+                        fpw[i] = fi;
+                    }
                 }
                 block_sync<NW>();
             }
@@ -1003,8 +1032,8 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     KahanF3 acc;
                     acc.init(fb[k]);
                     if (A.pair_mode && T.use_pairs) {
-                        acc.add(pf[k]);
                         if constexpr (ROT) {
+                            acc.add(pf[k]);
                             acc.add(rr);
                         } else {
                             for (int w = 0; w < NW; w++) acc.add(f3(s_fp[w * A.lds_atoms + i]));
