@@ -102,3 +102,52 @@ def test_full_grid_forces_and_relaxation(ctx, full, oracle, params):
         assert abs(eo.sum() - r["energy"]) / mol.natoms < 1e-6
         rf = oracle.relax_fire(osys, oracle.FireParams(**fire_params_internal()), 3000, 1e-3, apos=Xf[sl])
         assert rf["converged"] and abs(rf["E"] - r["energy"]) / mol.natoms < 1e-6, (s, rf["E"], r["energy"])
+
+
+def test_full_uniform_batch(ctx, full, oracle, params):
+    """The uniform batch at its full size: 4 096 replicas on the 240 x 240 x
+    200 grid, FIRE to Fmax < 1e-3 eV/A within 2 000 steps, the default
+    schedule (energy order, re-ranked after 400 steps).  Properties that hold
+    at any size: every system converged or stopped at the cap, none
+    non-finite; a repeated relaxation, the batch order in one launch and a
+    sample of systems relaxed in a batch of their own give the same bits; the
+    sample's reported energies are the FP64 energies of its device positions
+    and FP64 FIRE from there moves the energy by < 1e-6 eV/atom."""
+    import surfmol_amd as S
+    nsys, max_steps = 4096, 2000
+    mol, X = ptcda_replicas(nsys, full["spec"]["L"])
+    b = _batch(ctx, full, mol, X)
+    fp = S.FireParams(**fire_params_internal())
+    res = b.relax_batched(fp, max_steps, 1e-3)
+    Xf = b.positions().copy()
+    st = res["status"]
+    assert set(np.unique(st)) <= {S.SMR_SYS_CONVERGED, S.SMR_SYS_MAXSTEPS}, np.unique(st)
+    conv = st == S.SMR_SYS_CONVERGED
+    assert conv.sum() >= nsys - 16, conv.sum()
+    assert (res["fmax"][conv] < 1e-3).all()
+    assert (res["steps"][~conv] == max_steps).all()
+    assert np.isfinite(res["energy"]).all()
+    print(f"converged {conv.sum()} of {nsys}, steps mean {res['steps'].mean():.1f} max {res['steps'].max()}")
+    # repeated, and in the batch order in one launch
+    b.reset()
+    res2 = b.relax_batched(fp, max_steps, 1e-3)
+    assert np.array_equal(b.positions(), Xf) and np.array_equal(res2["steps"], res["steps"])
+    b.set_order(False)
+    b.set_rerank(0)
+    b.reset()
+    res3 = b.relax_batched(fp, max_steps, 1e-3)
+    assert np.array_equal(b.positions(), Xf) and np.array_equal(res3["energy"], res["energy"])
+    # a sample alone: the longest relaxations and a spread of others
+    sample = sorted(set(np.argsort(-res["steps"])[:3].tolist() + list(range(5, nsys, 821))))
+    bs = _batch(ctx, full, mol, [X[s] for s in sample])
+    rs = bs.relax_batched(fp, max_steps, 1e-3)
+    Xs = bs.positions()
+    for k, s in enumerate(sample):
+        sl, slb = slice(b.offsets[s], b.offsets[s + 1]), slice(bs.offsets[k], bs.offsets[k + 1])
+        assert np.array_equal(Xs[slb], Xf[sl]) and rs["steps"][k] == res["steps"][s], s
+        osys = oracle_system(oracle, params, mol, X[s], full["C"], full["spec"])
+        fo, eo = oracle.eval_forces(osys, Xf[sl])
+        assert abs(eo.sum() - res["energy"][s]) / mol.natoms < 1e-6, (s, eo.sum(), res["energy"][s])
+        if conv[s]:
+            rf = oracle.relax_fire(osys, oracle.FireParams(**fire_params_internal()), 3000, 1e-3, apos=Xf[sl])
+            assert rf["converged"] and abs(rf["E"] - res["energy"][s]) / mol.natoms < 1e-6, (s, rf["E"])
