@@ -153,9 +153,11 @@ __global__ __launch_bounds__(256) void k_energy64(DevTables T, const float4 *__r
         }
         e[4] += 2.0 * acc;
     }
-    // surface: E = sum_stencil Bx By Bz (PLQH . C)
+    // surface: E = sum_stencil Bx By Bz (PLQH . C); the block's threads
+    // share the (atom, stencil column) items, 16 per atom
     if (T.grid) {
-        for (int i = tid; i < N; i += blockDim.x) {
+        for (int it = tid; it < 16 * N; it += blockDim.x) {
+            const int i = it >> 4, a = (it >> 2) & 3, b = it & 3;
             const d3 p = ld(P, i);
             const double ux = p.x * T.ginv64_x, uy = p.y * T.ginv64_y, uz = p.z * T.ginv64_z;
             const double fx = floor(ux), fy = floor(uy);
@@ -168,24 +170,16 @@ __global__ __launch_bounds__(256) void k_energy64(DevTables T, const float4 *__r
             int ix = sd.iux + (int)fx - 1, iy = sd.iuy + (int)fy - 1;
             ix %= T.gnx; if (ix < 0) ix += T.gnx;
             iy %= T.gny; if (iy < 0) iy += T.gny;
+            int xi = ix + a; if (xi >= T.gnx) xi -= T.gnx;
+            int yi = iy + b; if (yi >= T.gny) yi -= T.gny;
+            const float4 *col = T.grid + ((size_t)(xi * T.gny + yi) * T.gnz + (iz - 1));
             const double4 w = T.plq64[td.atom_off + i];
-            double ex = 0.0;
-            for (int a = 0; a < 4; a++) {
-                int xi = ix + a; if (xi >= T.gnx) xi -= T.gnx;
-                double ey = 0.0;
-                for (int b = 0; b < 4; b++) {
-                    int yi = iy + b; if (yi >= T.gny) yi -= T.gny;
-                    const float4 *col = T.grid + ((size_t)(xi * T.gny + yi) * T.gnz + (iz - 1));
-                    double ez = 0.0;
-                    for (int c = 0; c < 4; c++) {
-                        const float4 C = col[c];
-                        ez += bz[c] * (w.x * (double)C.x + w.y * (double)C.y + w.z * (double)C.z);
-                    }
-                    ey += by[b] * ez;
-                }
-                ex += bx[a] * ey;
+            double ez = 0.0;
+            for (int c = 0; c < 4; c++) {
+                const float4 C = col[c];
+                ez += bz[c] * (w.x * (double)C.x + w.y * (double)C.y + w.z * (double)C.z);
             }
-            e[5] += ex;
+            e[5] += bx[a] * by[b] * ez;
         }
     }
     // block reduction (wave shuffles, then waves in order)
