@@ -862,8 +862,24 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     const int i = c * 64 + lane;
                     const bool own = i < N;
                     const float4 pi4 = own ? s_pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    const float4 ri = own ? s_req[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    // the REQ row with the radial factor's constants folded in
+                    // (-12 on sqrtE, the Coulomb sign)
+                    const float4 ri0 = own ? s_req[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    const float4 ri = make_float4(ri0.x, -12.f * ri0.y, -ri0.z, 0.f);
                     const int4 ng = own ? s_ngl[i] : make_int4(-1, -1, -1, -1);
+                    // the sweeps this lane drops: its bonded partners on its
+                    // own side (sweep (ng - i) mod N), the upper half of an
+                    // even N's last sweep, and every sweep of a lane without
+                    // an atom; a 64-bit mask per window of 64 sweeps
+                    const int ngs[4] = {ng.x, ng.y, ng.z, ng.w};
+                    int doff[5];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        int o = ngs[k] - i;
+                        if (o < 0) o += N;
+                        doff[k] = ngs[k] >= 0 ? o : -1;
+                    }
+                    doff[4] = (even && i >= nd) ? nd : -1;
                     KahanF3 pacc;
                     pacc.init(make_float3(0.f, 0.f, 0.f));
                     // Positions and REQ rows do not change during the sweep:
@@ -872,39 +888,44 @@ __global__ __launch_bounds__(BLOCK, (APT == 1 && !PERSIST) ? 2 : 1) void k_relax
                     // bounds).  Within a sweep the partners are distinct, so
                     // the reaction slot is read at the top of the iteration
                     // (after the previous sweep's store) and its latency
-                    // overlaps the pair.  Branch-free: the radial factor is
-                    // zeroed for lanes without an atom, bonded partners and
-                    // the upper half of an even N's last sweep.  The own
-                    // forces of four sweeps are added in FP32 before the
-                    // compensated add.
+                    // overlaps the pair.  Branch-free: the mask zeroes the
+                    // radial factor of dropped sweeps.  The own forces of four
+                    // sweeps are added in FP32 before the compensated add.
                     int jn = (own ? i : 0) + d_lo;   // first partner (d_lo < N)
                     if (jn >= N) jn -= N;
                     float4 pj = s_pos[jn], rj = s_req[jn];
-                    for (int d0 = d_lo; d0 <= d_hi; d0 += 4) {
-                        float3 bs = make_float3(0.f, 0.f, 0.f);
+                    for (int w0 = d_lo; w0 <= d_hi; w0 += 64) {
+                        const int w1 = min(d_hi, w0 + 63);
+                        unsigned long long m = own ? 0ull : ~0ull;
 #pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            const int d = d0 + u;
-                            if (d > d_hi) break;
-                            const int j = jn;
-                            const float4 pc = pj, rc = rj;
-                            jn = (j + 1 == N) ? 0 : j + 1;
-                            pj = s_pos[jn];
-                            rj = s_req[jn];
-                            float4 fj = fpw[j];
-                            const float3 dp = f3(pc) - f3(pi4);
-                            float fr = ljq_pair_fr(dp, ri, rc, T.r2damp);
-                            const bool skip = !own || (even && d == nd && i >= nd) || j == ng.x || j == ng.y ||
-                                              j == ng.z || j == ng.w;
-                            if (skip) fr = 0.f;
-                            const float3 f = dp * fr;
-                            bs = bs + f;
-                            if (own) {
-                                fj.x -= f.x; fj.y -= f.y; fj.z -= f.z;
-                                fpw[j] = fj;
-                            }
+                        for (int k = 0; k < 5; k++) {
+                            const unsigned t = (unsigned)(doff[k] - w0);
+                            if (t < 64u) m |= 1ull << t;
                         }
-                        pacc.add(bs);
+                        for (int d0 = w0; d0 <= w1; d0 += 4) {
+                            const unsigned mb = (unsigned)(m >> (d0 - w0));
+                            float3 bs = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                if (d0 + u > w1) break;
+                                const int j = jn;
+                                const float4 pc = pj, rc = rj;
+                                jn = (j + 1 == N) ? 0 : j + 1;
+                                pj = s_pos[jn];
+                                rj = s_req[jn];
+                                float4 fj = fpw[j];
+                                const float3 dp = f3(pc) - f3(pi4);
+                                float fr = ljq_pair_fr_folded(dp, ri, rc, T.r2damp);
+                                if ((mb >> u) & 1u) fr = 0.f;
+                                const float3 f = dp * fr;
+                                bs = bs + f;
+                                if (own) {
+                                    fj.x -= f.x; fj.y -= f.y; fj.z -= f.z;
+                                    fpw[j] = fj;
+                                }
+                            }
+                            pacc.add(bs);
+                        }
                     }
                     if (own) {
                         const float3 v = pacc.value();
