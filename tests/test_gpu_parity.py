@@ -220,3 +220,25 @@ def test_energy_order_changes_no_result(ctx, surf):
     for o in out[1:]:
         assert o[0] == out[0][0]
         assert np.array_equal(o[1], out[0][1])
+
+
+def test_tiled_grid_same_bits(ctx, surf):
+    """The surface is periodic with the cell, so the cell's coefficients tiled
+    2 x 2 describe the same potential on a grid twice as wide: a batch relaxes
+    to the same bits on either (each system's integer node offset wraps with
+    the grid's own width)."""
+    import surfmol_amd as S
+    spec = surf["spec"]
+    C = surf["grid"].coefficients()
+    gs2 = S.GridSpec(2 * spec["nx"], 2 * spec["ny"], spec["nz"], spec["p0"], spec["step"], ALPHA, RDAMP)
+    g2 = S.GridFF.upload(ctx, gs2, np.tile(C, (2, 2, 1, 1)))
+    mol, X = ptcda_replicas(16, 2 * spec["L"])
+    fp = S.FireParams(**fire_params_internal())
+    out = []
+    for g in (surf["grid"], g2):
+        hm = S.Molecule.build(mol.elems, X[0], mol.charges, bonds=mol.bonds)
+        b = S.Batch(ctx, [hm], np.zeros(len(X), dtype=np.int32), np.concatenate(X), grid=g, rdamp=RDAMP)
+        r = b.relax_batched(fp, 1500, 1e-3)
+        out.append((b.positions().copy(), r["steps"].copy(), r["energy"].copy()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
